@@ -4,7 +4,7 @@
  * Plain pointers and sizes only.  Every entry point returns 0 on success or a
  * negative MPCD_E* code; mpcd_gpu_last_error() then returns the message the
  * reference would have thrown as std::runtime_error (the C++ layer in
- * mpcd/*.h rethrows it).  Particle arrays at this boundary use the
+ * include/mpcd/ rethrows it).  Particle arrays at this boundary use the
  * reference's interleaved layout [x1,y1,x2,y2,...] in particle-id order
  * (types.h:74-79); inside the library the state is FP64 structure-of-arrays.
  *
@@ -107,6 +107,89 @@ typedef struct mpcd_cell_fields {
     int32_t* N;
     double *Ux, *Uy, *Px, *Py, *T, *rho, *Pkin, *Pvir, *P, *rhoTarget;
 } mpcd_cell_fields;
+
+/* The closure grids of LiquidClosureResult (liquid_closure.h:36-42); any
+ * pointer may be NULL. */
+typedef struct mpcd_closure_grids {
+    double *pdriveRaw, *pdrive, *duRepairX, *duRepairY, *duEOSX, *duEOSY;
+    int32_t* interzoneMask;
+} mpcd_closure_grids;
+
+/* Metrics of one resident time step. */
+typedef struct mpcd_step_metrics {
+    mpcd_wall_info wall;
+    mpcd_zone_pass_metrics base;
+    mpcd_zone_pass_metrics shifted;
+    mpcd_closure_metrics closure;
+} mpcd_step_metrics;
+
+/* Opaque device-resident simulation (state, binnings, layouts). */
+typedef struct mpcd_gpu_sim mpcd_gpu_sim;
+
+/* ---- errors / device ---- */
+const char* mpcd_gpu_last_error(void);
+int mpcd_gpu_device_count(int* count);
+
+/* ---- stateless stage calls (host buffers in, host buffers out) ---- */
+
+/* replaces mpcd::apply_bc_general (boundary_conditions.h:9-13) */
+int mpcd_gpu_apply_bc_general(const mpcd_params* p, double* x, double* v, uint64_t rng_seed, mpcd_wall_info* info);
+
+/* body force + streaming + apply_bc_general in one pass: the driver loop
+ * main_benchmark_poiseuille_openmp_profiled_shifted_profiled.cpp:242-249
+ * (do_stream = 0 gives apply_bc_general alone) */
+int mpcd_gpu_force_stream_bc(const mpcd_params* p, double* x, double* v, uint64_t rng_seed, int do_stream,
+                             mpcd_wall_info* info);
+
+/* replaces mpcd::srd_cell_id_with_random_shift (srd_collision.h:9-11) */
+int mpcd_gpu_srd_cell_id_with_random_shift(const mpcd_params* p, const double* x, uint64_t rng_seed, int32_t* cid);
+
+/* replaces mpcd::srd_collision_step (srd_collision.h:12-15) */
+int mpcd_gpu_srd_collision_step(const mpcd_params* p, double* v, const int32_t* cid, uint64_t rng_seed);
+
+/* replaces mpcd::compute_cell_fields (common_grid.h:10-13) */
+int mpcd_gpu_compute_cell_fields(const mpcd_params* p, const double* x, const double* v, double rhoTarget,
+                                 mpcd_cell_fields* out);
+
+/* replaces mpcd::build_zone_layout (zone_pass.h:60); rects4 (4 per zone:
+ * xs, xe, ys, ye) may be NULL */
+int mpcd_gpu_build_zone_layout(const mpcd_params* p, int mode, int32_t* ownerMap, int32_t* nZones, int32_t* rects4);
+
+/* replaces mpcd::run_zone_kernel_inplace (zone_pass.h:62-66) for a
+ * rectangular zone [xs,xe) x [ys,ye); stats6 = ZoneKernelStats::stats */
+int mpcd_gpu_run_zone_kernel(const mpcd_params* p, double* x, double* v, int xs, int xe, int ys, int ye,
+                             uint64_t rng_seed, double* stats6, int32_t* moved2);
+
+/* replaces mpcd::run_zone_pass_openmp / run_zone_pass_sync_serial
+ * (zone_pass_openmp.h:35-53): snapshot semantics, zones on the GPU */
+int mpcd_gpu_zone_pass(const mpcd_params* p, double* x, double* v, int layoutMode, uint64_t rngSeedBase,
+                       mpcd_zone_pass_metrics* m, int32_t* ownerMap);
+
+/* replaces mpcd::run_liquid_closure (liquid_closure.h:46-48) */
+int mpcd_gpu_liquid_closure(const mpcd_params* p, const double* xRef, const double* vRef, const double* xRed,
+                            const double* vRed, double* xOut, double* vOut, mpcd_closure_metrics* m,
+                            mpcd_closure_grids* grids);
+
+/* ---- resident stepping: the driver loop body
+ * (main_benchmark_poiseuille_openmp_profiled_shifted_profiled.cpp:240-279)
+ * with the state kept in HBM between steps ---- */
+int mpcd_gpu_sim_create(const mpcd_params* p, mpcd_gpu_sim** out);
+int mpcd_gpu_sim_destroy(mpcd_gpu_sim* s);
+int mpcd_gpu_sim_set_state(mpcd_gpu_sim* s, const double* x, const double* v);
+int mpcd_gpu_sim_get_state(mpcd_gpu_sim* s, double* x, double* v);
+/* steps first_step .. first_step+n_steps-1 with the driver's per-step seeds;
+ * `last` (nullable) receives the last step's metrics */
+int mpcd_gpu_sim_step(mpcd_gpu_sim* s, int first_step, int n_steps, mpcd_step_metrics* last);
+/* same, enqueued without a host wait; mpcd_gpu_sim_check then waits and
+ * reports a wall-mode error raised by any enqueued step */
+int mpcd_gpu_sim_step_async(mpcd_gpu_sim* s, int first_step, int n_steps);
+int mpcd_gpu_sim_check(mpcd_gpu_sim* s);
+/* one step that also returns the ref / after-base / red states */
+int mpcd_gpu_sim_step_stages(mpcd_gpu_sim* s, int step, double* x_ref, double* v_ref, double* x_base, double* v_base,
+                             double* x_red, double* v_red, mpcd_step_metrics* m);
+int mpcd_gpu_sim_synchronize(mpcd_gpu_sim* s);
+/* the hipStream_t the simulation launches on */
+int mpcd_gpu_sim_stream(mpcd_gpu_sim* s, void** stream);
 
 #ifdef __cplusplus
 }

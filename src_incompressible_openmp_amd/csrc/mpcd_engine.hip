@@ -1,0 +1,1393 @@
+/*
+ * mpcd_engine.hip -- gfx950 kernels and device orchestration of the SRD/MPCD
+ * step, and the C ABI declared in include/mpcd_gpu.h.
+ *
+ * Device state is FP64 structure-of-arrays (x, y, vx, vy) indexed by slot,
+ * with pid[slot] the reference particle index (slot == pid as uploaded).
+ * Every per-cell stage reads the particles of a cell through a counting-sort
+ * binning (cnt, start, list) whose per-cell runs are in ascending pid, so the
+ * per-cell sums run in the reference's order.
+ */
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "mpcd_core.h"
+#include "mpcd_host.h"
+
+using namespace mpcdk;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+struct HipError : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+
+#define HIP_OK(expr)                                                                                  \
+    do {                                                                                              \
+        hipError_t e_ = (expr);                                                                       \
+        if (e_ != hipSuccess)                                                                         \
+            throw HipError(std::string(#expr) + ": " + hipGetErrorString(e_));                        \
+    } while (0)
+
+constexpr int TPB = 256;
+constexpr int ZONE_TPB = 64;
+constexpr int ZONE_LDS_CAP = 64 * 1024;
+constexpr int SCAN_TILE = 1024; /* 256 threads x 4 */
+
+inline unsigned nblk(long long n, int tpb = TPB) { return (unsigned)((n + tpb - 1) / tpb); }
+
+/* ===================================================================== */
+/*                               kernels                                 */
+/* ===================================================================== */
+
+__global__ void k_deinterleave(const double* __restrict__ a, int n, double* __restrict__ x, double* __restrict__ y) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        x[i] = a[2 * (size_t)i];
+        y[i] = a[2 * (size_t)i + 1];
+    }
+}
+
+/* gather by pid: out[2*pid] = x[slot] */
+__global__ void k_interleave(const double* __restrict__ x, const double* __restrict__ y, const int* __restrict__ pid,
+                             int n, double* __restrict__ a) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s < n) {
+        const size_t p = (size_t)pid[s];
+        a[2 * p] = x[s];
+        a[2 * p + 1] = y[s];
+    }
+}
+
+__global__ void k_iota(int* a, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) a[i] = i;
+}
+
+__global__ void k_fill_int(int* a, int n, int v) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) a[i] = v;
+}
+
+/* Block reduction of a WallAcc into one partial per block (fixed tree, so the
+ * result does not depend on scheduling). */
+__device__ void wall_block_reduce(WallAcc a, WallAcc* out) {
+    __shared__ double sd[13][TPB];
+    const int t = threadIdx.x;
+    sd[0][t] = (double)a.nBot;
+    sd[1][t] = (double)a.nTop;
+    sd[2][t] = (double)a.nLeft;
+    sd[3][t] = (double)a.nRight;
+    sd[4][t] = a.dE;
+    sd[5][t] = a.dPxBot;
+    sd[6][t] = a.dPxTop;
+    sd[7][t] = a.dPyBot;
+    sd[8][t] = a.dPyTop;
+    sd[9][t] = a.dPxLeft;
+    sd[10][t] = a.dPxRight;
+    sd[11][t] = a.dPyLeft;
+    sd[12][t] = a.dPyRight;
+    __syncthreads();
+    for (int s = TPB / 2; s > 0; s >>= 1) {
+        if (t < s)
+            for (int k = 0; k < 13; ++k) sd[k][t] += sd[k][t + s];
+        __syncthreads();
+    }
+    if (t == 0) {
+        WallAcc r;
+        r.nBot = (int)sd[0][0];
+        r.nTop = (int)sd[1][0];
+        r.nLeft = (int)sd[2][0];
+        r.nRight = (int)sd[3][0];
+        r.dE = sd[4][0];
+        r.dPxBot = sd[5][0];
+        r.dPxTop = sd[6][0];
+        r.dPyBot = sd[7][0];
+        r.dPyTop = sd[8][0];
+        r.dPxLeft = sd[9][0];
+        r.dPxRight = sd[10][0];
+        r.dPyLeft = sd[11][0];
+        r.dPyRight = sd[12][0];
+        out[blockIdx.x] = r;
+    }
+}
+
+/* K1: force + stream + wall BC (+ shifted collision cell). */
+__global__ void __launch_bounds__(TPB) k_stream_bc(Geom g, BCParams b, uint64_t seed, int do_stream, double* __restrict__ x,
+                                                   double* __restrict__ y, double* __restrict__ vx,
+                                                   double* __restrict__ vy, const int* __restrict__ pid,
+                                                   int* __restrict__ cid, double sx, double sy, double xmax,
+                                                   double ymax, WallAcc* __restrict__ wall, int* __restrict__ err) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    WallAcc acc;
+    wall_acc_zero(acc);
+    if (i < g.n) {
+        double px = x[i], py = y[i], pvx = vx[i], pvy = vy[i];
+        const int rc = bc_particle(g, b, seed, (uint32_t)pid[i], do_stream, px, py, pvx, pvy, acc);
+        if (rc != 0) atomicOr(err, 1);
+        x[i] = px;
+        y[i] = py;
+        vx[i] = pvx;
+        vy[i] = pvy;
+        if (cid) cid[i] = shifted_cell_of(g, px, py, sx, sy, xmax, ymax);
+    }
+    if (wall) wall_block_reduce(acc, wall);
+}
+
+__global__ void k_shift_cells(Geom g, const double* __restrict__ x, const double* __restrict__ y, double sx,
+                              double sy, double xmax, double ymax, int* __restrict__ cid) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < g.n) cid[i] = shifted_cell_of(g, x[i], y[i], sx, sy, xmax, ymax);
+}
+
+__global__ void k_cells(Geom g, const double* __restrict__ x, const double* __restrict__ y, int* __restrict__ cid) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < g.n) cid[i] = cell_of(g, x[i], y[i]);
+}
+
+/* ---- counting sort ---- */
+
+__global__ void k_hist(const int* __restrict__ cid, int n, int* __restrict__ cnt) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) atomicAdd(&cnt[cid[i]], 1);
+}
+
+/* exclusive scan, level 1: each block scans SCAN_TILE entries */
+__global__ void __launch_bounds__(TPB) k_scan_tiles(const int* __restrict__ in, int m, int* __restrict__ out,
+                                                    int* __restrict__ tileSum) {
+    __shared__ int s[TPB];
+    const int base = blockIdx.x * SCAN_TILE + threadIdx.x * 4;
+    int v[4];
+    int tot = 0;
+    for (int k = 0; k < 4; ++k) {
+        v[k] = (base + k < m) ? in[base + k] : 0;
+        tot += v[k];
+    }
+    s[threadIdx.x] = tot;
+    __syncthreads();
+    for (int off = 1; off < TPB; off <<= 1) {
+        const int t = (threadIdx.x >= (unsigned)off) ? s[threadIdx.x - off] : 0;
+        __syncthreads();
+        s[threadIdx.x] += t;
+        __syncthreads();
+    }
+    int run = s[threadIdx.x] - tot;
+    for (int k = 0; k < 4; ++k) {
+        if (base + k < m) out[base + k] = run;
+        run += v[k];
+    }
+    if (threadIdx.x == TPB - 1) tileSum[blockIdx.x] = s[TPB - 1];
+}
+
+/* level 2: one block scans the tile sums (sequential chunks per thread) */
+__global__ void __launch_bounds__(TPB) k_scan_top(int* __restrict__ tileSum, int nt) {
+    __shared__ int s[TPB];
+    const int per = (nt + TPB - 1) / TPB;
+    const int b0 = threadIdx.x * per;
+    int tot = 0;
+    for (int k = 0; k < per; ++k)
+        if (b0 + k < nt) tot += tileSum[b0 + k];
+    s[threadIdx.x] = tot;
+    __syncthreads();
+    for (int off = 1; off < TPB; off <<= 1) {
+        const int t = (threadIdx.x >= (unsigned)off) ? s[threadIdx.x - off] : 0;
+        __syncthreads();
+        s[threadIdx.x] += t;
+        __syncthreads();
+    }
+    int run = s[threadIdx.x] - tot;
+    for (int k = 0; k < per; ++k)
+        if (b0 + k < nt) {
+            const int v = tileSum[b0 + k];
+            tileSum[b0 + k] = run;
+            run += v;
+        }
+}
+
+__global__ void k_scan_add(int* __restrict__ out, int m, const int* __restrict__ tileSum) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < m) out[i] += tileSum[i / SCAN_TILE];
+}
+
+__global__ void k_scatter(const int* __restrict__ cid, int n, const int* __restrict__ start, int* __restrict__ cursor,
+                          int* __restrict__ list) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        const int c = cid[i];
+        list[start[c] + atomicAdd(&cursor[c], 1)] = i;
+    }
+}
+
+/* order each cell's run by pid (runs are short: insertion sort) */
+__global__ void k_segsort(const int* __restrict__ start, int Nc, int* __restrict__ list, const int* __restrict__ pid) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= Nc) return;
+    const int b = start[c], e = start[c + 1];
+    for (int i = b + 1; i < e; ++i) {
+        const int key = list[i];
+        const int kp = pid[key];
+        int j = i - 1;
+        while (j >= b && pid[list[j]] > kp) {
+            list[j + 1] = list[j];
+            --j;
+        }
+        list[j + 1] = key;
+    }
+}
+
+/* ---- collision ---- */
+
+__global__ void k_collide(CollideParams cp, uint64_t seed, const int* __restrict__ start,
+                          const int* __restrict__ list, int Nc, double* __restrict__ vx, double* __restrict__ vy) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= Nc) return;
+    const int b = start[c];
+    collide_cell(cp, seed, c, list + b, start[c + 1] - b, vx, vy);
+}
+
+/* deterministic block partial sums of (vx, vy) */
+__global__ void __launch_bounds__(TPB) k_sum2(const double* __restrict__ a, const double* __restrict__ b, int n,
+                                              double* __restrict__ part) {
+    __shared__ double s0[TPB], s1[TPB];
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    s0[threadIdx.x] = (i < n) ? a[i] : 0.0;
+    s1[threadIdx.x] = (i < n) ? b[i] : 0.0;
+    __syncthreads();
+    for (int s = TPB / 2; s > 0; s >>= 1) {
+        if (threadIdx.x < (unsigned)s) {
+            s0[threadIdx.x] += s0[threadIdx.x + s];
+            s1[threadIdx.x] += s1[threadIdx.x + s];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        part[2 * blockIdx.x] = s0[0];
+        part[2 * blockIdx.x + 1] = s1[0];
+    }
+}
+
+/* single-block reduction of `np` partial tuples of width w into out[0..w) */
+__global__ void __launch_bounds__(TPB) k_reduce_parts(const double* __restrict__ part, int np, int w,
+                                                      double* __restrict__ out) {
+    __shared__ double s[TPB];
+    for (int k = 0; k < w; ++k) {
+        double acc = 0.0;
+        for (int j = threadIdx.x; j < np; j += TPB) acc += part[(size_t)j * w + k];
+        s[threadIdx.x] = acc;
+        __syncthreads();
+        for (int st = TPB / 2; st > 0; st >>= 1) {
+            if (threadIdx.x < (unsigned)st) s[threadIdx.x] += s[threadIdx.x + st];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) out[k] = s[0];
+        __syncthreads();
+    }
+}
+
+__global__ void k_sub_mean(double* __restrict__ vx, double* __restrict__ vy, int n, const double* __restrict__ sums) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        const double mx = sums[0] / (double)n, my = sums[1] / (double)n;
+        vx[i] -= mx;
+        vy[i] -= my;
+    }
+}
+
+/* ---- cell fields ---- */
+
+struct FieldsOut {
+    int* N;
+    double *Ux, *Uy, *Px, *Py, *T, *rho, *Pkin, *Pvir, *P;
+};
+
+__global__ void k_cell_fields(const int* __restrict__ start, const int* __restrict__ list, int Nc,
+                              const double* __restrict__ vx, const double* __restrict__ vy, double Vc, double K,
+                              double rhoT, FieldsOut f) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= Nc) return;
+    const int b = start[c];
+    const CellStats s = cell_stats(list + b, start[c + 1] - b, vx, vy, Vc, K, rhoT);
+    if (f.N) f.N[c] = s.N;
+    if (f.Ux) f.Ux[c] = s.Ux;
+    if (f.Uy) f.Uy[c] = s.Uy;
+    if (f.Px) f.Px[c] = s.Px;
+    if (f.Py) f.Py[c] = s.Py;
+    if (f.T) f.T[c] = s.T;
+    if (f.rho) f.rho[c] = s.rho;
+    if (f.Pkin) f.Pkin[c] = s.Pkin;
+    if (f.Pvir) f.Pvir[c] = s.Pvir;
+    if (f.P) f.P[c] = s.P;
+}
+
+/* ---- zone pass: one workgroup (one wavefront) per zone ---- */
+
+struct ZoneArgs {
+    Geom g;
+    ZoneParams zp;
+    const ZoneRect* rects;
+    int nZones;
+    uint64_t seedBase;
+    int singleSeed; /* 1: seedBase is the zone's own seed (single-zone call) */
+    const int* gcnt;
+    const int* gstart;
+    const int* glist;
+    const int* gpid;
+    const int* gcid;
+    const int* owner; /* nullptr: every zone owns all its particles */
+    const int* zoff;  /* per-zone scratch offsets (exclusive scan of sizes) */
+    long long scratchCap;
+    double *x, *y;
+    const double *vxin, *vyin;
+    double *vxout, *vyout;
+    int *next, *tmpg, *slotOf;
+    ZoneOut* out;
+    int* err;
+    unsigned char* gscratch; /* used instead of LDS when the zone arrays do not fit */
+    size_t gstride;
+};
+
+/* particles of each zone: its columns' runs of the binning list */
+__global__ void k_zone_sizes(const ZoneRect* __restrict__ rects, int nZ, const int* __restrict__ start, int Ny,
+                             int* __restrict__ sizes) {
+    const int zi = blockIdx.x * blockDim.x + threadIdx.x;
+    if (zi > nZ) return;
+    if (zi == nZ) {
+        sizes[zi] = 0;
+        return;
+    }
+    const ZoneRect r = rects[zi];
+    int P = 0;
+    for (int gx = r.xs; gx < r.xe; ++gx) P += start[(r.ye - 1 + Ny * gx) + 1] - start[r.ys + Ny * gx];
+    sizes[zi] = P;
+}
+
+__global__ void __launch_bounds__(ZONE_TPB) k_zone(ZoneArgs a) {
+    extern __shared__ __align__(16) unsigned char lds[];
+    const int zi = blockIdx.x;
+    if (zi >= a.nZones) return;
+    const ZoneRect r = a.rects[zi];
+    __shared__ ZoneCtx z;
+    __shared__ int ok;
+    if (threadIdx.x == 0) {
+        z.g = a.g;
+        z.zp = a.zp;
+        z.xs = r.xs;
+        z.xe = r.xe;
+        z.ys = r.ys;
+        z.ye = r.ye;
+        z.zw = r.xe - r.xs;
+        z.zh = r.ye - r.ys;
+        z.C = z.zw * z.zh;
+        z.zid = zi + 1;
+        z.off = a.zoff ? a.zoff[zi] : 0;
+        z.seed = a.singleSeed ? a.seedBase : a.seedBase + 104729ULL * (uint64_t)(zi + 1);
+        z.gcnt = a.gcnt;
+        z.gstart = a.gstart;
+        z.glist = a.glist;
+        z.gpid = a.gpid;
+        z.gcid = a.gcid;
+        z.owner = a.owner;
+        z.x = a.x;
+        z.y = a.y;
+        z.vxin = a.vxin;
+        z.vyin = a.vyin;
+        z.vxout = a.vxout;
+        z.vyout = a.vyout;
+        z.next = a.next;
+        z.tmpg = a.tmpg;
+        z.slotOf = a.slotOf;
+        unsigned char* base = a.gscratch ? a.gscratch + (size_t)zi * a.gstride : lds;
+        zone_carve(z, base, z.C, z.zw);
+        zone_setup_serial(z);
+        ok = ((long long)z.off + (long long)z.P <= a.scratchCap) ? 1 : 0;
+        if (!ok) atomicOr(a.err, 2);
+    }
+    __syncthreads();
+    if (!ok) return; /* uniform across the workgroup */
+    for (int lc = threadIdx.x; lc < z.C; lc += blockDim.x) zone_setup_cell(z, lc);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        zone_dense_passes(z);
+        zone_sparse_pass(z);
+    }
+    __syncthreads();
+    for (int lc = threadIdx.x; lc < z.C; lc += blockDim.x) zone_correct_cell(z, lc);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        zone_final_stats(z);
+        ZoneOut o;
+        o.nHigh = z.nHigh;
+        o.nLow = z.nLow;
+        o.movedDense = z.movedDense;
+        o.movedSparse = z.movedSparse;
+        o.stats4 = z.stats4;
+        o.stats5 = z.stats5;
+        a.out[zi] = o;
+    }
+}
+
+/* ---- closure (liquid_closure.cpp:340-436) ---- */
+
+struct ClosureArgs {
+    int Nc;
+    double gamma;
+    /* ref / red cell sums */
+    const double *refPx, *refPy, *refUx, *refUy;
+    const double *redPx, *redPy, *redUx, *redUy;
+    double *duRX, *duRY;
+    double* part; /* per block: num, den */
+};
+
+__global__ void __launch_bounds__(TPB) k_closure_repair(ClosureArgs a) {
+    __shared__ double s0[TPB], s1[TPB];
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    double num = 0.0, den = 0.0;
+    if (c < a.Nc) {
+        const double ax = (a.refPx[c] - a.redPx[c]) / a.gamma;
+        const double ay = (a.refPy[c] - a.redPy[c]) / a.gamma;
+        a.duRX[c] = ax;
+        a.duRY[c] = ay;
+        const double ex = a.redUx[c] - a.refUx[c], ey = a.redUy[c] - a.refUy[c];
+        num = -(ex * ax + ey * ay);
+        den = ax * ax + ay * ay;
+    }
+    s0[threadIdx.x] = num;
+    s1[threadIdx.x] = den;
+    __syncthreads();
+    for (int s = TPB / 2; s > 0; s >>= 1) {
+        if (threadIdx.x < (unsigned)s) {
+            s0[threadIdx.x] += s0[threadIdx.x + s];
+            s1[threadIdx.x] += s1[threadIdx.x + s];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        a.part[2 * blockIdx.x] = s0[0];
+        a.part[2 * blockIdx.x + 1] = s1[0];
+    }
+}
+
+/* beta from (num, den) (liquid_closure.cpp:364-377): scal[0..1] = num, den;
+ * writes scal[2] = betaOpt, scal[3] = betaApplied. */
+__global__ void k_beta(double* scal, int useOptimal, double betaRepair) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    const double num = scal[0], den = scal[1];
+    double bopt = 0.0;
+    if (isfinite(num) && isfinite(den) && den > 0.0) bopt = dmax(num / den, 0.0);
+    double app;
+    if (!useOptimal) app = betaRepair;
+    else if (!isfinite(bopt) || !isfinite(den) || den < 1e-20) app = betaRepair;
+    else app = (bopt < 0.0) ? 0.0 : ((2.0 < bopt) ? 2.0 : bopt);
+    scal[2] = bopt;
+    scal[3] = app;
+}
+
+/* pdrive with the interzone cross smoothing (liquid_closure.cpp:401-419) */
+__global__ void k_pdrive(Geom g, const double* __restrict__ raw, const int* __restrict__ mask, double blend,
+                         int anyMask, double* __restrict__ pd) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= g.Nc) return;
+    double v = raw[c];
+    if (anyMask && mask[c] != 0) {
+        const int ix = c / g.Ny, iy = c - ix * g.Ny;
+        double acc = raw[c];
+        int n = 1;
+        /* left, right, down, up (liquid_closure.cpp:233-236) */
+        const int ddx[4] = {-1, 1, 0, 0}, ddy[4] = {0, 0, -1, 1};
+        for (int k = 0; k < 4; ++k) {
+            int jx = ix + ddx[k], jy = iy + ddy[k];
+            if (g.perx) jx = wrapi(jx, g.Nx);
+            else if (jx < 0 || jx >= g.Nx) continue;
+            if (g.pery) jy = wrapi(jy, g.Ny);
+            else if (jy < 0 || jy >= g.Ny) continue;
+            acc += raw[jy + g.Ny * jx];
+            ++n;
+        }
+        const double ps = acc / (double)n;
+        v = (1.0 - blend) * raw[c] + blend * ps;
+    }
+    pd[c] = v;
+}
+
+/* duEOS = -(dt/rhoT) * grad(pdrive) * betaEOS (liquid_closure.cpp:243-295, :424-430) */
+__global__ void k_eos(Geom g, const double* __restrict__ p, double dtOverRho, double betaEOS, double* __restrict__ ex,
+                      double* __restrict__ ey) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= g.Nc) return;
+    const int ix = c / g.Ny, iy = c - ix * g.Ny;
+    const double dx = g.dxc, dy = g.dyc;
+    double gx, gy;
+    if (g.perx) gx = (p[iy + g.Ny * wrapi(ix + 1, g.Nx)] - p[iy + g.Ny * wrapi(ix - 1, g.Nx)]) / (2.0 * dx);
+    else if (ix == 0) gx = (p[iy + g.Ny * (ix + 1)] - p[c]) / dx;
+    else if (ix == g.Nx - 1) gx = (p[c] - p[iy + g.Ny * (ix - 1)]) / dx;
+    else gx = (p[iy + g.Ny * (ix + 1)] - p[iy + g.Ny * (ix - 1)]) / (2.0 * dx);
+    if (g.pery) gy = (p[wrapi(iy + 1, g.Ny) + g.Ny * ix] - p[wrapi(iy - 1, g.Ny) + g.Ny * ix]) / (2.0 * dy);
+    else if (iy == 0) gy = (p[(iy + 1) + g.Ny * ix] - p[c]) / dy;
+    else if (iy == g.Ny - 1) gy = (p[c] - p[(iy - 1) + g.Ny * ix]) / dy;
+    else gy = (p[(iy + 1) + g.Ny * ix] - p[(iy - 1) + g.Ny * ix]) / (2.0 * dy);
+    ex[c] = -dtOverRho * gx * betaEOS;
+    ey[c] = -dtOverRho * gy * betaEOS;
+}
+
+/* v = (v + beta*duRepair[c]) + duEOS[c] for the particle's (red) cell
+ * (apply_cell_kick twice, liquid_closure.cpp:379-436).  Optional per-block
+ * partials: sum |kR|^2, max |kR|, sum |kE|^2, max |kE|, sum dvx, sum dvy. */
+__global__ void __launch_bounds__(TPB) k_kick(Geom g, const double* __restrict__ x, const double* __restrict__ y,
+                                              double* __restrict__ vx, double* __restrict__ vy,
+                                              const double* __restrict__ duRX, const double* __restrict__ duRY,
+                                              const double* __restrict__ duEX, const double* __restrict__ duEY,
+                                              const double* __restrict__ scal, double* __restrict__ part) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const double beta = scal[3];
+    double r2 = 0.0, rm = 0.0, e2 = 0.0, em = 0.0, dvx = 0.0, dvy = 0.0;
+    if (i < g.n) {
+        const int c = cell_of(g, x[i], y[i]);
+        const double kx = beta * duRX[c], ky = beta * duRY[c];
+        const double v0x = vx[i], v0y = vy[i];
+        double nx = v0x + kx, ny = v0y + ky;
+        const double ex = duEX[c], ey = duEY[c];
+        nx += ex;
+        ny += ey;
+        vx[i] = nx;
+        vy[i] = ny;
+        if (part) {
+            const double kr = MPCD_SQRT(kx * kx + ky * ky);
+            const double ke = MPCD_SQRT(ex * ex + ey * ey);
+            r2 = kr * kr;
+            rm = kr;
+            e2 = ke * ke;
+            em = ke;
+            dvx = nx - v0x;
+            dvy = ny - v0y;
+        }
+    }
+    if (!part) return;
+    __shared__ double s[6][TPB];
+    s[0][threadIdx.x] = r2;
+    s[1][threadIdx.x] = rm;
+    s[2][threadIdx.x] = e2;
+    s[3][threadIdx.x] = em;
+    s[4][threadIdx.x] = dvx;
+    s[5][threadIdx.x] = dvy;
+    __syncthreads();
+    for (int st = TPB / 2; st > 0; st >>= 1) {
+        if (threadIdx.x < (unsigned)st) {
+            s[0][threadIdx.x] += s[0][threadIdx.x + st];
+            s[1][threadIdx.x] = dmax(s[1][threadIdx.x], s[1][threadIdx.x + st]);
+            s[2][threadIdx.x] += s[2][threadIdx.x + st];
+            s[3][threadIdx.x] = dmax(s[3][threadIdx.x], s[3][threadIdx.x + st]);
+            s[4][threadIdx.x] += s[4][threadIdx.x + st];
+            s[5][threadIdx.x] += s[5][threadIdx.x + st];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        for (int k = 0; k < 6; ++k) part[6 * (size_t)blockIdx.x + k] = s[k][0];
+}
+
+/* per-cell closure metrics partials: sum hypot(beta*duR), sum pdriveRaw, sum pdrive */
+__global__ void __launch_bounds__(TPB) k_closure_cell_metrics(int Nc, const double* __restrict__ duRX,
+                                                              const double* __restrict__ duRY,
+                                                              const double* __restrict__ raw,
+                                                              const double* __restrict__ pd,
+                                                              const double* __restrict__ scal, double* __restrict__ part) {
+    __shared__ double s[3][TPB];
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    const double beta = scal[3];
+    double a = 0.0, b = 0.0, d = 0.0;
+    if (c < Nc) {
+        a = mpcd_hypot(beta * duRX[c], beta * duRY[c]);
+        b = raw[c];
+        d = pd[c];
+    }
+    s[0][threadIdx.x] = a;
+    s[1][threadIdx.x] = b;
+    s[2][threadIdx.x] = d;
+    __syncthreads();
+    for (int st = TPB / 2; st > 0; st >>= 1) {
+        if (threadIdx.x < (unsigned)st)
+            for (int k = 0; k < 3; ++k) s[k][threadIdx.x] += s[k][threadIdx.x + st];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        for (int k = 0; k < 3; ++k) part[3 * (size_t)blockIdx.x + k] = s[k][0];
+}
+
+/* ===================================================================== */
+/*                        host-side geometry                             */
+/* ===================================================================== */
+
+/* ===================================================================== */
+/*                               engine                                  */
+/* ===================================================================== */
+
+template <class T>
+T* dalloc(size_t n) {
+    void* p = nullptr;
+    if (n == 0) n = 1;
+    HIP_OK(hipMalloc(&p, n * sizeof(T)));
+    return (T*)p;
+}
+
+struct Binning {
+    int *cnt = nullptr, *start = nullptr, *cursor = nullptr, *list = nullptr;
+};
+
+}  // namespace
+
+struct mpcd_gpu_sim {
+    mpcd_params p{};
+    Geom g{};
+    int n = 0, Nc = 0;
+    hipStream_t stream = nullptr;
+    double *x = nullptr, *y = nullptr, *vx = nullptr, *vy = nullptr;
+    double *vx2 = nullptr, *vy2 = nullptr; /* zone-pass output velocities (swapped after each pass) */
+    int* pid = nullptr;
+    int* cid = nullptr;
+    Binning bin;
+    int* tileSum = nullptr;
+    int nTiles = 0;
+    int *next = nullptr, *tmpg = nullptr, *slotOf = nullptr;
+    long long scratchCap = 0;
+    int *zsize = nullptr, *zoff = nullptr;
+    double* stage = nullptr; /* 2n doubles, host<->device transfers */
+    double *refPx = nullptr, *refPy = nullptr, *refUx = nullptr, *refUy = nullptr, *pdriveRaw = nullptr;
+    double *redPx = nullptr, *redPy = nullptr, *redUx = nullptr, *redUy = nullptr;
+    double *duRX = nullptr, *duRY = nullptr, *duEX = nullptr, *duEY = nullptr, *pdrive = nullptr;
+    double* cellTmp = nullptr; /* Nc doubles for Pkin scratch */
+    int* mask = nullptr;
+    int anyMask = 0, nMasked = 0;
+    double* part = nullptr;
+    size_t partCap = 0;
+    double* scal = nullptr; /* device scalars */
+    int* err = nullptr;
+    WallAcc* wallPart = nullptr;
+    ZoneRect* rects[2] = {nullptr, nullptr};
+    int* owner[2] = {nullptr, nullptr};
+    int nz[2] = {0, 0};
+    int maxCells[2] = {0, 0}, maxCols[2] = {0, 0};
+    ZoneOut* zout = nullptr;
+    int zoutCap = 0;
+    unsigned char* zscratch = nullptr;
+    size_t zscratchCap = 0;
+    ZoneParams zp{};
+    CollideParams cp{};
+    BCParams bc{};
+    double xmax = 0, ymax = 0;
+
+    explicit mpcd_gpu_sim(const mpcd_params& prm) : p(prm) {
+        validate(p);
+        g = make_geom(p);
+        n = p.n;
+        Nc = p.Nc;
+        HIP_OK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+        x = dalloc<double>(n);
+        y = dalloc<double>(n);
+        vx = dalloc<double>(n);
+        vy = dalloc<double>(n);
+        vx2 = dalloc<double>(n);
+        vy2 = dalloc<double>(n);
+        pid = dalloc<int>(n);
+        cid = dalloc<int>(n);
+        bin.cnt = dalloc<int>((size_t)Nc + 1);
+        bin.start = dalloc<int>((size_t)Nc + 1);
+        bin.cursor = dalloc<int>((size_t)Nc + 1);
+        bin.list = dalloc<int>(n);
+        nTiles = (int)((Nc + 1 + SCAN_TILE - 1) / SCAN_TILE);
+        tileSum = dalloc<int>(nTiles);
+        stage = dalloc<double>(2 * (size_t)n);
+        double** cells[] = {&refPx, &refPy, &refUx, &refUy, &pdriveRaw, &redPx, &redPy, &redUx, &redUy,
+                            &duRX, &duRY, &duEX, &duEY, &pdrive, &cellTmp};
+        for (double** c : cells) *c = dalloc<double>(Nc);
+        mask = dalloc<int>(Nc);
+        scal = dalloc<double>(64);
+        err = dalloc<int>(1);
+        ensure_parts((size_t)std::max(nblk(n) * 6, nblk(Nc) * 3) + 64);
+        wallPart = dalloc<WallAcc>(nblk(n));
+        hipLaunchKernelGGL(k_iota, dim3(nblk(n)), dim3(TPB), 0, stream, pid, n);
+        check_launch();
+        /* constants */
+        bc = make_bc_params(p);
+        xmax = std::nextafter(p.Lx, 0.0);
+        ymax = std::nextafter(p.Ly, 0.0);
+        cp = make_collide_params(p);
+        zp = make_zone_params(p);
+        /* layouts (zones of the shifted layout overlap: a cell is covered by
+         * up to four zones, so per-zone particle scratch is sized by the
+         * largest coverage) and the interzone mask */
+        int cover = 1;
+        for (int L = 0; L < 2; ++L) {
+            std::vector<int32_t> own;
+            const auto r = layout_rects(p, L == 0 ? MPCD_LAYOUT_BASE : MPCD_LAYOUT_SHIFTED, &own);
+            nz[L] = (int)r.size();
+            std::vector<int> cov((size_t)Nc, 0);
+            for (const auto& q : r) {
+                maxCells[L] = std::max(maxCells[L], (q.xe - q.xs) * (q.ye - q.ys));
+                maxCols[L] = std::max(maxCols[L], q.xe - q.xs);
+                for (int ix = q.xs; ix < q.xe; ++ix)
+                    for (int iy = q.ys; iy < q.ye; ++iy) cover = std::max(cover, ++cov[(size_t)(iy + p.Ny * ix)]);
+            }
+            rects[L] = dalloc<ZoneRect>(r.size());
+            HIP_OK(hipMemcpy(rects[L], r.data(), r.size() * sizeof(ZoneRect), hipMemcpyHostToDevice));
+            owner[L] = dalloc<int>((size_t)Nc);
+            HIP_OK(hipMemcpy(owner[L], own.data(), own.size() * sizeof(int), hipMemcpyHostToDevice));
+        }
+        scratchCap = (long long)cover * (long long)n;
+        next = dalloc<int>((size_t)scratchCap + 1);
+        tmpg = dalloc<int>((size_t)scratchCap + 1);
+        slotOf = dalloc<int>((size_t)scratchCap + 1);
+        zsize = dalloc<int>((size_t)std::max(nz[0], nz[1]) + 2);
+        zoff = dalloc<int>((size_t)std::max(nz[0], nz[1]) + 2);
+        ensure_zout(std::max(nz[0], nz[1]));
+        const auto m = interzone_mask_host(p);
+        nMasked = (int)std::count(m.begin(), m.end(), 1);
+        anyMask = nMasked > 0;
+        HIP_OK(hipMemcpy(mask, m.data(), m.size() * sizeof(int), hipMemcpyHostToDevice));
+        HIP_OK(hipStreamSynchronize(stream));
+    }
+
+    ~mpcd_gpu_sim() {
+        void* ptrs[] = {x, y, vx, vy, vx2, vy2, pid, cid, bin.cnt, bin.start, bin.cursor, bin.list, tileSum, next,
+                        tmpg, slotOf, zsize, zoff, owner[0], owner[1], stage,
+                        refPx, refPy, refUx, refUy, pdriveRaw, redPx, redPy, redUx, redUy, duRX, duRY, duEX, duEY,
+                        pdrive, cellTmp, mask, part, scal, err, wallPart, rects[0], rects[1], zout, zscratch};
+        for (void* q : ptrs)
+            if (q) (void)hipFree(q);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+
+    void ensure_parts(size_t m) {
+        if (m <= partCap) return;
+        if (part) HIP_OK(hipFree(part));
+        part = dalloc<double>(m);
+        partCap = m;
+    }
+    void ensure_zout(int m) {
+        if (m <= zoutCap) return;
+        if (zout) HIP_OK(hipFree(zout));
+        zout = dalloc<ZoneOut>(m);
+        zoutCap = m;
+    }
+
+    void check_launch() { HIP_OK(hipGetLastError()); }
+
+    /* ---- transfers ---- */
+    void upload_xy(const double* h, double* dx_, double* dy_) {
+        HIP_OK(hipMemcpyAsync(stage, h, 2 * (size_t)n * sizeof(double), hipMemcpyHostToDevice, stream));
+        hipLaunchKernelGGL(k_deinterleave, dim3(nblk(n)), dim3(TPB), 0, stream, stage, n, dx_, dy_);
+        check_launch();
+    }
+    void download_xy(const double* dx_, const double* dy_, double* h) {
+        hipLaunchKernelGGL(k_interleave, dim3(nblk(n)), dim3(TPB), 0, stream, dx_, dy_, pid, n, stage);
+        check_launch();
+        HIP_OK(hipMemcpyAsync(h, stage, 2 * (size_t)n * sizeof(double), hipMemcpyDeviceToHost, stream));
+        HIP_OK(hipStreamSynchronize(stream));
+    }
+    void set_state(const double* hx, const double* hv) {
+        hipLaunchKernelGGL(k_iota, dim3(nblk(n)), dim3(TPB), 0, stream, pid, n);
+        check_launch();
+        if (hx) upload_xy(hx, x, y);
+        if (hv) upload_xy(hv, vx, vy);
+        HIP_OK(hipStreamSynchronize(stream));
+    }
+    void get_state(double* hx, double* hv) {
+        if (hx) download_xy(x, y, hx);
+        if (hv) download_xy(vx, vy, hv);
+    }
+
+    /* ---- binning: cnt/start/list of the cell ids in `cid` ---- */
+    void bin_cells() {
+        /* fills by kernel, not memset, so the sequence can be graph-captured */
+        hipLaunchKernelGGL(k_fill_int, dim3(nblk(Nc + 1)), dim3(TPB), 0, stream, bin.cnt, Nc + 1, 0);
+        hipLaunchKernelGGL(k_fill_int, dim3(nblk(Nc + 1)), dim3(TPB), 0, stream, bin.cursor, Nc + 1, 0);
+        hipLaunchKernelGGL(k_hist, dim3(nblk(n)), dim3(TPB), 0, stream, cid, n, bin.cnt);
+        hipLaunchKernelGGL(k_scan_tiles, dim3(nTiles), dim3(TPB), 0, stream, bin.cnt, Nc + 1, bin.start, tileSum);
+        hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(TPB), 0, stream, tileSum, nTiles);
+        hipLaunchKernelGGL(k_scan_add, dim3(nblk(Nc + 1)), dim3(TPB), 0, stream, bin.start, Nc + 1, tileSum);
+        hipLaunchKernelGGL(k_scatter, dim3(nblk(n)), dim3(TPB), 0, stream, cid, n, bin.start, bin.cursor, bin.list);
+        hipLaunchKernelGGL(k_segsort, dim3(nblk(Nc)), dim3(TPB), 0, stream, bin.start, Nc, bin.list, pid);
+        check_launch();
+    }
+    void bin_unshifted() {
+        hipLaunchKernelGGL(k_cells, dim3(nblk(n)), dim3(TPB), 0, stream, g, x, y, cid);
+        check_launch();
+        bin_cells();
+    }
+
+    void clear_err() {
+        hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, stream, err, 1, 0);
+        check_launch();
+    }
+    void check_err() {
+        int h = 0;
+        HIP_OK(hipMemcpyAsync(&h, err, sizeof(int), hipMemcpyDeviceToHost, stream));
+        HIP_OK(hipStreamSynchronize(stream));
+        if (h & 1) throw ApiError(MPCD_E_BC_MODE, "Unsupported boundary mode at a wall a particle crossed");
+        if (h & 2) throw ApiError(MPCD_E_CAPACITY, "zone scratch capacity exceeded");
+    }
+
+    /* ---- stages ---- */
+    void stream_bc(uint64_t seed, int do_stream, const double* shift, mpcd_wall_info* wi) {
+        hipLaunchKernelGGL(k_stream_bc, dim3(nblk(n)), dim3(TPB), 0, stream, g, bc, seed, do_stream, x, y, vx, vy,
+                           pid, shift ? cid : (int*)nullptr, shift ? shift[0] : 0.0, shift ? shift[1] : 0.0, xmax,
+                           ymax, wi ? wallPart : (WallAcc*)nullptr, err);
+        check_launch();
+        if (wi) {
+            const int nb = (int)nblk(n);
+            std::vector<WallAcc> h((size_t)nb);
+            HIP_OK(hipMemcpyAsync(h.data(), wallPart, (size_t)nb * sizeof(WallAcc), hipMemcpyDeviceToHost, stream));
+            HIP_OK(hipStreamSynchronize(stream));
+            std::memset(wi, 0, sizeof(*wi));
+            for (const auto& a : h) {
+                wi->nBot += a.nBot;
+                wi->nTop += a.nTop;
+                wi->nLeft += a.nLeft;
+                wi->nRight += a.nRight;
+                wi->dEwall += a.dE;
+                wi->dPxBot += a.dPxBot;
+                wi->dPxTop += a.dPxTop;
+                wi->dPyBot += a.dPyBot;
+                wi->dPyTop += a.dPyTop;
+                wi->dPxLeft += a.dPxLeft;
+                wi->dPxRight += a.dPxRight;
+                wi->dPyLeft += a.dPyLeft;
+                wi->dPyRight += a.dPyRight;
+            }
+        }
+    }
+
+    void collide(uint64_t seed) {
+        hipLaunchKernelGGL(k_collide, dim3(nblk(Nc)), dim3(TPB), 0, stream, cp, seed, bin.start, bin.list, Nc, vx, vy);
+        check_launch();
+        if (p.keepMeanFlow) {
+            const int nb = (int)nblk(n);
+            hipLaunchKernelGGL(k_sum2, dim3(nb), dim3(TPB), 0, stream, vx, vy, n, part);
+            hipLaunchKernelGGL(k_reduce_parts, dim3(1), dim3(TPB), 0, stream, part, nb, 2, scal + 8);
+            hipLaunchKernelGGL(k_sub_mean, dim3(nb), dim3(TPB), 0, stream, vx, vy, n, scal + 8);
+            check_launch();
+        }
+    }
+
+    double vc() const { return (p.Lx / (double)p.Nx) * (p.Ly / (double)p.Ny); }
+    double rho_target() const { return p.gamma / vc(); }
+
+    /* ref cell sums for the closure, from the current (unshifted) binning */
+    void ref_fields() {
+        FieldsOut f{};
+        f.Ux = refUx;
+        f.Uy = refUy;
+        f.Px = refPx;
+        f.Py = refPy;
+        f.P = pdriveRaw; /* P = Pkin + Pvir = pdriveRaw (liquid_closure.cpp:401-405) */
+        hipLaunchKernelGGL(k_cell_fields, dim3(nblk(Nc)), dim3(TPB), 0, stream, bin.start, bin.list, Nc, vx, vy, vc(),
+                           p.Kvirial, rho_target(), f);
+        check_launch();
+    }
+
+    void check_pass_modes() const {
+        if (!p.useIncompressibleRedistribution || !p.useZoneRedistribution)
+            throw ApiError(MPCD_E_INVALID,
+                           "zone pass requires useIncompressibleRedistribution=true and useZoneRedistribution=true");
+        if (p.redistributionEnableSurfaceTopology || p.redistributionWallWettingEnabled ||
+            p.useInterfaceVelocityReorientation)
+            throw ApiError(MPCD_E_INVALID, "zone pass is restricted to no surface / no wetting / no reorientation");
+    }
+
+    /* One snapshot zone pass over the current unshifted binning (whose cid
+     * holds the input cells).  single = 1: one zone that owns all its
+     * particles, seeded with seedBase itself (run_zone_kernel_inplace). */
+    void zone_pass(uint64_t seedBase, const ZoneRect* rectsDev, int nZ, int maxC, int maxW, const int* ownerDev,
+                   int single) {
+        if (!single) {
+            hipLaunchKernelGGL(k_zone_sizes, dim3(nblk(nZ + 1)), dim3(TPB), 0, stream, rectsDev, nZ, bin.start, p.Ny,
+                               zsize);
+            const int nt = (nZ + 1 + SCAN_TILE - 1) / SCAN_TILE;
+            hipLaunchKernelGGL(k_scan_tiles, dim3(nt), dim3(TPB), 0, stream, zsize, nZ + 1, zoff, tileSum);
+            hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(TPB), 0, stream, tileSum, nt);
+            hipLaunchKernelGGL(k_scan_add, dim3(nblk(nZ + 1)), dim3(TPB), 0, stream, zoff, nZ + 1, tileSum);
+            check_launch();
+        } else {
+            /* particles outside the zone keep their velocity */
+            HIP_OK(hipMemcpyAsync(vx2, vx, (size_t)n * sizeof(double), hipMemcpyDeviceToDevice, stream));
+            HIP_OK(hipMemcpyAsync(vy2, vy, (size_t)n * sizeof(double), hipMemcpyDeviceToDevice, stream));
+        }
+        ZoneArgs a{};
+        a.g = g;
+        a.zp = zp;
+        a.rects = rectsDev;
+        a.nZones = nZ;
+        a.seedBase = seedBase;
+        a.singleSeed = single;
+        a.gcnt = bin.cnt;
+        a.gstart = bin.start;
+        a.glist = bin.list;
+        a.gpid = pid;
+        a.gcid = cid;
+        a.owner = single ? nullptr : ownerDev;
+        a.zoff = single ? nullptr : zoff;
+        a.scratchCap = scratchCap;
+        a.x = x;
+        a.y = y;
+        a.vxin = vx;
+        a.vyin = vy;
+        a.vxout = vx2;
+        a.vyout = vy2;
+        a.next = next;
+        a.tmpg = tmpg;
+        a.slotOf = slotOf;
+        a.out = zout;
+        a.err = err;
+        const size_t bytes = zone_bytes(maxC, maxW);
+        size_t lds = bytes;
+        if (bytes > (size_t)ZONE_LDS_CAP) {
+            const size_t need = bytes * (size_t)nZ;
+            if (need > zscratchCap) {
+                if (zscratch) HIP_OK(hipFree(zscratch));
+                zscratch = dalloc<unsigned char>(need);
+                zscratchCap = need;
+            }
+            a.gscratch = zscratch;
+            a.gstride = bytes;
+            lds = 0;
+        }
+        hipLaunchKernelGGL(k_zone, dim3(nZ), dim3(ZONE_TPB), lds, stream, a);
+        check_launch();
+        std::swap(vx, vx2);
+        std::swap(vy, vy2);
+    }
+
+    void layout_pass(int L, uint64_t seedBase) {
+        zone_pass(seedBase, rects[L], nz[L], maxCells[L], maxCols[L], owner[L], 0);
+    }
+
+    void pass_metrics(int nZ, mpcd_zone_pass_metrics* m) {
+        std::vector<ZoneOut> h((size_t)nZ);
+        HIP_OK(hipMemcpyAsync(h.data(), zout, (size_t)nZ * sizeof(ZoneOut), hipMemcpyDeviceToHost, stream));
+        HIP_OK(hipStreamSynchronize(stream));
+        std::memset(m, 0, sizeof(*m));
+        m->nZones = nZ;
+        double acc = 0.0, mx = 0.0;
+        int cntz = 0;
+        for (const auto& o : h) {
+            m->nParticlesMovedDense += o.movedDense;
+            m->nParticlesMovedSparse += o.movedSparse;
+            if (o.stats4 > 0.0) {
+                acc += o.stats4 * o.stats4;
+                ++cntz;
+            }
+            mx = std::max(mx, o.stats5);
+        }
+        m->corrRmsDU = cntz > 0 ? std::sqrt(acc / (double)cntz) : 0.0;
+        m->corrMaxDU = mx;
+    }
+
+    /* closure: ref sums already in ref*; red binning current */
+    void closure(mpcd_closure_metrics* cm) {
+        if (!p.useLiquidClosure) throw ApiError(MPCD_E_INVALID, "run_liquid_closure requires useLiquidClosure=true");
+        FieldsOut f{};
+        f.Ux = redUx;
+        f.Uy = redUy;
+        f.Px = redPx;
+        f.Py = redPy;
+        hipLaunchKernelGGL(k_cell_fields, dim3(nblk(Nc)), dim3(TPB), 0, stream, bin.start, bin.list, Nc, vx, vy, vc(),
+                           p.Kvirial, rho_target(), f);
+        ClosureArgs ca{};
+        ca.Nc = Nc;
+        ca.gamma = p.gamma;
+        ca.refPx = refPx;
+        ca.refPy = refPy;
+        ca.refUx = refUx;
+        ca.refUy = refUy;
+        ca.redPx = redPx;
+        ca.redPy = redPy;
+        ca.redUx = redUx;
+        ca.redUy = redUy;
+        ca.duRX = duRX;
+        ca.duRY = duRY;
+        ca.part = part;
+        const int nbc = (int)nblk(Nc);
+        hipLaunchKernelGGL(k_closure_repair, dim3(nbc), dim3(TPB), 0, stream, ca);
+        hipLaunchKernelGGL(k_reduce_parts, dim3(1), dim3(TPB), 0, stream, part, nbc, 2, scal);
+        hipLaunchKernelGGL(k_beta, dim3(1), dim3(64), 0, stream, scal, p.useOptimalBetaRepair, p.betaRepair);
+        hipLaunchKernelGGL(k_pdrive, dim3(nbc), dim3(TPB), 0, stream, g, pdriveRaw, mask, p.smoothPdriveInterzoneBlend,
+                           anyMask, pdrive);
+        const double dtr = p.dt / rho_target();
+        hipLaunchKernelGGL(k_eos, dim3(nbc), dim3(TPB), 0, stream, g, pdrive, dtr, p.betaEOS, duEX, duEY);
+        check_launch();
+        const int nbp = (int)nblk(n);
+        if (cm) {
+            hipLaunchKernelGGL(k_closure_cell_metrics, dim3(nbc), dim3(TPB), 0, stream, Nc, duRX, duRY, pdriveRaw,
+                               pdrive, scal, part);
+            hipLaunchKernelGGL(k_reduce_parts, dim3(1), dim3(TPB), 0, stream, part, nbc, 3, scal + 16);
+        }
+        hipLaunchKernelGGL(k_kick, dim3(nbp), dim3(TPB), 0, stream, g, x, y, vx, vy, duRX, duRY, duEX, duEY, scal,
+                           cm ? part : (double*)nullptr);
+        check_launch();
+        if (cm) {
+            std::vector<double> h((size_t)nbp * 6);
+            HIP_OK(hipMemcpyAsync(h.data(), part, h.size() * sizeof(double), hipMemcpyDeviceToHost, stream));
+            double sc[24];
+            HIP_OK(hipMemcpyAsync(sc, scal, sizeof(sc), hipMemcpyDeviceToHost, stream));
+            HIP_OK(hipStreamSynchronize(stream));
+            double r2 = 0, rm = 0, e2 = 0, em = 0, dvx = 0, dvy = 0;
+            for (int b = 0; b < nbp; ++b) {
+                r2 += h[6 * (size_t)b];
+                rm = std::max(rm, h[6 * (size_t)b + 1]);
+                e2 += h[6 * (size_t)b + 2];
+                em = std::max(em, h[6 * (size_t)b + 3]);
+                dvx += h[6 * (size_t)b + 4];
+                dvy += h[6 * (size_t)b + 5];
+            }
+            std::memset(cm, 0, sizeof(*cm));
+            cm->betaRepairNum = sc[0];
+            cm->betaRepairDen = sc[1];
+            cm->betaRepairOpt = sc[2];
+            cm->betaRepairApplied = sc[3];
+            cm->betaEOSApplied = p.betaEOS;
+            cm->meanAbsRepairDU = sc[16] / (double)std::max(Nc, 1);
+            cm->meanPdriveRaw = sc[17] / (double)Nc;
+            cm->meanPdrive = sc[18] / (double)Nc;
+            cm->rmsRepairDU = n > 0 ? std::sqrt(r2 / (double)n) : 0.0;
+            cm->maxRepairDU = n > 0 ? rm : 0.0;
+            cm->rmsEOSDU = n > 0 ? std::sqrt(e2 / (double)n) : 0.0;
+            cm->maxEOSDU = n > 0 ? em : 0.0;
+            cm->momentumDeltaX = dvx;
+            cm->momentumDeltaY = dvy;
+            cm->nMaskedInterzoneCells = nMasked;
+        }
+    }
+
+    /* one full time step with the benchmark driver's seeds
+     * (main_benchmark_poiseuille_openmp_profiled_shifted_profiled.cpp:240-279) */
+    void step(int s, mpcd_step_metrics* sm, double* const* stages) {
+        check_pass_modes();
+        const uint64_t us = (uint64_t)s;
+        double sh[2];
+        mpcd_draw_shift(130363ULL + 10007ULL * us, p.a0, &sh[0], &sh[1]);
+        stream_bc(104729ULL + 10007ULL * us, 1, sh, sm ? &sm->wall : nullptr);
+        bin_cells();
+        collide(433494437ULL + 10007ULL * us);
+        bin_unshifted();
+        if (stages && stages[0]) get_state(stages[0], stages[1]);
+        ref_fields();
+        const uint64_t zs = 88172645463325252ULL + 10007ULL * us;
+        layout_pass(0, zs);
+        if (sm) pass_metrics(nz[0], &sm->base);
+        bin_unshifted();
+        if (stages && stages[2]) get_state(stages[2], stages[3]);
+        layout_pass(1, zs + 4099ULL);
+        if (sm) pass_metrics(nz[1], &sm->shifted);
+        bin_unshifted();
+        if (stages && stages[4]) get_state(stages[4], stages[5]);
+        closure(sm ? &sm->closure : nullptr);
+    }
+};
+
+/* ===================================================================== */
+/*                                C ABI                                  */
+/* ===================================================================== */
+
+namespace {
+
+template <class F>
+int guarded(F&& f) {
+    try {
+        f();
+        return MPCD_OK;
+    } catch (const ApiError& e) {
+        g_last_error = e.what();
+        return e.code;
+    } catch (const HipError& e) {
+        g_last_error = e.what();
+        return MPCD_E_HIP;
+    } catch (const std::exception& e) {
+        g_last_error = e.what();
+        return MPCD_E_INVALID;
+    }
+}
+
+/* Scratch simulations reused across stateless calls of the same shape. */
+std::mutex g_cache_mu;
+mpcd_gpu_sim* g_cache = nullptr;
+
+mpcd_gpu_sim& scratch_sim(const mpcd_params& p) {
+    if (g_cache && std::memcmp(&g_cache->p, &p, sizeof(p)) == 0) return *g_cache;
+    delete g_cache;
+    g_cache = nullptr;
+    g_cache = new mpcd_gpu_sim(p);
+    return *g_cache;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* mpcd_gpu_last_error(void) { return g_last_error.c_str(); }
+
+int mpcd_gpu_device_count(int* count) {
+    return guarded([&] {
+        int c = 0;
+        HIP_OK(hipGetDeviceCount(&c));
+        *count = c;
+    });
+}
+
+int mpcd_gpu_sim_create(const mpcd_params* p, mpcd_gpu_sim** out) {
+    return guarded([&] {
+        if (!p || !out) throw ApiError(MPCD_E_INVALID, "null argument");
+        *out = new mpcd_gpu_sim(*p);
+    });
+}
+
+int mpcd_gpu_sim_destroy(mpcd_gpu_sim* s) {
+    return guarded([&] { delete s; });
+}
+
+int mpcd_gpu_sim_set_state(mpcd_gpu_sim* s, const double* x, const double* v) {
+    return guarded([&] { s->set_state(x, v); });
+}
+
+int mpcd_gpu_sim_get_state(mpcd_gpu_sim* s, double* x, double* v) {
+    return guarded([&] { s->get_state(x, v); });
+}
+
+int mpcd_gpu_sim_step(mpcd_gpu_sim* s, int first_step, int n_steps, mpcd_step_metrics* last) {
+    return guarded([&] {
+        s->clear_err();
+        for (int k = 0; k < n_steps; ++k) s->step(first_step + k, (k == n_steps - 1) ? last : nullptr, nullptr);
+        s->check_err();
+    });
+}
+
+/* Enqueue steps without waiting or checking the wall-mode flag (timed loops). */
+int mpcd_gpu_sim_step_async(mpcd_gpu_sim* s, int first_step, int n_steps) {
+    return guarded([&] {
+        for (int k = 0; k < n_steps; ++k) s->step(first_step + k, nullptr, nullptr);
+    });
+}
+
+int mpcd_gpu_sim_check(mpcd_gpu_sim* s) {
+    return guarded([&] { s->check_err(); });
+}
+
+int mpcd_gpu_sim_step_stages(mpcd_gpu_sim* s, int step, double* x_ref, double* v_ref, double* x_base, double* v_base,
+                             double* x_red, double* v_red, mpcd_step_metrics* m) {
+    return guarded([&] {
+        double* st[6] = {x_ref, v_ref, x_base, v_base, x_red, v_red};
+        s->clear_err();
+        s->step(step, m, st);
+        s->check_err();
+    });
+}
+
+int mpcd_gpu_sim_synchronize(mpcd_gpu_sim* s) {
+    return guarded([&] { HIP_OK(hipStreamSynchronize(s->stream)); });
+}
+
+int mpcd_gpu_sim_stream(mpcd_gpu_sim* s, void** stream) {
+    return guarded([&] { *stream = (void*)s->stream; });
+}
+
+int mpcd_gpu_force_stream_bc(const mpcd_params* p, double* x, double* v, uint64_t rng_seed, int do_stream,
+                             mpcd_wall_info* info) {
+    return guarded([&] {
+        std::lock_guard<std::mutex> lk(g_cache_mu);
+        auto& s = scratch_sim(*p);
+        s.set_state(x, v);
+        mpcd_wall_info wi{};
+        s.clear_err();
+        s.stream_bc(rng_seed, do_stream, nullptr, &wi);
+        s.check_err();
+        s.get_state(x, v);
+        if (info) *info = wi;
+    });
+}
+
+int mpcd_gpu_apply_bc_general(const mpcd_params* p, double* x, double* v, uint64_t rng_seed, mpcd_wall_info* info) {
+    return mpcd_gpu_force_stream_bc(p, x, v, rng_seed, 0, info);
+}
+
+int mpcd_gpu_srd_cell_id_with_random_shift(const mpcd_params* p, const double* x, uint64_t rng_seed, int32_t* cid) {
+    return guarded([&] {
+        std::lock_guard<std::mutex> lk(g_cache_mu);
+        auto& s = scratch_sim(*p);
+        s.set_state(x, nullptr);
+        double sx, sy;
+        mpcd_draw_shift(rng_seed, p->a0, &sx, &sy);
+        hipLaunchKernelGGL(k_shift_cells, dim3(nblk(s.n)), dim3(TPB), 0, s.stream, s.g, s.x, s.y, sx, sy, s.xmax,
+                           s.ymax, s.cid);
+        s.check_launch();
+        HIP_OK(hipMemcpyAsync(cid, s.cid, (size_t)s.n * sizeof(int), hipMemcpyDeviceToHost, s.stream));
+        HIP_OK(hipStreamSynchronize(s.stream));
+    });
+}
+
+int mpcd_gpu_srd_collision_step(const mpcd_params* p, double* v, const int32_t* cid, uint64_t rng_seed) {
+    return guarded([&] {
+        std::lock_guard<std::mutex> lk(g_cache_mu);
+        for (int i = 0; i < p->n; ++i)
+            if (cid[i] < 0 || cid[i] >= p->Nc) throw ApiError(MPCD_E_INVALID, "srd_collision_step: cell id out of range");
+        auto& s = scratch_sim(*p);
+        s.set_state(nullptr, v);
+        HIP_OK(hipMemcpyAsync(s.cid, cid, (size_t)s.n * sizeof(int), hipMemcpyHostToDevice, s.stream));
+        s.bin_cells();
+        s.collide(rng_seed);
+        s.get_state(nullptr, v);
+    });
+}
+
+int mpcd_gpu_compute_cell_fields(const mpcd_params* p, const double* x, const double* v, double rhoTarget,
+                                 mpcd_cell_fields* out) {
+    return guarded([&] {
+        std::lock_guard<std::mutex> lk(g_cache_mu);
+        auto& s = scratch_sim(*p);
+        s.set_state(x, v);
+        s.bin_unshifted();
+        std::vector<double*> d(9);
+        for (auto& q : d) q = dalloc<double>((size_t)s.Nc);
+        int* dN = dalloc<int>((size_t)s.Nc);
+        FieldsOut f{dN, d[0], d[1], d[2], d[3], d[4], d[5], d[6], d[7], d[8]};
+        hipLaunchKernelGGL(k_cell_fields, dim3(nblk(s.Nc)), dim3(TPB), 0, s.stream, s.bin.start, s.bin.list, s.Nc,
+                           s.vx, s.vy, s.vc(), p->Kvirial, rhoTarget, f);
+        s.check_launch();
+        const size_t nb = (size_t)s.Nc * sizeof(double);
+        if (out->N) HIP_OK(hipMemcpyAsync(out->N, dN, (size_t)s.Nc * sizeof(int), hipMemcpyDeviceToHost, s.stream));
+        double* hs[9] = {out->Ux, out->Uy, out->Px, out->Py, out->T, out->rho, out->Pkin, out->Pvir, out->P};
+        for (int k = 0; k < 9; ++k)
+            if (hs[k]) HIP_OK(hipMemcpyAsync(hs[k], d[(size_t)k], nb, hipMemcpyDeviceToHost, s.stream));
+        HIP_OK(hipStreamSynchronize(s.stream));
+        if (out->rhoTarget)
+            for (int c = 0; c < s.Nc; ++c) out->rhoTarget[c] = rhoTarget;
+        for (auto q : d) HIP_OK(hipFree(q));
+        HIP_OK(hipFree(dN));
+    });
+}
+
+int mpcd_gpu_build_zone_layout(const mpcd_params* p, int mode, int32_t* ownerMap, int32_t* nZones, int32_t* rects4) {
+    return guarded([&] {
+        validate(*p);
+        std::vector<int32_t> own;
+        const auto r = layout_rects(*p, mode, ownerMap ? &own : nullptr);
+        if (ownerMap) std::memcpy(ownerMap, own.data(), own.size() * sizeof(int32_t));
+        if (nZones) *nZones = (int32_t)r.size();
+        if (rects4)
+            for (size_t k = 0; k < r.size(); ++k) {
+                rects4[4 * k] = r[k].xs;
+                rects4[4 * k + 1] = r[k].xe;
+                rects4[4 * k + 2] = r[k].ys;
+                rects4[4 * k + 3] = r[k].ye;
+            }
+    });
+}
+
+int mpcd_gpu_run_zone_kernel(const mpcd_params* p, double* x, double* v, int xs, int xe, int ys, int ye,
+                             uint64_t rng_seed, double* stats6, int32_t* moved2) {
+    return guarded([&] {
+        std::lock_guard<std::mutex> lk(g_cache_mu);
+        if (xs < 0 || ys < 0 || xe > p->Nx || ye > p->Ny || xs >= xe || ys >= ye)
+            throw ApiError(MPCD_E_INVALID, "zone rectangle out of range");
+        auto& s = scratch_sim(*p);
+        s.set_state(x, v);
+        s.bin_unshifted();
+        ZoneRect r{xs, xe, ys, ye};
+        ZoneRect* d = dalloc<ZoneRect>(1);
+        HIP_OK(hipMemcpyAsync(d, &r, sizeof(r), hipMemcpyHostToDevice, s.stream));
+        s.clear_err();
+        s.zone_pass(rng_seed, d, 1, (xe - xs) * (ye - ys), xe - xs, nullptr, 1);
+        ZoneOut o{};
+        HIP_OK(hipMemcpyAsync(&o, s.zout, sizeof(o), hipMemcpyDeviceToHost, s.stream));
+        s.get_state(x, v);
+        HIP_OK(hipFree(d));
+        s.check_err();
+        if (stats6) {
+            stats6[0] = (double)o.nHigh;
+            stats6[1] = (double)o.nLow;
+            stats6[2] = (double)(o.movedDense + o.movedSparse);
+            stats6[3] = (double)(o.movedDense + o.movedSparse);
+            stats6[4] = o.stats4;
+            stats6[5] = o.stats5;
+        }
+        if (moved2) {
+            moved2[0] = o.movedDense;
+            moved2[1] = o.movedSparse;
+        }
+    });
+}
+
+int mpcd_gpu_zone_pass(const mpcd_params* p, double* x, double* v, int layoutMode, uint64_t rngSeedBase,
+                       mpcd_zone_pass_metrics* m, int32_t* ownerMap) {
+    return guarded([&] {
+        std::lock_guard<std::mutex> lk(g_cache_mu);
+        auto& s = scratch_sim(*p);
+        s.check_pass_modes();
+        s.set_state(x, v);
+        s.bin_unshifted();
+        std::vector<int32_t> own;
+        const auto r = layout_rects(*p, layoutMode, &own);
+        if (layoutMode != MPCD_LAYOUT_BASE && layoutMode != MPCD_LAYOUT_SHIFTED && layoutMode != MPCD_LAYOUT_SINGLE_FULL)
+            throw ApiError(MPCD_E_INVALID, "unknown layout mode");
+        int maxC = 0, maxW = 0;
+        for (const auto& q : r) {
+            maxC = std::max(maxC, (q.xe - q.xs) * (q.ye - q.ys));
+            maxW = std::max(maxW, q.xe - q.xs);
+        }
+        ZoneRect* d = dalloc<ZoneRect>(r.size());
+        int* dOwn = dalloc<int>(own.size());
+        HIP_OK(hipMemcpyAsync(d, r.data(), r.size() * sizeof(ZoneRect), hipMemcpyHostToDevice, s.stream));
+        HIP_OK(hipMemcpyAsync(dOwn, own.data(), own.size() * sizeof(int), hipMemcpyHostToDevice, s.stream));
+        s.ensure_zout((int)r.size());
+        s.clear_err();
+        s.zone_pass(rngSeedBase, d, (int)r.size(), maxC, maxW, dOwn, 0);
+        s.check_err();
+        mpcd_zone_pass_metrics mm{};
+        s.pass_metrics((int)r.size(), &mm);
+        s.get_state(x, v);
+        HIP_OK(hipFree(d));
+        HIP_OK(hipFree(dOwn));
+        if (m) *m = mm;
+        if (ownerMap) std::memcpy(ownerMap, own.data(), own.size() * sizeof(int32_t));
+    });
+}
+
+int mpcd_gpu_liquid_closure(const mpcd_params* p, const double* xRef, const double* vRef, const double* xRed,
+                            const double* vRed, double* xOut, double* vOut, mpcd_closure_metrics* m,
+                            mpcd_closure_grids* grids) {
+    return guarded([&] {
+        std::lock_guard<std::mutex> lk(g_cache_mu);
+        auto& s = scratch_sim(*p);
+        if (!p->useLiquidClosure) throw ApiError(MPCD_E_INVALID, "run_liquid_closure requires useLiquidClosure=true");
+        s.set_state(xRef, vRef);
+        s.bin_unshifted();
+        s.ref_fields();
+        s.set_state(xRed, vRed);
+        s.bin_unshifted();
+        mpcd_closure_metrics cm{};
+        s.closure(&cm);
+        if (grids) {
+            const size_t nb = (size_t)s.Nc * sizeof(double);
+            const double* src[6] = {s.pdriveRaw, s.pdrive, s.duRX, s.duRY, s.duEX, s.duEY};
+            double* dst[6] = {grids->pdriveRaw, grids->pdrive, grids->duRepairX, grids->duRepairY, grids->duEOSX,
+                              grids->duEOSY};
+            for (int k = 0; k < 6; ++k)
+                if (dst[k]) HIP_OK(hipMemcpyAsync(dst[k], src[k], nb, hipMemcpyDeviceToHost, s.stream));
+            if (grids->interzoneMask)
+                HIP_OK(hipMemcpyAsync(grids->interzoneMask, s.mask, (size_t)s.Nc * sizeof(int), hipMemcpyDeviceToHost,
+                                      s.stream));
+            HIP_OK(hipStreamSynchronize(s.stream));
+        }
+        s.get_state(xOut, vOut);
+        if (m) *m = cm;
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,263 @@
+"""ctypes binding of libmpcd_gpu.so (the C ABI in include/mpcd_gpu.h).
+
+The GPU library is the only compute path: loading raises if the library has
+not been built, and every call raises ``MpcdError`` with the library's
+message when it returns an error code (the C++ layer does the same with
+std::runtime_error, as the reference does).
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from typing import Dict, Optional
+
+import numpy as np
+
+from .params import MpcdParams
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG, "libmpcd_gpu.so")
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int32)
+
+
+class MpcdError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(msg)
+        self.code = code
+
+
+class WallInfo(ctypes.Structure):
+    _fields_ = [("nBot", ctypes.c_int32), ("nTop", ctypes.c_int32), ("nLeft", ctypes.c_int32),
+                ("nRight", ctypes.c_int32)] + [(k, ctypes.c_double) for k in (
+                    "dEwall", "dPxBot", "dPxTop", "dPyBot", "dPyTop", "dPxLeft", "dPxRight", "dPyLeft", "dPyRight")]
+
+
+class PassMetrics(ctypes.Structure):
+    _fields_ = [("nZones", ctypes.c_int32), ("nParticlesMovedDense", ctypes.c_int32),
+                ("nParticlesMovedSparse", ctypes.c_int32), ("_pad", ctypes.c_int32),
+                ("corrRmsDU", ctypes.c_double), ("corrMaxDU", ctypes.c_double)]
+
+
+CLOSURE_METRIC_NAMES = ("betaRepairOpt", "betaRepairApplied", "betaRepairNum", "betaRepairDen", "betaEOSApplied",
+                        "meanPdriveRaw", "meanPdrive", "rmsRepairDU", "meanAbsRepairDU", "maxRepairDU", "rmsEOSDU",
+                        "maxEOSDU", "momentumDeltaX", "momentumDeltaY")
+
+
+class ClosureMetrics(ctypes.Structure):
+    _fields_ = [(k, ctypes.c_double) for k in CLOSURE_METRIC_NAMES] + [
+        ("nMaskedInterzoneCells", ctypes.c_int32), ("_pad", ctypes.c_int32)]
+
+
+FIELD_NAMES = ("Ux", "Uy", "Px", "Py", "T", "rho", "Pkin", "Pvir", "P", "rhoTarget")
+
+
+class CellFieldsC(ctypes.Structure):
+    _fields_ = [("N", _ip)] + [(k, _dp) for k in FIELD_NAMES]
+
+
+class ClosureGrids(ctypes.Structure):
+    _fields_ = [(k, _dp) for k in ("pdriveRaw", "pdrive", "duRepairX", "duRepairY", "duEOSX", "duEOSY")] + [
+        ("interzoneMask", _ip)]
+
+
+class StepMetrics(ctypes.Structure):
+    _fields_ = [("wall", WallInfo), ("base", PassMetrics), ("shifted", PassMetrics), ("closure", ClosureMetrics)]
+
+
+def struct_to_dict(s: ctypes.Structure) -> Dict[str, object]:
+    out = {}
+    for k, _ in s._fields_:
+        if k.startswith("_"):
+            continue
+        val = getattr(s, k)
+        out[k] = struct_to_dict(val) if isinstance(val, ctypes.Structure) else val
+    return out
+
+
+def _ptr(a: np.ndarray, ct=ctypes.c_double):
+    return a.ctypes.data_as(ctypes.POINTER(ct))
+
+
+def _f64(a: np.ndarray) -> np.ndarray:
+    if a.dtype != np.float64 or not a.flags["C_CONTIGUOUS"] or not a.flags["WRITEABLE"]:
+        raise TypeError("expected a writable C-contiguous float64 array")
+    return a
+
+
+_LIB: Optional[ctypes.CDLL] = None
+
+
+def load(path: str = LIB_PATH) -> ctypes.CDLL:
+    """Load the GPU library; raises if it has not been built."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not os.path.exists(path):
+        raise FileNotFoundError(f"{path} is missing: build it with __graft_entry__.build() "
+                                "(or python -m src_incompressible_openmp_amd.build)")
+    lib = ctypes.CDLL(path)
+    lib.mpcd_gpu_last_error.restype = ctypes.c_char_p
+    lib.mpcd_gpu_sim_create.argtypes = [ctypes.POINTER(MpcdParams), ctypes.POINTER(ctypes.c_void_p)]
+    for name in ("mpcd_gpu_sim_destroy", "mpcd_gpu_sim_set_state", "mpcd_gpu_sim_get_state", "mpcd_gpu_sim_step",
+                 "mpcd_gpu_sim_step_async", "mpcd_gpu_sim_check", "mpcd_gpu_sim_step_stages",
+                 "mpcd_gpu_sim_synchronize", "mpcd_gpu_sim_stream"):
+        getattr(lib, name).restype = ctypes.c_int
+    _LIB = lib
+    return lib
+
+
+def _check(rc: int) -> None:
+    if rc != 0:
+        raise MpcdError(rc, load().mpcd_gpu_last_error().decode())
+
+
+def device_count() -> int:
+    c = ctypes.c_int(0)
+    _check(load().mpcd_gpu_device_count(ctypes.byref(c)))
+    return c.value
+
+
+# ---------------- stateless stage calls ----------------
+
+def force_stream_bc(p: MpcdParams, x: np.ndarray, v: np.ndarray, seed: int, do_stream: bool = True):
+    wi = WallInfo()
+    _check(load().mpcd_gpu_force_stream_bc(ctypes.byref(p), _ptr(_f64(x)), _ptr(_f64(v)), ctypes.c_uint64(seed),
+                                           int(do_stream), ctypes.byref(wi)))
+    return struct_to_dict(wi)
+
+
+def apply_bc_general(p: MpcdParams, x: np.ndarray, v: np.ndarray, seed: int):
+    return force_stream_bc(p, x, v, seed, do_stream=False)
+
+
+def srd_cell_id_with_random_shift(p: MpcdParams, x: np.ndarray, seed: int) -> np.ndarray:
+    cid = np.zeros(p.n, np.int32)
+    x = np.ascontiguousarray(x, np.float64)
+    _check(load().mpcd_gpu_srd_cell_id_with_random_shift(ctypes.byref(p), _ptr(x), ctypes.c_uint64(seed),
+                                                         _ptr(cid, ctypes.c_int32)))
+    return cid
+
+
+def srd_collision_step(p: MpcdParams, v: np.ndarray, cid: np.ndarray, seed: int) -> None:
+    cid = np.ascontiguousarray(cid, np.int32)
+    _check(load().mpcd_gpu_srd_collision_step(ctypes.byref(p), _ptr(_f64(v)), _ptr(cid, ctypes.c_int32),
+                                              ctypes.c_uint64(seed)))
+
+
+def compute_cell_fields(p: MpcdParams, x: np.ndarray, v: np.ndarray, rho_target: float) -> Dict[str, np.ndarray]:
+    arrs = {"N": np.zeros(p.Nc, np.int32)}
+    for k in FIELD_NAMES:
+        arrs[k] = np.zeros(p.Nc)
+    s = CellFieldsC(_ptr(arrs["N"], ctypes.c_int32), *[_ptr(arrs[k]) for k in FIELD_NAMES])
+    x = np.ascontiguousarray(x, np.float64)
+    v = np.ascontiguousarray(v, np.float64)
+    _check(load().mpcd_gpu_compute_cell_fields(ctypes.byref(p), _ptr(x), _ptr(v), ctypes.c_double(rho_target),
+                                               ctypes.byref(s)))
+    return arrs
+
+
+def build_zone_layout(p: MpcdParams, mode: int):
+    owner = np.zeros(p.Nc, np.int32)
+    nz = ctypes.c_int32(0)
+    _check(load().mpcd_gpu_build_zone_layout(ctypes.byref(p), mode, _ptr(owner, ctypes.c_int32), ctypes.byref(nz),
+                                             None))
+    rects = np.zeros(4 * nz.value, np.int32)
+    _check(load().mpcd_gpu_build_zone_layout(ctypes.byref(p), mode, None, None, _ptr(rects, ctypes.c_int32)))
+    return owner, rects.reshape(-1, 4)
+
+
+def run_zone_kernel(p: MpcdParams, x: np.ndarray, v: np.ndarray, rect, seed: int):
+    stats = np.zeros(6)
+    moved = np.zeros(2, np.int32)
+    xs, xe, ys, ye = (int(t) for t in rect)
+    _check(load().mpcd_gpu_run_zone_kernel(ctypes.byref(p), _ptr(_f64(x)), _ptr(_f64(v)), xs, xe, ys, ye,
+                                           ctypes.c_uint64(seed), _ptr(stats), _ptr(moved, ctypes.c_int32)))
+    return stats, moved
+
+
+def zone_pass(p: MpcdParams, x: np.ndarray, v: np.ndarray, layout: int, seed_base: int):
+    m = PassMetrics()
+    owner = np.zeros(p.Nc, np.int32)
+    _check(load().mpcd_gpu_zone_pass(ctypes.byref(p), _ptr(_f64(x)), _ptr(_f64(v)), layout,
+                                     ctypes.c_uint64(seed_base), ctypes.byref(m), _ptr(owner, ctypes.c_int32)))
+    return struct_to_dict(m), owner
+
+
+def liquid_closure(p: MpcdParams, x_ref, v_ref, x_red, v_red):
+    n2 = 2 * p.n
+    xo, vo = np.zeros(n2), np.zeros(n2)
+    m = ClosureMetrics()
+    grids = {k: np.zeros(p.Nc) for k in ("pdriveRaw", "pdrive", "duRepairX", "duRepairY", "duEOSX", "duEOSY")}
+    mask = np.zeros(p.Nc, np.int32)
+    g = ClosureGrids(*[_ptr(grids[k]) for k in ("pdriveRaw", "pdrive", "duRepairX", "duRepairY", "duEOSX",
+                                                "duEOSY")], _ptr(mask, ctypes.c_int32))
+    args = [np.ascontiguousarray(a, np.float64) for a in (x_ref, v_ref, x_red, v_red)]
+    _check(load().mpcd_gpu_liquid_closure(ctypes.byref(p), *[_ptr(a) for a in args], _ptr(xo), _ptr(vo),
+                                          ctypes.byref(m), ctypes.byref(g)))
+    out = {"x": xo, "v": vo, "metrics": struct_to_dict(m), "mask": mask}
+    out.update(grids)
+    return out
+
+
+# ---------------- resident simulation ----------------
+
+class Sim:
+    """Device-resident SRD/MPCD state advanced by whole time steps."""
+
+    def __init__(self, p: MpcdParams):
+        self.p = p
+        self._h = ctypes.c_void_p()
+        _check(load().mpcd_gpu_sim_create(ctypes.byref(p), ctypes.byref(self._h)))
+
+    def close(self) -> None:
+        if self._h:
+            _check(load().mpcd_gpu_sim_destroy(self._h))
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_state(self, x: np.ndarray, v: np.ndarray) -> None:
+        x = np.ascontiguousarray(x, np.float64)
+        v = np.ascontiguousarray(v, np.float64)
+        if x.size != 2 * self.p.n or v.size != 2 * self.p.n:
+            raise ValueError("state size does not match params.n")
+        _check(load().mpcd_gpu_sim_set_state(self._h, _ptr(x), _ptr(v)))
+
+    def get_state(self):
+        x, v = np.zeros(2 * self.p.n), np.zeros(2 * self.p.n)
+        _check(load().mpcd_gpu_sim_get_state(self._h, _ptr(x), _ptr(v)))
+        return x, v
+
+    def step(self, first_step: int, n_steps: int = 1, metrics: bool = False):
+        m = StepMetrics() if metrics else None
+        _check(load().mpcd_gpu_sim_step(self._h, first_step, n_steps, ctypes.byref(m) if m is not None else None))
+        return struct_to_dict(m) if m is not None else None
+
+    def step_async(self, first_step: int, n_steps: int = 1) -> None:
+        _check(load().mpcd_gpu_sim_step_async(self._h, first_step, n_steps))
+
+    def check(self) -> None:
+        _check(load().mpcd_gpu_sim_check(self._h))
+
+    def synchronize(self) -> None:
+        _check(load().mpcd_gpu_sim_synchronize(self._h))
+
+    def stream(self) -> int:
+        s = ctypes.c_void_p()
+        _check(load().mpcd_gpu_sim_stream(self._h, ctypes.byref(s)))
+        return s.value or 0
+
+    def step_stages(self, step: int):
+        n2 = 2 * self.p.n
+        st = {k: np.zeros(n2) for k in ("x_ref", "v_ref", "x_base", "v_base", "x_red", "v_red")}
+        m = StepMetrics()
+        _check(load().mpcd_gpu_sim_step_stages(self._h, step, *[_ptr(st[k]) for k in (
+            "x_ref", "v_ref", "x_base", "v_base", "x_red", "v_red")], ctypes.byref(m)))
+        st["metrics"] = struct_to_dict(m)
+        return st

@@ -1,0 +1,184 @@
+/*
+ * core_host.cpp -- host build of the GPU library's per-particle, per-cell and
+ * per-zone logic (src_incompressible_openmp_amd/csrc/mpcd_core.h), driven the
+ * way mpcd_engine.hip's kernels drive it: one zone at a time with the lanes of
+ * a workgroup run in sequence.  Test infrastructure: the host logic check
+ * compares it with the oracle's counter mode, so a logic error in the shared
+ * kernel bodies shows on the CPU before any GPU run.
+ */
+#include <cstring>
+#include <numeric>
+#include <vector>
+
+#include "mpcd_core.h"
+#include "mpcd_host.h"
+
+using namespace mpcdk;
+
+namespace {
+
+struct Bin {
+    std::vector<int> cnt, start, list;
+};
+
+/* counting sort of slots by cell; runs in ascending pid (slot == pid here) */
+Bin bin_by(const std::vector<int>& cid, int Nc) {
+    Bin b;
+    b.cnt.assign((size_t)Nc + 1, 0);
+    for (int c : cid) b.cnt[(size_t)c]++;
+    b.start.assign((size_t)Nc + 1, 0);
+    for (int c = 0; c < Nc; ++c) b.start[(size_t)c + 1] = b.start[(size_t)c] + b.cnt[(size_t)c];
+    b.list.assign(cid.size(), 0);
+    std::vector<int> cur(b.start.begin(), b.start.end() - 1);
+    for (size_t i = 0; i < cid.size(); ++i) b.list[(size_t)cur[(size_t)cid[i]]++] = (int)i;
+    return b;
+}
+
+void split(const double* a, int n, std::vector<double>& x, std::vector<double>& y) {
+    x.resize((size_t)n);
+    y.resize((size_t)n);
+    for (int i = 0; i < n; ++i) {
+        x[(size_t)i] = a[2 * (size_t)i];
+        y[(size_t)i] = a[2 * (size_t)i + 1];
+    }
+}
+void join(const std::vector<double>& x, const std::vector<double>& y, double* a) {
+    for (size_t i = 0; i < x.size(); ++i) {
+        a[2 * i] = x[i];
+        a[2 * i + 1] = y[i];
+    }
+}
+
+}  // namespace
+
+extern "C" int core_force_stream_bc(const mpcd_params* p, double* xa, double* va, uint64_t seed, int do_stream) {
+    const Geom g = make_geom(*p);
+    const BCParams b = make_bc_params(*p);
+    int rc = 0;
+    for (int i = 0; i < p->n; ++i) {
+        WallAcc acc;
+        wall_acc_zero(acc);
+        double x = xa[2 * (size_t)i], y = xa[2 * (size_t)i + 1], vx = va[2 * (size_t)i], vy = va[2 * (size_t)i + 1];
+        if (bc_particle(g, b, seed, (uint32_t)i, do_stream, x, y, vx, vy, acc) != 0) rc = MPCD_E_BC_MODE;
+        xa[2 * (size_t)i] = x;
+        xa[2 * (size_t)i + 1] = y;
+        va[2 * (size_t)i] = vx;
+        va[2 * (size_t)i + 1] = vy;
+    }
+    return rc;
+}
+
+extern "C" int core_collide(const mpcd_params* p, double* va, const int32_t* cid, uint64_t seed) {
+    const CollideParams cp = make_collide_params(*p);
+    std::vector<int> c(cid, cid + p->n);
+    const Bin b = bin_by(c, p->Nc);
+    std::vector<double> vx, vy;
+    split(va, p->n, vx, vy);
+    for (int cell = 0; cell < p->Nc; ++cell)
+        collide_cell(cp, seed, cell, b.list.data() + b.start[(size_t)cell], b.cnt[(size_t)cell], vx.data(), vy.data());
+    if (p->keepMeanFlow) { /* the engine reduces with a tree; here in particle order */
+        double mx = 0.0, my = 0.0;
+        for (int i = 0; i < p->n; ++i) {
+            mx += vx[(size_t)i];
+            my += vy[(size_t)i];
+        }
+        mx /= (double)p->n;
+        my /= (double)p->n;
+        for (int i = 0; i < p->n; ++i) {
+            vx[(size_t)i] -= mx;
+            vy[(size_t)i] -= my;
+        }
+    }
+    join(vx, vy, va);
+    return 0;
+}
+
+/* One snapshot zone pass as mpcd_engine.hip runs it: per-zone scratch at
+ * exact offsets, velocities from the pass input, owned particles written. */
+extern "C" int core_zone_pass(const mpcd_params* p, double* xa, double* va, int layout, uint64_t seedBase,
+                              int32_t* moved2, double* corr2) {
+    const Geom g = make_geom(*p);
+    const ZoneParams zp = make_zone_params(*p);
+    std::vector<double> x, y, vx, vy;
+    split(xa, p->n, x, y);
+    split(va, p->n, vx, vy);
+    std::vector<double> vxo(vx.size(), -777.0), vyo(vy.size(), -777.0);
+    std::vector<int> cid((size_t)p->n), pid((size_t)p->n);
+    std::iota(pid.begin(), pid.end(), 0);
+    for (int i = 0; i < p->n; ++i) cid[(size_t)i] = cell_of(g, x[(size_t)i], y[(size_t)i]);
+    const Bin b = bin_by(cid, p->Nc);
+    std::vector<int32_t> owner;
+    const auto rects = layout_rects(*p, layout, &owner);
+    /* per-zone particle counts and offsets */
+    std::vector<int> off(rects.size() + 1, 0);
+    for (size_t zi = 0; zi < rects.size(); ++zi) {
+        int P = 0;
+        for (int gx = rects[zi].xs; gx < rects[zi].xe; ++gx)
+            P += b.start[(size_t)(rects[zi].ye - 1 + p->Ny * gx) + 1] - b.start[(size_t)(rects[zi].ys + p->Ny * gx)];
+        off[zi + 1] = off[zi] + P;
+    }
+    std::vector<int> next((size_t)off.back() + 1), tmpg((size_t)off.back() + 1), slotOf((size_t)off.back() + 1);
+    int md = 0, ms = 0;
+    double acc = 0.0, mx = 0.0;
+    int nc = 0;
+    for (size_t zi = 0; zi < rects.size(); ++zi) {
+        const ZoneRect r = rects[zi];
+        ZoneCtx z{};
+        z.g = g;
+        z.zp = zp;
+        z.xs = r.xs;
+        z.xe = r.xe;
+        z.ys = r.ys;
+        z.ye = r.ye;
+        z.zw = r.xe - r.xs;
+        z.zh = r.ye - r.ys;
+        z.C = z.zw * z.zh;
+        z.zid = (int)zi + 1;
+        z.off = off[zi];
+        z.seed = seedBase + 104729ULL * (uint64_t)(zi + 1);
+        z.gcnt = b.cnt.data();
+        z.gstart = b.start.data();
+        z.glist = b.list.data();
+        z.gpid = pid.data();
+        z.gcid = cid.data();
+        z.owner = owner.data();
+        z.x = x.data();
+        z.y = y.data();
+        z.vxin = vx.data();
+        z.vyin = vy.data();
+        z.vxout = vxo.data();
+        z.vyout = vyo.data();
+        z.next = next.data();
+        z.tmpg = tmpg.data();
+        z.slotOf = slotOf.data();
+        std::vector<unsigned char> buf(zone_bytes(z.C, z.zw) + 16);
+        zone_carve(z, buf.data(), z.C, z.zw);
+        zone_setup_serial(z);
+        if (z.P != off[zi + 1] - off[zi]) return -1;
+        for (int lc = 0; lc < z.C; ++lc) zone_setup_cell(z, lc);
+        zone_dense_passes(z);
+        zone_sparse_pass(z);
+        for (int lc = 0; lc < z.C; ++lc) zone_correct_cell(z, lc);
+        zone_final_stats(z);
+        md += z.movedDense;
+        ms += z.movedSparse;
+        if (z.stats4 > 0.0) {
+            acc += z.stats4 * z.stats4;
+            ++nc;
+        }
+        mx = std::max(mx, z.stats5);
+    }
+    for (size_t i = 0; i < vxo.size(); ++i)
+        if (vxo[i] == -777.0) return -2; /* a particle no zone owns */
+    join(x, y, xa);
+    join(vxo, vyo, va);
+    if (moved2) {
+        moved2[0] = md;
+        moved2[1] = ms;
+    }
+    if (corr2) {
+        corr2[0] = nc > 0 ? std::sqrt(acc / (double)nc) : 0.0;
+        corr2[1] = mx;
+    }
+    return 0;
+}

@@ -26,38 +26,8 @@ _dp = ctypes.POINTER(ctypes.c_double)
 _ip = ctypes.POINTER(ctypes.c_int32)
 
 
-class WallInfo(ctypes.Structure):
-    _fields_ = [("nBot", ctypes.c_int32), ("nTop", ctypes.c_int32), ("nLeft", ctypes.c_int32),
-                ("nRight", ctypes.c_int32)] + [(k, ctypes.c_double) for k in (
-                    "dEwall", "dPxBot", "dPxTop", "dPyBot", "dPyTop", "dPxLeft", "dPxRight", "dPyLeft", "dPyRight")]
-
-
-class PassMetrics(ctypes.Structure):
-    _fields_ = [("nZones", ctypes.c_int32), ("nParticlesMovedDense", ctypes.c_int32),
-                ("nParticlesMovedSparse", ctypes.c_int32), ("_pad", ctypes.c_int32),
-                ("corrRmsDU", ctypes.c_double), ("corrMaxDU", ctypes.c_double)]
-
-
-CLOSURE_METRIC_NAMES = ("betaRepairOpt", "betaRepairApplied", "betaRepairNum", "betaRepairDen", "betaEOSApplied",
-                        "meanPdriveRaw", "meanPdrive", "rmsRepairDU", "meanAbsRepairDU", "maxRepairDU", "rmsEOSDU",
-                        "maxEOSDU", "momentumDeltaX", "momentumDeltaY")
-
-
-class ClosureMetrics(ctypes.Structure):
-    _fields_ = [(k, ctypes.c_double) for k in CLOSURE_METRIC_NAMES] + [
-        ("nMaskedInterzoneCells", ctypes.c_int32), ("_pad", ctypes.c_int32)]
-
-
-class CellFieldsC(ctypes.Structure):
-    _fields_ = [("N", _ip)] + [(k, _dp) for k in ("Ux", "Uy", "Px", "Py", "T", "rho", "Pkin", "Pvir", "P",
-                                                   "rhoTarget")]
-
-
-FIELD_NAMES = ("Ux", "Uy", "Px", "Py", "T", "rho", "Pkin", "Pvir", "P", "rhoTarget")
-
-
-def struct_to_dict(s: ctypes.Structure) -> Dict[str, float]:
-    return {k: getattr(s, k) for k, _ in s._fields_ if not k.startswith("_")}
+from src_incompressible_openmp_amd.gpu import (CLOSURE_METRIC_NAMES, FIELD_NAMES, CellFieldsC,  # noqa: E402
+                                              ClosureMetrics, PassMetrics, WallInfo, struct_to_dict)
 
 
 def _ptr(a: np.ndarray, ct=ctypes.c_double):

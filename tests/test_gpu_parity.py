@@ -1,0 +1,263 @@
+"""GPU parity: libmpcd_gpu.so (through its C ABI) against the oracle's
+counter-based mode on the same inputs and the same seeds.
+
+Bar: bitwise for positions, cell indices, occupancy and redistribution
+moves, and for every FP64 quantity whose sum order the GPU reproduces (per
+cell sums run over particles in ascending id, as in the reference); 1e-12
+relative for velocities that pass through a tree reduction (the closure's
+beta = num/den over all cells, the optional mean-flow removal).
+"""
+import numpy as np
+import pytest
+
+from src_incompressible_openmp_amd.params import LAYOUT_BASE, LAYOUT_SHIFTED, poiseuille_params, to_struct
+from tests.golden.cases import SMALL_CASES, case_params, case_state, ref_red_pair
+from tests.oracle_lib import CTR
+
+pytestmark = pytest.mark.gpu
+
+CASES = SMALL_CASES + ("cfg0",)
+RTOL_V = 1e-12  # velocities after a tree-reduced scalar
+
+
+@pytest.fixture(scope="module")
+def gpu():
+    from src_incompressible_openmp_amd import gpu as g
+
+    g.load()
+    assert g.device_count() >= 1, "no GPU visible"
+    return g
+
+
+def bits(a, b):
+    return np.ascontiguousarray(a).tobytes() == np.ascontiguousarray(b).tobytes()
+
+
+def vclose(a, b, rtol=RTOL_V):
+    scale = max(1.0, float(np.max(np.abs(b))))
+    return float(np.max(np.abs(a - b))) <= rtol * scale
+
+
+def post_collision(oracle, case):
+    p = to_struct(case_params(case))
+    x, v, _, _ = case_state(case)
+    oracle.force_stream(p, x, v)
+    oracle.apply_bc(CTR, p, x, v, 104729)
+    cid, _ = oracle.shift_cell_ids(CTR, p, x, 130363)
+    oracle.collide(CTR, p, v, cid, 433494437)
+    return p, x, v
+
+
+@pytest.mark.parametrize("case", CASES)
+def test_force_stream_bc(gpu, oracle, case):
+    p = to_struct(case_params(case))
+    x, v, _, _ = case_state(case)
+    xo, vo = x.copy(), v.copy()
+    oracle.force_stream(p, xo, vo)
+    wo = oracle.apply_bc(CTR, p, xo, vo, 104729)
+    xg, vg = x.copy(), v.copy()
+    wg = gpu.force_stream_bc(p, xg, vg, 104729)
+    assert bits(xg, xo) and bits(vg, vo)
+    for k in ("nBot", "nTop", "nLeft", "nRight"):
+        assert wg[k] == wo[k]
+    for k in ("dPyBot", "dPyTop", "dEwall"):
+        assert abs(wg[k] - wo[k]) <= 1e-9 * max(1.0, abs(wo[k]))
+
+
+@pytest.mark.parametrize("case", CASES)
+def test_shift_cell_ids(gpu, oracle, case):
+    p = to_struct(case_params(case))
+    x, _, _, _ = case_state(case)
+    co, _ = oracle.shift_cell_ids(CTR, p, x, 130363)
+    cg = gpu.srd_cell_id_with_random_shift(p, x, 130363)
+    assert np.array_equal(cg, co)
+
+
+@pytest.mark.parametrize("case", CASES)
+def test_collision(gpu, oracle, case):
+    p = to_struct(case_params(case))
+    x, v, _, _ = case_state(case)
+    cid, _ = oracle.shift_cell_ids(CTR, p, x, 130363)
+    vo, vg = v.copy(), v.copy()
+    oracle.collide(CTR, p, vo, cid, 433494437)
+    gpu.srd_collision_step(p, vg, cid, 433494437)
+    if p.keepMeanFlow:
+        assert vclose(vg, vo)
+    else:
+        assert bits(vg, vo)
+
+
+@pytest.mark.parametrize("case", CASES)
+def test_cell_fields(gpu, oracle, case):
+    p, x, v = post_collision(oracle, case)
+    rt = p.gamma / ((p.Lx / p.Nx) * (p.Ly / p.Ny))
+    fo = oracle.cell_fields(p, x, v, rt)
+    fg = gpu.compute_cell_fields(p, x, v, rt)
+    for k in fo:
+        assert bits(fg[k], fo[k]), k
+
+
+@pytest.mark.parametrize("case", CASES)
+@pytest.mark.parametrize("layout", [LAYOUT_BASE, LAYOUT_SHIFTED])
+def test_zone_pass(gpu, oracle, case, layout):
+    p, x, v = post_collision(oracle, case)
+    xo, vo = x.copy(), v.copy()
+    mo, owo = oracle.zone_pass(CTR, p, xo, vo, layout, 88172645463325252)
+    xg, vg = x.copy(), v.copy()
+    mg, owg = gpu.zone_pass(p, xg, vg, layout, 88172645463325252)
+    assert np.array_equal(owg, owo)
+    assert mg["nParticlesMovedDense"] == mo["nParticlesMovedDense"]
+    assert mg["nParticlesMovedSparse"] == mo["nParticlesMovedSparse"]
+    assert bits(xg, xo)
+    assert bits(vg, vo)
+    assert mg["corrRmsDU"] == mo["corrRmsDU"] and mg["corrMaxDU"] == mo["corrMaxDU"]
+
+
+@pytest.mark.parametrize("case", ["small", "bcmix", "override"])
+def test_single_zone_kernel(gpu, oracle, case):
+    """run_zone_kernel_inplace on individual zones of both layouts."""
+    p, x, v = post_collision(oracle, case)
+    for layout in (LAYOUT_BASE, LAYOUT_SHIFTED):
+        _, rects = gpu.build_zone_layout(p, layout)
+        for zi in range(0, len(rects), max(1, len(rects) // 4)):
+            seed = 9001 + zi
+            xo, vo = x.copy(), v.copy()
+            st, _, mv = oracle.zone_kernel(CTR, p, xo, vo, layout, zi, seed)
+            xg, vg = x.copy(), v.copy()
+            sg, mg = gpu.run_zone_kernel(p, xg, vg, rects[zi], seed)
+            assert np.array_equal(mg, mv)
+            assert bits(xg, xo) and bits(vg, vo)
+            assert bits(sg, st)
+
+
+@pytest.mark.parametrize("case", CASES)
+def test_closure(gpu, oracle, case):
+    p = to_struct(case_params(case))
+    x, v, _, _ = case_state(case)
+    xr, vr = ref_red_pair(x, v, p.Lx, p.Ly)
+    o = oracle.closure(p, x, v, xr, vr)
+    g = gpu.liquid_closure(p, x, v, xr, vr)
+    for k in ("pdriveRaw", "pdrive", "duRepairX", "duRepairY", "duEOSX", "duEOSY"):
+        assert bits(g[k], o[k]), k
+    assert np.array_equal(g["mask"], o["mask"])
+    assert bits(g["x"], o["x"])
+    assert vclose(g["v"], o["v"])
+    mo, mg = o["metrics"], g["metrics"]
+    assert mg["nMaskedInterzoneCells"] == mo["nMaskedInterzoneCells"]
+    for k in ("betaRepairNum", "betaRepairDen", "betaRepairOpt", "betaRepairApplied", "meanPdriveRaw", "meanPdrive",
+              "rmsRepairDU", "maxRepairDU", "meanAbsRepairDU", "rmsEOSDU", "maxEOSDU"):
+        assert abs(mg[k] - mo[k]) <= 1e-10 * max(1.0, abs(mo[k])), k
+
+
+@pytest.mark.parametrize("case", CASES)
+def test_full_step_stages(gpu, oracle, case):
+    """One resident step (benchmark driver seeds, step 3) stage by stage."""
+    p = to_struct(case_params(case))
+    x, v, _, _ = case_state(case)
+    xo, vo = x.copy(), v.copy()
+    so = oracle.step(CTR, p, xo, vo, 3, stages=True)
+    sim = gpu.Sim(p)
+    sim.set_state(x, v)
+    sg = sim.step_stages(3)
+    xg, vg = sim.get_state()
+    exact_v = not p.keepMeanFlow
+    for a in ("ref", "base", "red"):
+        assert bits(sg["x_" + a], so["x_" + a]), a
+        if exact_v:
+            assert bits(sg["v_" + a], so["v_" + a]), a
+        else:
+            assert vclose(sg["v_" + a], so["v_" + a]), a
+    assert bits(xg, xo)
+    assert vclose(vg, vo)
+    m = sg["metrics"]
+    for L in ("base", "shifted"):
+        assert m[L]["nParticlesMovedDense"] == so[L]["nParticlesMovedDense"]
+        assert m[L]["nParticlesMovedSparse"] == so[L]["nParticlesMovedSparse"]
+    assert m["wall"]["nBot"] == so["wall"]["nBot"] and m["wall"]["nTop"] == so["wall"]["nTop"]
+
+
+@pytest.mark.parametrize("case", ["small", "cfg0"])
+def test_steps_replayed(gpu, oracle, case):
+    """Steps 1..6: each step starts both codes from the oracle's state."""
+    p = to_struct(case_params(case))
+    x, v, _, _ = case_state(case)
+    sim = gpu.Sim(p)
+    for s in range(1, 7):
+        sim.set_state(x, v)
+        sim.step(s)
+        xg, vg = sim.get_state()
+        oracle.step(CTR, p, x, v, s)
+        assert bits(xg, x), s
+        assert vclose(vg, v), s
+
+
+def test_resident_run_matches_replay(gpu, oracle):
+    """Ten resident steps on the GPU equal ten oracle steps (the tiny beta
+    rounding difference does not flip any cell decision on this case)."""
+    p = to_struct(case_params("small"))
+    x, v, _, _ = case_state("small")
+    sim = gpu.Sim(p)
+    sim.set_state(x, v)
+    sim.step(1, 10)
+    xg, vg = sim.get_state()
+    for s in range(1, 11):
+        oracle.step(CTR, p, x, v, s)
+    assert np.max(np.abs(xg - x)) < 1e-9
+    assert vclose(vg, v, 1e-9)
+
+
+def test_determinism(gpu):
+    p = to_struct(case_params("cfg0"))
+    x, v, _, _ = case_state("cfg0")
+    outs = []
+    for _ in range(2):
+        sim = gpu.Sim(p)
+        sim.set_state(x, v)
+        sim.step(1, 3)
+        outs.append(sim.get_state())
+    assert bits(outs[0][0], outs[1][0]) and bits(outs[0][1], outs[1][1])
+
+
+def test_unsupported_wall_mode_raises(gpu):
+    prm = case_params("small")
+    prm["boundary_bottom"] = "sticky"
+    p = to_struct(prm)
+    x, v, _, _ = case_state("small")
+    x = x.copy()
+    x[1::2] -= 0.1  # some particles below the bottom wall
+    with pytest.raises(gpu.MpcdError) as e:
+        gpu.apply_bc_general(p, x, v.copy(), 1)
+    assert e.value.code == -3
+    # a mode nobody crosses is not an error (the reference throws only on a hit)
+    prm["boundary_bottom"] = "thermalize"
+    prm["boundary_top"] = "sticky"
+    p = to_struct(prm)
+    x2, v2, _, _ = case_state("small")
+    gpu.apply_bc_general(p, x2, v2, 1)
+
+
+def test_disallowed_zone_modes_raise(gpu):
+    prm = case_params("small")
+    prm["useZoneRedistribution"] = False
+    p = to_struct(prm)
+    x, v, _, _ = case_state("small")
+    with pytest.raises(gpu.MpcdError):
+        gpu.zone_pass(p, x, v, LAYOUT_BASE, 1)
+
+
+def test_medium_grid_step(gpu, oracle):
+    """256x128 cells, 20 per cell (655,360 particles), one full step."""
+    prm = poiseuille_params(Nx=256, Ny=128, gamma=20.0)
+    p = to_struct(prm)
+    from src_incompressible_openmp_amd.fileio import uniform_state
+
+    x, v, _, _ = uniform_state(p.n, p.Lx, p.Ly, seed=11)
+    xo, vo = x.copy(), v.copy()
+    so = oracle.step(CTR, p, xo, vo, 1, stages=True)
+    sim = gpu.Sim(p)
+    sim.set_state(x, v)
+    sg = sim.step_stages(1)
+    xg, vg = sim.get_state()
+    for a in ("ref", "base", "red"):
+        assert bits(sg["x_" + a], so["x_" + a]) and bits(sg["v_" + a], so["v_" + a]), a
+    assert bits(xg, xo) and vclose(vg, vo)
