@@ -123,6 +123,20 @@ typedef struct mpcd_step_metrics {
     mpcd_closure_metrics closure;
 } mpcd_step_metrics;
 
+/* Kernel classes timed by mpcd_gpu_sim_profile (events recorded on the
+ * simulation's stream around each launch group). */
+enum {
+    MPCD_PROF_STREAM_BC = 0,   /* force + stream + wall BC + shifted cell */
+    MPCD_PROF_BIN = 1,         /* counting-sort binnings (4 per step) */
+    MPCD_PROF_COLLIDE = 2,     /* SRD rotation + thermostat */
+    MPCD_PROF_CELL_FIELDS = 3, /* ref cell sums for the closure */
+    MPCD_PROF_ZONE_PASS = 4,   /* a whole zone pass (2 per step) */
+    MPCD_PROF_ZONE_KERNEL = 5, /* the zone kernel launch alone (2 per step) */
+    MPCD_PROF_CLOSURE = 6,     /* closure: fields, beta, stencil, kick */
+    MPCD_PROF_KICK = 7,        /* the closure's per-particle kick alone */
+    MPCD_PROF_CLASSES = 8
+};
+
 /* Opaque device-resident simulation (state, binnings, layouts). */
 typedef struct mpcd_gpu_sim mpcd_gpu_sim;
 
@@ -188,6 +202,12 @@ int mpcd_gpu_sim_check(mpcd_gpu_sim* s);
 int mpcd_gpu_sim_step_stages(mpcd_gpu_sim* s, int step, double* x_ref, double* v_ref, double* x_base, double* v_base,
                              double* x_red, double* v_red, mpcd_step_metrics* m);
 int mpcd_gpu_sim_synchronize(mpcd_gpu_sim* s);
+int mpcd_gpu_device_synchronize(void);
+/* enable (1) / disable (0) per-class kernel timing; resets the totals */
+int mpcd_gpu_sim_profile(mpcd_gpu_sim* s, int enable);
+/* total milliseconds and launch-group counts per MPCD_PROF_* class since the
+ * last reset (arrays of MPCD_PROF_CLASSES entries) */
+int mpcd_gpu_sim_profile_read(mpcd_gpu_sim* s, double* total_ms, int32_t* launches);
 /* the hipStream_t the simulation launches on */
 int mpcd_gpu_sim_stream(mpcd_gpu_sim* s, void** stream);
 

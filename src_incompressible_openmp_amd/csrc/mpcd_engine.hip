@@ -685,6 +685,49 @@ struct mpcd_gpu_sim {
     BCParams bc{};
     double xmax = 0, ymax = 0;
 
+    /* per-class kernel timing with events on this stream (MPCD_PROF_*) */
+    bool prof = false;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> ev[MPCD_PROF_CLASSES];
+    std::vector<hipEvent_t> evPool;
+    size_t evUsed = 0;
+    hipEvent_t take_event() {
+        if (evUsed == evPool.size()) {
+            hipEvent_t e;
+            HIP_OK(hipEventCreate(&e));
+            evPool.push_back(e);
+        }
+        return evPool[evUsed++];
+    }
+    void pbeg(int c) {
+        if (!prof) return;
+        hipEvent_t a = take_event();
+        HIP_OK(hipEventRecord(a, stream));
+        ev[c].push_back({a, nullptr});
+    }
+    void pend(int c) {
+        if (!prof) return;
+        hipEvent_t b = take_event();
+        HIP_OK(hipEventRecord(b, stream));
+        ev[c].back().second = b;
+    }
+    void prof_reset() {
+        for (auto& v : ev) v.clear();
+        evUsed = 0;
+    }
+    void prof_read(double* ms, int* count) {
+        HIP_OK(hipStreamSynchronize(stream));
+        for (int c = 0; c < MPCD_PROF_CLASSES; ++c) {
+            double t = 0.0;
+            for (const auto& pr : ev[c]) {
+                float f = 0.f;
+                HIP_OK(hipEventElapsedTime(&f, pr.first, pr.second));
+                t += (double)f;
+            }
+            ms[c] = t;
+            count[c] = (int)ev[c].size();
+        }
+    }
+
     explicit mpcd_gpu_sim(const mpcd_params& prm) : p(prm) {
         validate(p);
         g = make_geom(p);
@@ -763,6 +806,7 @@ struct mpcd_gpu_sim {
                         pdrive, cellTmp, mask, part, scal, err, wallPart, rects[0], rects[1], zout, zscratch};
         for (void* q : ptrs)
             if (q) (void)hipFree(q);
+        for (hipEvent_t e : evPool) (void)hipEventDestroy(e);
         if (stream) (void)hipStreamDestroy(stream);
     }
 
@@ -960,7 +1004,9 @@ struct mpcd_gpu_sim {
             a.gstride = bytes;
             lds = 0;
         }
+        pbeg(MPCD_PROF_ZONE_KERNEL);
         hipLaunchKernelGGL(k_zone, dim3(nZ), dim3(ZONE_TPB), lds, stream, a);
+        pend(MPCD_PROF_ZONE_KERNEL);
         check_launch();
         std::swap(vx, vx2);
         std::swap(vy, vy2);
@@ -1030,8 +1076,10 @@ struct mpcd_gpu_sim {
                                pdrive, scal, part);
             hipLaunchKernelGGL(k_reduce_parts, dim3(1), dim3(TPB), 0, stream, part, nbc, 3, scal + 16);
         }
+        pbeg(MPCD_PROF_KICK);
         hipLaunchKernelGGL(k_kick, dim3(nbp), dim3(TPB), 0, stream, g, x, y, vx, vy, duRX, duRY, duEX, duEY, scal,
                            cm ? part : (double*)nullptr);
+        pend(MPCD_PROF_KICK);
         check_launch();
         if (cm) {
             std::vector<double> h((size_t)nbp * 6);
@@ -1074,22 +1122,42 @@ struct mpcd_gpu_sim {
         const uint64_t us = (uint64_t)s;
         double sh[2];
         mpcd_draw_shift(130363ULL + 10007ULL * us, p.a0, &sh[0], &sh[1]);
+        pbeg(MPCD_PROF_STREAM_BC);
         stream_bc(104729ULL + 10007ULL * us, 1, sh, sm ? &sm->wall : nullptr);
+        pend(MPCD_PROF_STREAM_BC);
+        pbeg(MPCD_PROF_BIN);
         bin_cells();
+        pend(MPCD_PROF_BIN);
+        pbeg(MPCD_PROF_COLLIDE);
         collide(433494437ULL + 10007ULL * us);
+        pend(MPCD_PROF_COLLIDE);
+        pbeg(MPCD_PROF_BIN);
         bin_unshifted();
+        pend(MPCD_PROF_BIN);
         if (stages && stages[0]) get_state(stages[0], stages[1]);
+        pbeg(MPCD_PROF_CELL_FIELDS);
         ref_fields();
+        pend(MPCD_PROF_CELL_FIELDS);
         const uint64_t zs = 88172645463325252ULL + 10007ULL * us;
+        pbeg(MPCD_PROF_ZONE_PASS);
         layout_pass(0, zs);
+        pend(MPCD_PROF_ZONE_PASS);
         if (sm) pass_metrics(nz[0], &sm->base);
+        pbeg(MPCD_PROF_BIN);
         bin_unshifted();
+        pend(MPCD_PROF_BIN);
         if (stages && stages[2]) get_state(stages[2], stages[3]);
+        pbeg(MPCD_PROF_ZONE_PASS);
         layout_pass(1, zs + 4099ULL);
+        pend(MPCD_PROF_ZONE_PASS);
         if (sm) pass_metrics(nz[1], &sm->shifted);
+        pbeg(MPCD_PROF_BIN);
         bin_unshifted();
+        pend(MPCD_PROF_BIN);
         if (stages && stages[4]) get_state(stages[4], stages[5]);
+        pbeg(MPCD_PROF_CLOSURE);
         closure(sm ? &sm->closure : nullptr);
+        pend(MPCD_PROF_CLOSURE);
     }
 };
 
@@ -1188,6 +1256,22 @@ int mpcd_gpu_sim_step_stages(mpcd_gpu_sim* s, int step, double* x_ref, double* v
         s->step(step, m, st);
         s->check_err();
     });
+}
+
+int mpcd_gpu_device_synchronize(void) {
+    return guarded([&] { HIP_OK(hipDeviceSynchronize()); });
+}
+
+int mpcd_gpu_sim_profile(mpcd_gpu_sim* s, int enable) {
+    return guarded([&] {
+        HIP_OK(hipStreamSynchronize(s->stream));
+        s->prof_reset();
+        s->prof = enable != 0;
+    });
+}
+
+int mpcd_gpu_sim_profile_read(mpcd_gpu_sim* s, double* total_ms, int32_t* launches) {
+    return guarded([&] { s->prof_read(total_ms, launches); });
 }
 
 int mpcd_gpu_sim_synchronize(mpcd_gpu_sim* s) {

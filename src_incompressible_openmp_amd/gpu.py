@@ -113,6 +113,13 @@ def _check(rc: int) -> None:
         raise MpcdError(rc, load().mpcd_gpu_last_error().decode())
 
 
+PROF_CLASSES = ("stream_bc", "bin", "collide", "cell_fields", "zone_pass", "zone_kernel", "closure", "kick")
+
+
+def device_synchronize() -> None:
+    _check(load().mpcd_gpu_device_synchronize())
+
+
 def device_count() -> int:
     c = ctypes.c_int(0)
     _check(load().mpcd_gpu_device_count(ctypes.byref(c)))
@@ -247,6 +254,16 @@ class Sim:
 
     def synchronize(self) -> None:
         _check(load().mpcd_gpu_sim_synchronize(self._h))
+
+    def profile(self, enable: bool) -> None:
+        """Start (True) or stop per-class kernel timing; starting resets it."""
+        _check(load().mpcd_gpu_sim_profile(self._h, int(enable)))
+
+    def profile_read(self):
+        ms = np.zeros(len(PROF_CLASSES))
+        cnt = np.zeros(len(PROF_CLASSES), np.int32)
+        _check(load().mpcd_gpu_sim_profile_read(self._h, _ptr(ms), _ptr(cnt, ctypes.c_int32)))
+        return {k: (float(ms[i]), int(cnt[i])) for i, k in enumerate(PROF_CLASSES)}
 
     def stream(self) -> int:
         s = ctypes.c_void_p()
