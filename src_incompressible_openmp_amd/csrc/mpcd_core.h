@@ -313,12 +313,44 @@ MPCD_HD CellStats cell_stats(const int* ids, int m, const double* vx, const doub
     return s;
 }
 
-/* ---------- zone redistribution (zone_pass.cpp:62-973) ---------- */
+/* ---------- zone redistribution (zone_pass.cpp:62-973) ----------
+ *
+ * One zone is processed by one wavefront.  The reference's zone kernel is
+ * sequential: a shuffled list of dense cells, each of which mutates the
+ * counts its successors read.  That order is kept; inside each step the
+ * wavefront works together wherever the work is independent by construction:
+ * all counter-based draws of a step (shuffle partners, tie breaks, score
+ * noise, new positions), candidate evaluation and ranking, order-preserving
+ * compactions and prefix sums.  Only list bookkeeping runs on lane 0.
+ * The same code runs on the host with one lane (nl = 1), where the wave
+ * primitives below reduce to scalars and the barrier to nothing. */
+
+#if defined(__HIP_DEVICE_COMPILE__)
+#define ZBAR() __syncthreads()
+MPCD_HD unsigned long long wave_ballot(bool f) { return __ballot(f); }
+MPCD_HD int wave_popc(unsigned long long m) { return __popcll(m); }
+MPCD_HD int wave_shfl_up_add(int v, int lane) {
+    for (int o = 1; o < 64; o <<= 1) {
+        const int t = __shfl_up(v, o, 64);
+        if (lane >= o) v += t;
+    }
+    return v;
+}
+MPCD_HD int wave_bcast(int v, int src) { return __shfl(v, src, 64); }
+#else
+#define ZBAR() ((void)0)
+MPCD_HD unsigned long long wave_ballot(bool f) { return f ? 1ull : 0ull; }
+MPCD_HD int wave_popc(unsigned long long m) { return __builtin_popcountll(m); }
+MPCD_HD int wave_shfl_up_add(int v, int) { return v; }
+MPCD_HD int wave_bcast(int v, int) { return v; }
+#endif
+MPCD_HD unsigned long long lanes_below(int lane) { return (lane >= 64) ? ~0ull : ((1ull << lane) - 1ull); }
 
 struct ZoneParams {
     double highThr, lowThrBulk;
     double lowFrac, highFrac; /* 1/(1+coef), 1+coef */
     double gamma, fluidFracGain, lowThrFloor, highThrFloor;
+    double lowLoc1, highLoc1; /* empty-cell thresholds for fluid fraction 1 */
     int useLocal, maxPasses, corrEnabled;
     signed char binTab[25]; /* angular bin of (dx,dy) in [-2,2]^2: index (dy+2)*5+(dx+2) */
 };
@@ -327,9 +359,11 @@ struct ZoneParams {
 struct Cand {
     int lc, gc;
     double deficitNeg;
-    int cnt, cheb, dist2;
+    int cnt, cheb, dist2, bin;
     double tie;
 };
+
+constexpr int ZONE_MAX_MOVE = 64; /* movers whose draws are staged at once */
 
 /* Working set of one zone.  Cell arrays are indexed by the zone-local cell
  * lc = (ix-xs)*zh + (iy-ys) (ascending global cell id); particles by the
@@ -364,15 +398,24 @@ struct ZoneCtx {
     int* tmpg;   /* [off, off+P): donor shuffles */
     int* slotOf; /* [off, off+P): slot of each local index */
     /* zone cell arrays */
-    int *cnt, *head, *tail, *dl, *cntS, *colL, *colG;
-    double *lowA, *highA, *PbX, *PbY, *dU2;
+    int *cnt, *head, *tail, *dl, *cntS, *jd, *colL, *colG;
+    double *lowA, *highA, *dU2; /* lowA/highA only with local fluid fractions */
     unsigned char *bulk0, *recv, *bb, *skip;
-    /* candidate scratch */
-    Cand* cand;   /* 24 */
-    int* ord;     /* 24 */
-    int* recvT;   /* 48 */
-    double* dsc;  /* 8 donor scores */
-    int* dord;    /* 8 */
+    /* small scratch */
+    Cand* cand;     /* 24 */
+    int* ord;       /* 24 */
+    int* nbGc;      /* 24 */
+    int* rnk;       /* 24 */
+    int* recvT;     /* 48 */
+    double* dsc;    /* 8 */
+    int* dord;      /* 8 */
+    int* dord2;     /* 8 */
+    double* nsm4;   /* 4 */
+    double* mvx;    /* ZONE_MAX_MOVE */
+    double* mvy;    /* ZONE_MAX_MOVE */
+    int* mvk;       /* ZONE_MAX_MOVE */
+    /* shared scalars */
+    int m, nOrd, nr, nn, nd, startBin, need, stage;
     /* results */
     int nHigh, nLow;
     int movedDense, movedSparse;
@@ -412,23 +455,29 @@ MPCD_HD double nsm_at(const ZoneCtx& z, int gx, int gy, int use_copy) {
     return acc / dmax(ws, 1e-30);
 }
 
-/* Active thresholds of a zone cell from the current counts
- * (zone_pass.cpp:321-345 and :715-725). */
+/* Active thresholds of a zone cell (zone_pass.cpp:321-345, :715-725): the
+ * bulk flag is taken when the maps are built; with fluid fraction 1 the
+ * empty-cell values are constants, otherwise they are stored per cell. */
 MPCD_HD void zone_maps(ZoneCtx& z, int lc, int use_copy) {
-    double f = 1.0;
-    if (z.zp.useLocal) {
-        const double fRaw = nsm_at(z, lc_gx(z, lc), lc_gy(z, lc), use_copy) / dmax(z.zp.gamma, 1e-12);
-        const double t = z.zp.fluidFracGain * fRaw;
-        f = (t < 0.0) ? 0.0 : ((1.0 < t) ? 1.0 : t); /* std::clamp(t, 0, 1) */
-    }
-    const double gl = z.zp.gamma * f;
-    const double lowLoc = dmax(z.zp.lowFrac * gl, z.zp.lowThrFloor);
-    const double highLoc = dmax(z.zp.highFrac * gl, z.zp.highThrFloor);
     const int bulk = z.cnt[lc] > 0;
     z.bulk0[lc] = (unsigned char)bulk;
-    z.lowA[lc] = bulk ? z.zp.lowThrBulk : lowLoc;
-    z.highA[lc] = bulk ? z.zp.highThr : highLoc;
+    if (!z.zp.useLocal) return;
+    const double fRaw = nsm_at(z, lc_gx(z, lc), lc_gy(z, lc), use_copy) / dmax(z.zp.gamma, 1e-12);
+    const double t = z.zp.fluidFracGain * fRaw;
+    const double f = (t < 0.0) ? 0.0 : ((1.0 < t) ? 1.0 : t); /* std::clamp(t, 0, 1) */
+    const double gl = z.zp.gamma * f;
+    z.lowA[lc] = bulk ? z.zp.lowThrBulk : dmax(z.zp.lowFrac * gl, z.zp.lowThrFloor);
+    z.highA[lc] = bulk ? z.zp.highThr : dmax(z.zp.highFrac * gl, z.zp.highThrFloor);
 }
+MPCD_HD double low_thr(const ZoneCtx& z, int lc) {
+    if (z.zp.useLocal) return z.lowA[lc];
+    return z.bulk0[lc] ? z.zp.lowThrBulk : z.zp.lowLoc1;
+}
+MPCD_HD double high_thr(const ZoneCtx& z, int lc) {
+    if (z.zp.useLocal) return z.highA[lc];
+    return z.bulk0[lc] ? z.zp.highThr : z.zp.highLoc1;
+}
+MPCD_HD double thr(const ZoneCtx& z, int lc, int high) { return high ? high_thr(z, lc) : low_thr(z, lc); }
 
 /* zone_pass.cpp:229-244 */
 MPCD_HD void min_step(const ZoneCtx& z, int sx, int sy, int tx, int ty, int& ddx, int& ddy) {
@@ -475,8 +524,7 @@ MPCD_HD void move_to(ZoneCtx& z, int lc, int slot, double ux, double uy) {
     }
 }
 
-/* unique_stable of a touched list then the bulk/skip masks
- * (zone_pass.cpp:492-525). */
+/* the bulk/skip masks of a transfer (zone_pass.cpp:492-525) */
 MPCD_HD void transfer_masks(ZoneCtx& z, int src, const int* rv, int nr) {
     bool allBulk = z.bulk0[src] != 0;
     for (int k = 0; k < nr && allBulk; ++k)
@@ -503,80 +551,140 @@ MPCD_HD bool cand_less(const Cand& a, const Cand& b) {
     return a.tie < b.tie;
 }
 
-/* Admissible targets of a dense cell within Chebyshev range 2 inside the zone,
- * ranked into the round-robin order of rank_dense_targets_unified
- * (zone_pass.cpp:413-490).  Writes zone-local cells to z.ord, returns count. */
-MPCD_HD int dense_targets(ZoneCtx& z, int src, const double* target, int pass, int phase) {
-    const int sgx = lc_gx(z, src), sgy = lc_gy(z, src), sgc = lc_gc(z, src);
-    int m = 0;
-    for (int dy = -2; dy <= 2; ++dy)
-        for (int dx = -2; dx <= 2; ++dx) {
-            if (dx == 0 && dy == 0) continue;
-            int jx, jy;
-            if (!shift_idx(z, sgx, sgy, dx, dy, jx, jy)) continue;
-            const int lc = zone_lc(z, jx, jy);
-            if (lc < 0) continue;
-            if (!((double)z.cnt[lc] < target[lc])) continue;
-            bool dup = false;
-            for (int k = 0; k < m; ++k)
-                if (z.cand[k].lc == lc) dup = true;
-            if (dup) continue;
-            Cand& c = z.cand[m++];
-            c.lc = lc;
-            c.gc = jy + z.g.Ny * jx;
-        }
-    if (m == 0) return 0;
-    const int startBin = mpcd_draw_start_bin(z.seed, pass, phase, (uint32_t)sgc);
-    int bin[24];
-    int binCount[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    for (int k = 0; k < m; ++k) {
-        Cand& c = z.cand[k];
-        int ddx, ddy;
-        min_step(z, sgx, sgy, lc_gx(z, c.lc), lc_gy(z, c.lc), ddx, ddy);
-        bin[k] = z.zp.binTab[(ddy + 2) * 5 + (ddx + 2)] & 7;
-        binCount[bin[k]]++;
-        c.deficitNeg = -dmax(0.0, target[c.lc] - (double)z.cnt[c.lc]);
-        c.cnt = z.cnt[c.lc];
-        c.cheb = imax(ddx < 0 ? -ddx : ddx, ddy < 0 ? -ddy : ddy);
-        c.dist2 = ddx * ddx + ddy * ddy;
-        c.tie = mpcd_draw_tie(z.seed, pass, phase, (uint32_t)sgc, (uint32_t)c.gc);
+/* Order-preserving compaction of the zone cells satisfying pred into z.dl;
+ * all lanes; returns the count (same on every lane). */
+template <class F>
+MPCD_HD int zone_compact(ZoneCtx& z, int lane, int nl, F pred) {
+    int base = 0;
+    for (int b0 = 0; b0 < z.C; b0 += nl) {
+        const int lc = b0 + lane;
+        const bool f = lc < z.C && pred(lc);
+        const unsigned long long msk = wave_ballot(f);
+        if (f) z.dl[base + wave_popc(msk & lanes_below(lane))] = lc;
+        base += wave_popc(msk);
     }
-    /* stable per-bin order: bucket in admissible order, then insertion sort */
-    int off[8];
-    int acc = 0;
-    for (int b = 0; b < 8; ++b) {
-        off[b] = acc;
-        acc += binCount[b];
-    }
-    int fill[8];
-    for (int b = 0; b < 8; ++b) fill[b] = off[b];
-    int byBin[24];
-    for (int k = 0; k < m; ++k) byBin[fill[bin[k]]++] = k;
-    for (int b = 0; b < 8; ++b) {
-        int* a = byBin + off[b];
-        for (int i = 1; i < binCount[b]; ++i) {
-            const int key = a[i];
-            int j = i - 1;
-            while (j >= 0 && cand_less(z.cand[key], z.cand[a[j]])) {
-                a[j + 1] = a[j];
-                --j;
-            }
-            a[j + 1] = key;
-        }
-    }
-    int nOut = 0;
-    for (int r = 0; nOut < m; ++r)
-        for (int s = 0; s < 8; ++s) {
-            const int b = (startBin + s) % 8;
-            if (r < binCount[b]) z.ord[nOut++] = z.cand[byBin[off[b] + r]].lc;
-        }
-    return nOut;
+    return base;
 }
 
-/* spread_particles_from_dense_source_round_robin (zone_pass.cpp:527-589).
- * Appends receiving cells (unique, first-seen order) to rv. */
-MPCD_HD int spread(ZoneCtx& z, int src, int nOrd, const double* target, int needOut, int pass, int* rv, int& nr) {
+/* Fisher-Yates of a[0..m) with partner draws computed in parallel into jd;
+ * the swaps are applied in order by lane 0.  `draw(i)` gives j in [0, i]. */
+template <class F>
+MPCD_HD void zone_shuffle(ZoneCtx& z, int* a, int m, int lane, int nl, F draw) {
+    for (int i = 1 + lane; i < m; i += nl) z.jd[i] = (int)draw((uint32_t)i);
+    ZBAR();
+    if (lane == 0)
+        for (int i = m - 1; i >= 1; --i) {
+            const int j = z.jd[i];
+            const int t = a[i];
+            a[i] = a[j];
+            a[j] = t;
+        }
+    ZBAR();
+}
+
+/* Admissible targets of dense cell src within Chebyshev range 2 inside the
+ * zone, ranked into the round-robin order of rank_dense_targets_unified
+ * (zone_pass.cpp:413-490): 8 angular bins from a random start bin, each bin
+ * sorted by the candidate key, then one per bin per round.  All lanes; the
+ * order lands in z.ord[0..z.nOrd). */
+MPCD_HD void dense_targets(ZoneCtx& z, int src, int high, int pass, int phase, int lane, int nl) {
+    const int sgx = lc_gx(z, src), sgy = lc_gy(z, src), sgc = lc_gc(z, src);
+    /* 24 neighbour offsets in (dy outer, dx inner) order, centre skipped */
+    for (int o = lane; o < 24; o += nl) {
+        const int f = o < 12 ? o : o + 1;
+        const int dy = f / 5 - 2, dx = f % 5 - 2;
+        int jx, jy, gc = -1;
+        if (shift_idx(z, sgx, sgy, dx, dy, jx, jy)) {
+            const int lc = zone_lc(z, jx, jy);
+            if (lc >= 0 && (double)z.cnt[lc] < thr(z, lc, high)) gc = jy + z.g.Ny * jx;
+        }
+        z.nbGc[o] = gc;
+    }
+    ZBAR();
+    int base = 0;
+    for (int b0 = 0; b0 < 24; b0 += nl) {
+        const int o = b0 + lane;
+        bool keep = false;
+        int gc = -1;
+        if (o < 24) {
+            gc = z.nbGc[o];
+            keep = gc >= 0;
+            for (int q = 0; q < o && keep; ++q) /* duplicates only on grids narrower than 5 cells */
+                if (z.nbGc[q] == gc) keep = false;
+        }
+        const unsigned long long msk = wave_ballot(keep);
+        if (keep) {
+            Cand& c = z.cand[base + wave_popc(msk & lanes_below(lane))];
+            const int jx = gc / z.g.Ny, jy = gc - jx * z.g.Ny;
+            const int lc = zone_lc(z, jx, jy);
+            int ddx, ddy;
+            min_step(z, sgx, sgy, jx, jy, ddx, ddy);
+            c.lc = lc;
+            c.gc = gc;
+            c.bin = z.zp.binTab[(ddy + 2) * 5 + (ddx + 2)] & 7;
+            c.deficitNeg = -dmax(0.0, thr(z, lc, high) - (double)z.cnt[lc]);
+            c.cnt = z.cnt[lc];
+            c.cheb = imax(ddx < 0 ? -ddx : ddx, ddy < 0 ? -ddy : ddy);
+            c.dist2 = ddx * ddx + ddy * ddy;
+            c.tie = mpcd_draw_tie(z.seed, pass, phase, (uint32_t)sgc, (uint32_t)gc);
+        }
+        base += wave_popc(msk);
+    }
+    const int m = base;
+    if (lane == 0) {
+        z.nOrd = m;
+        z.startBin = m > 0 ? mpcd_draw_start_bin(z.seed, pass, phase, (uint32_t)sgc) : 0;
+    }
+    ZBAR();
+    if (m == 0) return;
+    /* rank inside the bin: stable order of the candidate key */
+    for (int i = lane; i < m; i += nl) {
+        const Cand ci = z.cand[i];
+        int r = 0;
+        for (int j = 0; j < m; ++j) {
+            if (j == i || z.cand[j].bin != ci.bin) continue;
+            if (cand_less(z.cand[j], ci) || (j < i && !cand_less(ci, z.cand[j]))) ++r;
+        }
+        z.rnk[i] = r;
+    }
+    ZBAR();
+    /* round robin position: earlier rounds, then earlier bins from startBin */
+    for (int i = lane; i < m; i += nl) {
+        const int ri = z.rnk[i];
+        const int si = (z.cand[i].bin - z.startBin + 8) & 7;
+        int pos = 0;
+        for (int j = 0; j < m; ++j) {
+            const int rj = z.rnk[j];
+            if (rj < ri || (rj == ri && ((z.cand[j].bin - z.startBin + 8) & 7) < si)) ++pos;
+        }
+        z.ord[pos] = z.cand[i].lc;
+    }
+    ZBAR();
+}
+
+/* New-position draws for the first `cnt` particles of a cell's list (the
+ * only ones a spread can move), staged in mvx/mvy; all lanes. */
+MPCD_HD void stage_dense_draws(ZoneCtx& z, int src, int cnt, int pass, int lane, int nl) {
+    if (lane == 0) {
+        int k = z.head[src];
+        for (int q = 0; q < cnt && k >= 0; ++q) {
+            z.mvk[q] = k;
+            k = z.next[z.off + k];
+        }
+    }
+    ZBAR();
+    for (int q = lane; q < cnt; q += nl) {
+        const int slot = z.slotOf[z.off + z.mvk[q]];
+        mpcd_draw_dense_pos(z.seed, pass, (uint32_t)z.gpid[slot], &z.mvx[q], &z.mvy[q]);
+    }
+    ZBAR();
+}
+
+/* spread_particles_from_dense_source_round_robin (zone_pass.cpp:527-589),
+ * lane 0.  `used` counts the staged movers consumed so far. */
+MPCD_HD int spread(ZoneCtx& z, int src, int high, int needOut, int pass, int* rv, int& nr, int& used) {
     int moved = 0;
+    const int nOrd = z.nOrd;
     if (needOut <= 0 || nOrd == 0) return 0;
     int avail = z.cnt[src];
     if (avail <= 0) return 0;
@@ -584,12 +692,13 @@ MPCD_HD int spread(ZoneCtx& z, int src, int nOrd, const double* target, int need
         bool progressed = false;
         for (int kk = 0; kk < nOrd; ++kk) {
             const int nb = z.ord[kk];
-            const int deficit = imax(0, (int)MPCD_FLOOR(target[nb] - (double)z.cnt[nb]));
+            const double tnb = thr(z, nb, high);
+            const int deficit = imax(0, (int)MPCD_FLOOR(tnb - (double)z.cnt[nb]));
             if (deficit <= 0) continue;
             int remaining = 0;
             for (int jj = kk; jj < nOrd; ++jj) {
                 const int q = z.ord[jj];
-                if (imax(0, (int)MPCD_FLOOR(target[q] - (double)z.cnt[q])) > 0) ++remaining;
+                if (imax(0, (int)MPCD_FLOOR(thr(z, q, high) - (double)z.cnt[q])) > 0) ++remaining;
             }
             const int quota = imax(1, (int)MPCD_CEIL((double)needOut / (double)imax(1, remaining)));
             const int nt = imax(1, imin(imin(quota, deficit), imin(needOut, avail)));
@@ -599,8 +708,14 @@ MPCD_HD int spread(ZoneCtx& z, int src, int nOrd, const double* target, int need
                 if (z.head[src] < 0) z.tail[src] = -1;
                 const int slot = z.slotOf[z.off + k];
                 double ux, uy;
-                mpcd_draw_dense_pos(z.seed, pass, (uint32_t)z.gpid[slot], &ux, &uy);
+                if (used < ZONE_MAX_MOVE) {
+                    ux = z.mvx[used];
+                    uy = z.mvy[used];
+                } else {
+                    mpcd_draw_dense_pos(z.seed, pass, (uint32_t)z.gpid[slot], &ux, &uy);
+                }
                 move_to(z, nb, slot, ux, uy);
+                ++used;
                 list_append(z, nb, k);
             }
             avail -= nt;
@@ -620,111 +735,132 @@ MPCD_HD int spread(ZoneCtx& z, int src, int nOrd, const double* target, int need
     return moved;
 }
 
-/* Dense passes (zone_pass.cpp:708-781), run by one lane. */
-MPCD_HD void zone_dense_passes(ZoneCtx& z) {
+/* Dense passes (zone_pass.cpp:708-781), all lanes. */
+MPCD_HD void zone_dense_passes(ZoneCtx& z, int lane, int nl) {
     for (int pass = 1; pass <= z.zp.maxPasses; ++pass) {
-        for (int lc = 0; lc < z.C; ++lc) zone_maps(z, lc, 0);
-        int m = 0;
-        for (int lc = 0; lc < z.C; ++lc)
-            if ((double)z.cnt[lc] > z.highA[lc]) z.dl[m++] = lc;
-        for (int i = m - 1; i >= 1; --i) {
-            const int j = (int)mpcd_draw_dense_shuffle(z.seed, pass, (uint32_t)i);
-            const int t = z.dl[i];
-            z.dl[i] = z.dl[j];
-            z.dl[j] = t;
-        }
+        for (int lc = lane; lc < z.C; lc += nl) zone_maps(z, lc, 0);
+        ZBAR();
+        const int m = zone_compact(z, lane, nl, [&](int lc) { return (double)z.cnt[lc] > high_thr(z, lc); });
+        zone_shuffle(z, z.dl, m, lane, nl, [&](uint32_t i) { return mpcd_draw_dense_shuffle(z.seed, pass, i); });
         for (int k = 0; k < m; ++k) {
             const int c = z.dl[k];
             const int nc = z.cnt[c];
             if (nc <= 0) continue;
-            const double hl = z.highA[c];
+            const double hl = high_thr(z, c);
             if ((double)nc <= hl) continue;
-            ++z.nHigh;
-            int nr = 0;
+            const int needOut = imax(0, (int)MPCD_CEIL((double)nc - hl));
             int moved = 0;
-            const int needOut = imax(0, (int)MPCD_CEIL((double)z.cnt[c] - hl));
+            if (lane == 0) {
+                ++z.nHigh;
+                z.nr = 0;
+            }
             if (needOut > 0) {
-                int no = dense_targets(z, c, z.lowA, pass, 0);
-                moved += spread(z, c, no, z.lowA, needOut, pass, z.recvT, nr);
+                /* the spreads move at most needOut particles, all from the
+                 * head of the list: stage their draws once */
+                const int nStage = imin(imin(needOut, nc), ZONE_MAX_MOVE);
+                stage_dense_draws(z, c, nStage, pass, lane, nl);
+                int used = 0;
+                dense_targets(z, c, 0, pass, 0, lane, nl);
+                if (lane == 0) {
+                    moved += spread(z, c, 0, needOut, pass, z.recvT, z.nr, used);
+                    z.stage = used;
+                }
+                ZBAR();
                 const int remaining = imax(0, (int)MPCD_CEIL((double)z.cnt[c] - hl));
                 if (remaining > 0) {
-                    no = dense_targets(z, c, z.highA, pass, 1);
-                    moved += spread(z, c, no, z.highA, remaining, pass, z.recvT, nr);
+                    used = z.stage;
+                    dense_targets(z, c, 1, pass, 1, lane, nl);
+                    if (lane == 0) moved += spread(z, c, 1, remaining, pass, z.recvT, z.nr, used);
                 }
             }
-            for (int r = 0; r < nr; ++r) z.recv[z.recvT[r]] = 1;
-            transfer_masks(z, c, z.recvT, nr);
-            z.movedDense += moved;
+            if (lane == 0) {
+                for (int r = 0; r < z.nr; ++r) z.recv[z.recvT[r]] = 1;
+                transfer_masks(z, c, z.recvT, z.nr);
+                z.movedDense += moved;
+            }
+            ZBAR();
         }
     }
 }
 
 /* inward_direction_from_local_gradient (zone_pass.cpp:375-411) on the
- * sparse-phase count copy. */
-MPCD_HD void inward_dir(const ZoneCtx& z, int gx, int gy, double& dxo, double& dyo) {
+ * sparse-phase count copy; the four smoothed counts in parallel. */
+MPCD_HD void inward_dir(ZoneCtx& z, int gx, int gy, int lane, int nl, double& dxo, double& dyo) {
     const int Nx = z.g.Nx, Ny = z.g.Ny;
-    if (z.g.perx) dxo = 0.5 * (nsm_at(z, wrapi(gx + 1, Nx), gy, 1) - nsm_at(z, wrapi(gx - 1, Nx), gy, 1));
-    else if (gx == 0) dxo = nsm_at(z, gx + 1, gy, 1) - nsm_at(z, gx, gy, 1);
-    else if (gx == Nx - 1) dxo = nsm_at(z, gx, gy, 1) - nsm_at(z, gx - 1, gy, 1);
-    else dxo = 0.5 * (nsm_at(z, gx + 1, gy, 1) - nsm_at(z, gx - 1, gy, 1));
-    if (z.g.pery) dyo = 0.5 * (nsm_at(z, gx, wrapi(gy + 1, Ny), 1) - nsm_at(z, gx, wrapi(gy - 1, Ny), 1));
-    else if (gy == 0) dyo = nsm_at(z, gx, gy + 1, 1) - nsm_at(z, gx, gy, 1);
-    else if (gy == Ny - 1) dyo = nsm_at(z, gx, gy, 1) - nsm_at(z, gx, gy - 1, 1);
-    else dyo = 0.5 * (nsm_at(z, gx, gy + 1, 1) - nsm_at(z, gx, gy - 1, 1));
+    /* x pair (xa - xb) * fx, y pair (ya - yb) * fy */
+    int xa, xb, ya, yb;
+    double fx, fy;
+    if (z.g.perx) { xa = wrapi(gx + 1, Nx); xb = wrapi(gx - 1, Nx); fx = 0.5; }
+    else if (gx == 0) { xa = gx + 1; xb = gx; fx = 1.0; }
+    else if (gx == Nx - 1) { xa = gx; xb = gx - 1; fx = 1.0; }
+    else { xa = gx + 1; xb = gx - 1; fx = 0.5; }
+    if (z.g.pery) { ya = wrapi(gy + 1, Ny); yb = wrapi(gy - 1, Ny); fy = 0.5; }
+    else if (gy == 0) { ya = gy + 1; yb = gy; fy = 1.0; }
+    else if (gy == Ny - 1) { ya = gy; yb = gy - 1; fy = 1.0; }
+    else { ya = gy + 1; yb = gy - 1; fy = 0.5; }
+    for (int t = lane; t < 4; t += nl) {
+        const int qx = (t == 0) ? xa : (t == 1) ? xb : gx;
+        const int qy = (t == 2) ? ya : (t == 3) ? yb : gy;
+        z.nsm4[t] = nsm_at(z, qx, qy, 1);
+    }
+    ZBAR();
+    dxo = (fx == 0.5) ? 0.5 * (z.nsm4[0] - z.nsm4[1]) : (z.nsm4[0] - z.nsm4[1]);
+    dyo = (fy == 0.5) ? 0.5 * (z.nsm4[2] - z.nsm4[3]) : (z.nsm4[2] - z.nsm4[3]);
 }
 
-/* Sparse pass (zone_pass.cpp:783-854), run by one lane. */
-MPCD_HD void zone_sparse_pass(ZoneCtx& z) {
-    for (int lc = 0; lc < z.C; ++lc) zone_maps(z, lc, 0);
-    for (int lc = 0; lc < z.C; ++lc) z.cntS[lc] = z.cnt[lc];
-    int m = 0;
-    for (int lc = 0; lc < z.C; ++lc)
-        if (z.cnt[lc] > 0 && (double)z.cnt[lc] < z.lowA[lc]) z.dl[m++] = lc;
-    for (int i = m - 1; i >= 1; --i) {
-        const int j = (int)mpcd_draw_low_shuffle(z.seed, (uint32_t)i);
-        const int t = z.dl[i];
-        z.dl[i] = z.dl[j];
-        z.dl[j] = t;
+/* Sparse pass (zone_pass.cpp:783-854), all lanes. */
+MPCD_HD void zone_sparse_pass(ZoneCtx& z, int lane, int nl) {
+    for (int lc = lane; lc < z.C; lc += nl) {
+        zone_maps(z, lc, 0);
+        z.cntS[lc] = z.cnt[lc];
     }
+    ZBAR();
+    const int m = zone_compact(z, lane, nl, [&](int lc) { return z.cnt[lc] > 0 && (double)z.cnt[lc] < low_thr(z, lc); });
+    zone_shuffle(z, z.dl, m, lane, nl, [&](uint32_t i) { return mpcd_draw_low_shuffle(z.seed, i); });
     const int lowInt = (int)MPCD_CEIL(z.zp.lowThrBulk);
     for (int k = 0; k < m; ++k) {
         const int c = z.dl[k];
         const int nc = z.cnt[c];
         if (nc <= 0) continue;
-        ++z.nLow;
+        if (lane == 0) ++z.nLow;
         const int cgx = lc_gx(z, c), cgy = lc_gy(z, c), cgc = lc_gc(z, c);
-        int neigh[8];
-        int nn = 0;
-        for (int dy = -1; dy <= 1; ++dy)
-            for (int dx = -1; dx <= 1; ++dx) {
-                if (dx == 0 && dy == 0) continue;
-                int jx, jy;
-                if (!shift_idx(z, cgx, cgy, dx, dy, jx, jy)) continue;
-                const int q = zone_lc(z, jx, jy);
-                if (q < 0) continue;
-                bool dup = false;
-                for (int t = 0; t < nn; ++t)
-                    if (neigh[t] == q) dup = true;
-                if (!dup) neigh[nn++] = q;
+        /* zone neighbours at range 1 (local_neighbors_in_zone :263-278) */
+        for (int o = lane; o < 8; o += nl) {
+            const int f = o < 4 ? o : o + 1;
+            const int dy = f / 3 - 1, dx = f % 3 - 1;
+            int jx, jy, q = -1;
+            if (shift_idx(z, cgx, cgy, dx, dy, jx, jy)) q = zone_lc(z, jx, jy);
+            z.nbGc[o] = q;
+        }
+        ZBAR();
+        int nn = 0, nd = 0;
+        for (int b0 = 0; b0 < 8; b0 += nl) {
+            const int o = b0 + lane;
+            bool keep = false;
+            if (o < 8) {
+                const int q = z.nbGc[o];
+                keep = q >= 0;
+                for (int t = 0; t < o && keep; ++t)
+                    if (z.nbGc[t] == q) keep = false;
             }
+            const unsigned long long mk = wave_ballot(keep);
+            nn += wave_popc(mk);
+            const bool don = keep && z.cnt[z.nbGc[o]] > lowInt;
+            const unsigned long long md = wave_ballot(don);
+            if (don) z.dord[nd + wave_popc(md & lanes_below(lane))] = z.nbGc[o];
+            nd += wave_popc(md);
+        }
+        ZBAR();
         if (nn == 0) continue;
         int need = imax(0, lowInt - nc);
-        if (need <= 0) continue;
-        int nd = 0;
-        for (int t = 0; t < nn; ++t)
-            if (z.cnt[neigh[t]] > lowInt) z.dord[nd++] = neigh[t];
-        if (nd == 0) continue;
+        if (need <= 0 || nd == 0) continue;
         double gxd, gyd;
-        inward_dir(z, cgx, cgy, gxd, gyd);
+        inward_dir(z, cgx, cgy, lane, nl, gxd, gyd);
         /* rank_admissible_neighbors_inward (zone_pass.cpp:636-667) */
-        for (int i = nd - 1; i >= 1; --i) {
-            const int j = (int)mpcd_draw_donor_shuffle(z.seed, (uint32_t)cgc, (uint32_t)i);
-            const int t = z.dord[i];
-            z.dord[i] = z.dord[j];
-            z.dord[j] = t;
-        }
+        zone_shuffle(z, z.dord, nd, lane, nl,
+                     [&](uint32_t i) { return mpcd_draw_donor_shuffle(z.seed, (uint32_t)cgc, i); });
         const double ndir = mpcd_hypot(gxd, gyd);
-        for (int t = 0; t < nd; ++t) {
+        for (int t = lane; t < nd; t += nl) {
             const int q = z.dord[t];
             int ddx, ddy;
             min_step(z, cgx, cgy, lc_gx(z, q), lc_gy(z, q), ddx, ddy);
@@ -734,20 +870,16 @@ MPCD_HD void zone_sparse_pass(ZoneCtx& z) {
             const double noise = 1e-9 * mpcd_draw_donor_noise(z.seed, (uint32_t)cgc, (uint32_t)lc_gc(z, q));
             z.dsc[t] = 100.0 * align + nsm_at(z, lc_gx(z, q), lc_gy(z, q), 1) + noise;
         }
-        for (int i = 1; i < nd; ++i) { /* stable, descending score */
-            const int kq = z.dord[i];
-            const double ks = z.dsc[i];
-            int j = i - 1;
-            while (j >= 0 && ks > z.dsc[j]) {
-                z.dord[j + 1] = z.dord[j];
-                z.dsc[j + 1] = z.dsc[j];
-                --j;
-            }
-            z.dord[j + 1] = kq;
-            z.dsc[j + 1] = ks;
+        ZBAR();
+        for (int t = lane; t < nd; t += nl) { /* stable, descending score */
+            int r = 0;
+            for (int u = 0; u < nd; ++u)
+                if (z.dsc[u] > z.dsc[t] || (u < t && !(z.dsc[t] > z.dsc[u]))) ++r;
+            z.dord2[r] = z.dord[t];
         }
+        ZBAR();
         for (int t = 0; t < nd; ++t) {
-            const int d = z.dord[t];
+            const int d = z.dord2[t];
             const int md = z.cnt[d];
             if (md <= 0) continue;
             const int avail = md - lowInt;
@@ -756,100 +888,125 @@ MPCD_HD void zone_sparse_pass(ZoneCtx& z) {
             if (nt <= 0) continue;
             /* copy the donor list, Fisher-Yates it, move the first nt */
             int* tmp = z.tmpg + z.off;
-            int k = z.head[d];
-            for (int q = 0; q < md; ++q) {
-                tmp[q] = k;
-                k = z.next[z.off + k];
+            if (lane == 0) {
+                int kk = z.head[d];
+                for (int q = 0; q < md; ++q) {
+                    tmp[q] = kk;
+                    kk = z.next[z.off + kk];
+                }
             }
-            for (int i = md - 1; i >= 1; --i) {
-                const int j = (int)mpcd_draw_particle_shuffle(z.seed, (uint32_t)cgc, (uint32_t)lc_gc(z, d), (uint32_t)i);
-                const int tv = tmp[i];
-                tmp[i] = tmp[j];
-                tmp[j] = tv;
-            }
-            for (int q = 0; q < nt; ++q) {
+            const int dgc = lc_gc(z, d);
+            zone_shuffle(z, tmp, md, lane, nl,
+                         [&](uint32_t i) { return mpcd_draw_particle_shuffle(z.seed, (uint32_t)cgc, (uint32_t)dgc, i); });
+            for (int q = lane; q < nt && q < ZONE_MAX_MOVE; q += nl) {
                 const int slot = z.slotOf[z.off + tmp[q]];
-                double ux, uy;
-                mpcd_draw_sparse_pos(z.seed, (uint32_t)z.gpid[slot], &ux, &uy);
-                move_to(z, c, slot, ux, uy);
-                list_append(z, c, tmp[q]);
+                mpcd_draw_sparse_pos(z.seed, (uint32_t)z.gpid[slot], &z.mvx[q], &z.mvy[q]);
             }
-            z.head[d] = -1;
-            z.tail[d] = -1;
-            for (int q = nt; q < md; ++q) list_append(z, d, tmp[q]);
-            z.cnt[d] = md - nt;
-            z.cnt[c] += nt;
-            z.recv[c] = 1;
-            int one[1] = {c};
-            transfer_masks(z, d, one, 1);
-            z.movedSparse += nt;
+            ZBAR();
+            if (lane == 0) {
+                for (int q = 0; q < nt; ++q) {
+                    const int slot = z.slotOf[z.off + tmp[q]];
+                    double ux, uy;
+                    if (q < ZONE_MAX_MOVE) {
+                        ux = z.mvx[q];
+                        uy = z.mvy[q];
+                    } else {
+                        mpcd_draw_sparse_pos(z.seed, (uint32_t)z.gpid[slot], &ux, &uy);
+                    }
+                    move_to(z, c, slot, ux, uy);
+                    list_append(z, c, tmp[q]);
+                }
+                z.head[d] = -1;
+                z.tail[d] = -1;
+                for (int q = nt; q < md; ++q) list_append(z, d, tmp[q]);
+                z.cnt[d] = md - nt;
+                z.cnt[c] += nt;
+                z.recv[c] = 1;
+                int one[1] = {c};
+                transfer_masks(z, d, one, 1);
+                z.movedSparse += nt;
+            }
+            ZBAR();
             need -= nt;
             if (need <= 0) break;
         }
     }
 }
 
-/* Zone setup (zone_pass.cpp:680-685).  Lane 0 first: counts, local list
- * heads and the column map; then zone_setup_cell per cell on every lane:
- * slots, links and the momentum before the pass. */
-MPCD_HD void zone_setup_serial(ZoneCtx& z) {
-    int accl = 0;
-    for (int col = 0; col < z.zw; ++col) {
-        const int gx = z.xs + col;
-        z.colL[col] = accl;
-        z.colG[col] = z.gstart[z.ys + z.g.Ny * gx];
-        for (int r = 0; r < z.zh; ++r) {
-            const int lc = col * z.zh + r;
-            const int m = z.gcnt[(z.ys + r) + z.g.Ny * gx];
+/* Zone setup (zone_pass.cpp:680-685), all lanes: counts, list heads from a
+ * wave prefix sum over the zone's cells, the column map, then per cell the
+ * slots and links. */
+MPCD_HD void zone_setup(ZoneCtx& z, int lane, int nl) {
+    int base = 0;
+    for (int b0 = 0; b0 < z.C; b0 += nl) {
+        const int lc = b0 + lane;
+        const int m = lc < z.C ? z.gcnt[lc_gc(z, lc)] : 0;
+        const int incl = wave_shfl_up_add(m, lane);
+        if (lc < z.C) {
+            const int k0 = base + incl - m;
             z.cnt[lc] = m;
-            z.head[lc] = m > 0 ? accl : -1;
-            z.tail[lc] = m > 0 ? accl + m - 1 : -1;
-            accl += m;
+            z.head[lc] = m > 0 ? k0 : -1;
+            z.tail[lc] = m > 0 ? k0 + m - 1 : -1;
+            z.dl[lc] = k0; /* first local index of the cell */
+            z.recv[lc] = 0;
+            z.bb[lc] = 0;
+            z.skip[lc] = 0;
+            z.dU2[lc] = 0.0;
+            if (lc % z.zh == 0) {
+                z.colL[lc / z.zh] = k0;
+                z.colG[lc / z.zh] = z.gstart[lc_gc(z, lc)];
+            }
+        }
+        /* lanes past the zone add 0, so the last lane holds the chunk total */
+        const int chunk = wave_bcast(incl, nl - 1);
+        base += chunk;
+    }
+    if (lane == 0) {
+        z.P = base;
+        z.nHigh = 0;
+        z.nLow = 0;
+        z.movedDense = 0;
+        z.movedSparse = 0;
+        z.stats4 = 0.0;
+        z.stats5 = 0.0;
+        z.status = 0;
+    }
+}
+
+MPCD_HD void zone_setup_links(ZoneCtx& z, int lane, int nl) {
+    for (int lc = lane; lc < z.C; lc += nl) {
+        const int gc = lc_gc(z, lc);
+        const int s0 = z.gstart[gc], m = z.gcnt[gc];
+        const int k0 = z.dl[lc];
+        for (int j = 0; j < m; ++j) {
+            z.slotOf[z.off + k0 + j] = z.glist[s0 + j];
+            z.next[z.off + k0 + j] = (j + 1 < m) ? k0 + j + 1 : -1;
         }
     }
-    z.P = accl;
-    z.nHigh = 0;
-    z.nLow = 0;
-    z.movedDense = 0;
-    z.movedSparse = 0;
-    z.stats4 = 0.0;
-    z.stats5 = 0.0;
-    z.status = 0;
 }
 
-MPCD_HD void zone_setup_cell(ZoneCtx& z, int lc) {
-    const int col = lc / z.zh;
-    const int gc = lc_gc(z, lc);
-    const int s0 = z.gstart[gc], m = z.gcnt[gc];
-    const int k0 = z.colL[col] + (s0 - z.colG[col]);
-    double px = 0.0, py = 0.0;
-    for (int j = 0; j < m; ++j) {
-        const int slot = z.glist[s0 + j];
-        px += z.vxin[slot];
-        py += z.vyin[slot];
-        z.slotOf[z.off + k0 + j] = slot;
-        z.next[z.off + k0 + j] = (j + 1 < m) ? k0 + j + 1 : -1;
-    }
-    z.PbX[lc] = px;
-    z.PbY[lc] = py;
-    z.recv[lc] = 0;
-    z.bb[lc] = 0;
-    z.skip[lc] = 0;
-    z.dU2[lc] = 0.0;
-}
-
-/* Momentum correction of one zone cell (zone_pass.cpp:879-909); writes the
- * output velocity of every particle of the cell this zone owns. */
+/* Momentum correction of one zone cell (zone_pass.cpp:870-909); writes the
+ * output velocity of every particle of the cell this zone owns.  The
+ * momentum before is the sum over the cell's input run in particle order
+ * (the input velocities are never written during the pass). */
 MPCD_HD void zone_correct_cell(ZoneCtx& z, int lc) {
     const int m = z.cnt[lc];
     if (m == 0) return;
+    const int gc = lc_gc(z, lc);
+    const int s0 = z.gstart[gc], m0 = z.gcnt[gc];
+    double pbx = 0.0, pby = 0.0;
+    for (int j = 0; j < m0; ++j) {
+        const int slot = z.glist[s0 + j];
+        pbx += z.vxin[slot];
+        pby += z.vyin[slot];
+    }
     double px = 0.0, py = 0.0;
     for (int k = z.head[lc]; k >= 0; k = z.next[z.off + k]) {
         const int slot = z.slotOf[z.off + k];
         px += z.vxin[slot];
         py += z.vyin[slot];
     }
-    const double dPx = z.PbX[lc] - px, dPy = z.PbY[lc] - py;
+    const double dPx = pbx - px, dPy = pby - py;
     const double dux = dPx / (double)m, duy = dPy / (double)m;
     z.dU2[lc] = dux * dux + duy * duy;
     bool corr = z.zp.corrEnabled != 0;
@@ -896,32 +1053,35 @@ struct ZoneOut {
     double stats4, stats5;
 };
 
-
-
-/* bytes of the per-zone working set for `C` cells and `W` columns */
-MPCD_HD size_t zone_bytes(int C, int W) {
+/* bytes of the per-zone working set for C cells and W columns */
+MPCD_HD size_t zone_bytes(int C, int W, int useLocal) {
     size_t b = 0;
-    b += (size_t)C * (5 * sizeof(double));          /* lowA highA PbX PbY dU2 */
-    b += (size_t)C * (5 * sizeof(int));             /* cnt head tail dl cntS */
-    b += (size_t)W * (2 * sizeof(int));             /* colL colG */
-    b += (size_t)C * 4;                             /* bulk0 recv bb skip */
+    b += (size_t)C * sizeof(double) * (useLocal ? 3 : 1); /* dU2 (+ lowA highA) */
+    b += (size_t)C * (6 * sizeof(int));                    /* cnt head tail dl cntS jd */
+    b += (size_t)W * (2 * sizeof(int));                    /* colL colG */
+    b += (size_t)C * 4;                                    /* bulk0 recv bb skip */
     b = (b + 15) & ~(size_t)15;
-    b += 24 * sizeof(Cand) + 24 * sizeof(int) + 48 * sizeof(int) + 8 * sizeof(double) + 8 * sizeof(int);
+    b += 24 * sizeof(Cand) + 24 * 4 * sizeof(int) + 48 * sizeof(int) + 8 * sizeof(double) + 3 * 8 * sizeof(int) +
+         4 * sizeof(double) + ZONE_MAX_MOVE * (2 * sizeof(double) + sizeof(int));
     return (b + 15) & ~(size_t)15;
 }
 
-MPCD_HD void zone_carve(ZoneCtx& z, unsigned char* base, int C, int W) {
+MPCD_HD void zone_carve(ZoneCtx& z, unsigned char* base, int C, int W, int useLocal) {
     unsigned char* p = base;
-    z.lowA = (double*)p; p += (size_t)C * sizeof(double);
-    z.highA = (double*)p; p += (size_t)C * sizeof(double);
-    z.PbX = (double*)p; p += (size_t)C * sizeof(double);
-    z.PbY = (double*)p; p += (size_t)C * sizeof(double);
     z.dU2 = (double*)p; p += (size_t)C * sizeof(double);
+    if (useLocal) {
+        z.lowA = (double*)p; p += (size_t)C * sizeof(double);
+        z.highA = (double*)p; p += (size_t)C * sizeof(double);
+    } else {
+        z.lowA = nullptr;
+        z.highA = nullptr;
+    }
     z.cnt = (int*)p; p += (size_t)C * sizeof(int);
     z.head = (int*)p; p += (size_t)C * sizeof(int);
     z.tail = (int*)p; p += (size_t)C * sizeof(int);
     z.dl = (int*)p; p += (size_t)C * sizeof(int);
     z.cntS = (int*)p; p += (size_t)C * sizeof(int);
+    z.jd = (int*)p; p += (size_t)C * sizeof(int);
     z.colL = (int*)p; p += (size_t)W * sizeof(int);
     z.colG = (int*)p; p += (size_t)W * sizeof(int);
     z.bulk0 = p; p += C;
@@ -931,10 +1091,18 @@ MPCD_HD void zone_carve(ZoneCtx& z, unsigned char* base, int C, int W) {
     p = base + ((size_t)(p - base + 15) & ~(size_t)15);
     z.cand = (Cand*)p; p += 24 * sizeof(Cand);
     z.ord = (int*)p; p += 24 * sizeof(int);
+    z.nbGc = (int*)p; p += 24 * sizeof(int);
+    z.rnk = (int*)p; p += 24 * sizeof(int);
+    p += 24 * sizeof(int);
     z.recvT = (int*)p; p += 48 * sizeof(int);
     z.dsc = (double*)p; p += 8 * sizeof(double);
-    z.dord = (int*)p;
+    z.dord = (int*)p; p += 8 * sizeof(int);
+    z.dord2 = (int*)p; p += 8 * sizeof(int);
+    p += 8 * sizeof(int);
+    z.nsm4 = (double*)p; p += 4 * sizeof(double);
+    z.mvx = (double*)p; p += ZONE_MAX_MOVE * sizeof(double);
+    z.mvy = (double*)p; p += ZONE_MAX_MOVE * sizeof(double);
+    z.mvk = (int*)p;
 }
-
 
 }  // namespace mpcdk

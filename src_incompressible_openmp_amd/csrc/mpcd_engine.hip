@@ -375,10 +375,11 @@ __global__ void __launch_bounds__(ZONE_TPB) k_zone(ZoneArgs a) {
     extern __shared__ __align__(16) unsigned char lds[];
     const int zi = blockIdx.x;
     if (zi >= a.nZones) return;
-    const ZoneRect r = a.rects[zi];
+    const int lane = threadIdx.x, nl = ZONE_TPB;
     __shared__ ZoneCtx z;
     __shared__ int ok;
-    if (threadIdx.x == 0) {
+    if (lane == 0) {
+        const ZoneRect r = a.rects[zi];
         z.g = a.g;
         z.zp = a.zp;
         z.xs = r.xs;
@@ -407,23 +408,25 @@ __global__ void __launch_bounds__(ZONE_TPB) k_zone(ZoneArgs a) {
         z.tmpg = a.tmpg;
         z.slotOf = a.slotOf;
         unsigned char* base = a.gscratch ? a.gscratch + (size_t)zi * a.gstride : lds;
-        zone_carve(z, base, z.C, z.zw);
-        zone_setup_serial(z);
+        zone_carve(z, base, z.C, z.zw, a.zp.useLocal);
+    }
+    __syncthreads();
+    zone_setup(z, lane, nl);
+    __syncthreads();
+    if (lane == 0) {
         ok = ((long long)z.off + (long long)z.P <= a.scratchCap) ? 1 : 0;
         if (!ok) atomicOr(a.err, 2);
     }
     __syncthreads();
     if (!ok) return; /* uniform across the workgroup */
-    for (int lc = threadIdx.x; lc < z.C; lc += blockDim.x) zone_setup_cell(z, lc);
+    zone_setup_links(z, lane, nl);
     __syncthreads();
-    if (threadIdx.x == 0) {
-        zone_dense_passes(z);
-        zone_sparse_pass(z);
-    }
+    zone_dense_passes(z, lane, nl);
+    zone_sparse_pass(z, lane, nl);
     __syncthreads();
-    for (int lc = threadIdx.x; lc < z.C; lc += blockDim.x) zone_correct_cell(z, lc);
+    for (int lc = lane; lc < z.C; lc += nl) zone_correct_cell(z, lc);
     __syncthreads();
-    if (threadIdx.x == 0) {
+    if (lane == 0) {
         zone_final_stats(z);
         ZoneOut o;
         o.nHigh = z.nHigh;
@@ -991,7 +994,7 @@ struct mpcd_gpu_sim {
         a.slotOf = slotOf;
         a.out = zout;
         a.err = err;
-        const size_t bytes = zone_bytes(maxC, maxW);
+        const size_t bytes = zone_bytes(maxC, maxW, zp.useLocal);
         size_t lds = bytes;
         if (bytes > (size_t)ZONE_LDS_CAP) {
             const size_t need = bytes * (size_t)nZ;

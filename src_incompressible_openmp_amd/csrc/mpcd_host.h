@@ -145,6 +145,12 @@ inline ZoneParams make_zone_params(const mpcd_params& p) {
     zp.useLocal = p.useLocalFluidFractionThresholds;
     zp.maxPasses = p.maxRedistribPasses;
     zp.corrEnabled = p.enableMomentumCorrectionPostRedistribution;
+    { /* empty-cell thresholds at fluid fraction 1 (zone_pass.cpp:334-342) */
+        const double gl = p.gamma * 1.0;
+        const double lo = zp.lowFrac * gl, hi = zp.highFrac * gl;
+        zp.lowLoc1 = (lo < p.lowThrFloor) ? p.lowThrFloor : lo;
+        zp.highLoc1 = (hi < p.highThrFloor) ? p.highThrFloor : hi;
+    }
     /* angular bins of rank_dense_targets_unified (zone_pass.cpp:452-454) */
     for (int dy = -2; dy <= 2; ++dy)
         for (int dx = -2; dx <= 2; ++dx) {

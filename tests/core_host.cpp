@@ -151,13 +151,13 @@ extern "C" int core_zone_pass(const mpcd_params* p, double* xa, double* va, int 
         z.next = next.data();
         z.tmpg = tmpg.data();
         z.slotOf = slotOf.data();
-        std::vector<unsigned char> buf(zone_bytes(z.C, z.zw) + 16);
-        zone_carve(z, buf.data(), z.C, z.zw);
-        zone_setup_serial(z);
+        std::vector<unsigned char> buf(zone_bytes(z.C, z.zw, zp.useLocal) + 16);
+        zone_carve(z, buf.data(), z.C, z.zw, zp.useLocal);
+        zone_setup(z, 0, 1);
         if (z.P != off[zi + 1] - off[zi]) return -1;
-        for (int lc = 0; lc < z.C; ++lc) zone_setup_cell(z, lc);
-        zone_dense_passes(z);
-        zone_sparse_pass(z);
+        zone_setup_links(z, 0, 1);
+        zone_dense_passes(z, 0, 1);
+        zone_sparse_pass(z, 0, 1);
         for (int lc = 0; lc < z.C; ++lc) zone_correct_cell(z, lc);
         zone_final_stats(z);
         md += z.movedDense;
