@@ -126,8 +126,9 @@ __device__ void wall_block_reduce(WallAcc a, WallAcc* out) {
 __global__ void __launch_bounds__(TPB) k_stream_bc(Geom g, BCParams b, uint64_t seed, int do_stream, double* __restrict__ x,
                                                    double* __restrict__ y, double* __restrict__ vx,
                                                    double* __restrict__ vy, const int* __restrict__ pid,
-                                                   int* __restrict__ cid, double sx, double sy, double xmax,
-                                                   double ymax, WallAcc* __restrict__ wall, int* __restrict__ err) {
+                                                   int* __restrict__ cid, int* __restrict__ cidU, double sx,
+                                                   double sy, double xmax, double ymax, WallAcc* __restrict__ wall,
+                                                   int* __restrict__ err) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     WallAcc acc;
     wall_acc_zero(acc);
@@ -140,6 +141,7 @@ __global__ void __launch_bounds__(TPB) k_stream_bc(Geom g, BCParams b, uint64_t 
         vx[i] = pvx;
         vy[i] = pvy;
         if (cid) cid[i] = shifted_cell_of(g, px, py, sx, sy, xmax, ymax);
+        if (cidU) cidU[i] = cell_of(g, px, py);
     }
     if (wall) wall_block_reduce(acc, wall);
 }
@@ -157,9 +159,21 @@ __global__ void k_cells(Geom g, const double* __restrict__ x, const double* __re
 
 /* ---- counting sort ---- */
 
-__global__ void k_hist(const int* __restrict__ cid, int n, int* __restrict__ cnt) {
+/* Cell histogram.  Lanes of a wavefront that hit the same cell add once:
+ * the state is kept sorted by cell, so a wavefront covers a few cells. */
+__global__ void __launch_bounds__(TPB) k_hist(const int* __restrict__ cid, int n, int* __restrict__ cnt) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) atomicAdd(&cnt[cid[i]], 1);
+    const int lane = __lane_id();
+    const int c = (i < n) ? cid[i] : -1;
+    bool active = c >= 0;
+    for (unsigned long long act = __ballot(active); act; act = __ballot(active)) {
+        const int leader = __ffsll((long long)act) - 1;
+        const int lc = __shfl(c, leader, 64);
+        const bool mine = active && c == lc;
+        const unsigned long long grp = __ballot(mine);
+        if (lane == leader) atomicAdd(&cnt[lc], __popcll(grp));
+        if (mine) active = false;
+    }
 }
 
 /* exclusive scan, level 1: each block scans SCAN_TILE entries */
@@ -219,29 +233,74 @@ __global__ void k_scan_add(int* __restrict__ out, int m, const int* __restrict__
     if (i < m) out[i] += tileSum[i / SCAN_TILE];
 }
 
-__global__ void k_scatter(const int* __restrict__ cid, int n, const int* __restrict__ start, int* __restrict__ cursor,
-                          int* __restrict__ list) {
+/* Scatter slots into their cell's run (one atomic per cell per wavefront). */
+__global__ void __launch_bounds__(TPB) k_scatter(const int* __restrict__ cid, int n, const int* __restrict__ start,
+                                                 int* __restrict__ cursor, int* __restrict__ list) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) {
-        const int c = cid[i];
-        list[start[c] + atomicAdd(&cursor[c], 1)] = i;
+    const int lane = __lane_id();
+    const int c = (i < n) ? cid[i] : -1;
+    bool active = c >= 0;
+    for (unsigned long long act = __ballot(active); act; act = __ballot(active)) {
+        const int leader = __ffsll((long long)act) - 1;
+        const int lc = __shfl(c, leader, 64);
+        const bool mine = active && c == lc;
+        const unsigned long long grp = __ballot(mine);
+        int base = 0;
+        if (lane == leader) base = atomicAdd(&cursor[lc], __popcll(grp));
+        base = __shfl(base, leader, 64);
+        if (mine) {
+            list[start[c] + base + __popcll(grp & ((1ull << lane) - 1ull))] = i;
+            active = false;
+        }
     }
 }
 
-/* order each cell's run by pid (runs are short: insertion sort) */
-__global__ void k_segsort(const int* __restrict__ start, int Nc, int* __restrict__ list, const int* __restrict__ pid) {
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= Nc) return;
+/* Order each cell's run by pid.  A block takes SEG_CELLS consecutive cells,
+ * whose runs are one contiguous range of the list, stages (pid, slot) keys
+ * in LDS and insertion-sorts each run (runs are short and nearly sorted);
+ * a range too large for LDS is sorted in place in global memory. */
+constexpr int SEG_CELLS = 256;
+constexpr int SEG_CAP = 6144;
+__global__ void __launch_bounds__(SEG_CELLS) k_segsort(const int* __restrict__ start, int Nc, int* __restrict__ list,
+                                                       const int* __restrict__ pid) {
+    __shared__ unsigned long long key[SEG_CAP];
+    const int c0 = blockIdx.x * SEG_CELLS;
+    const int c1 = min(c0 + SEG_CELLS, Nc);
+    const int b0 = start[c0], b1 = start[c1];
+    const int c = c0 + (int)threadIdx.x;
+    if (b1 - b0 <= SEG_CAP) {
+        for (int q = b0 + (int)threadIdx.x; q < b1; q += SEG_CELLS) {
+            const int slot = list[q];
+            key[q - b0] = ((unsigned long long)(unsigned)pid[slot] << 32) | (unsigned)slot;
+        }
+        __syncthreads();
+        if (c < c1) {
+            const int b = start[c] - b0, e = start[c + 1] - b0;
+            for (int i = b + 1; i < e; ++i) {
+                const unsigned long long kv = key[i];
+                int j = i - 1;
+                while (j >= b && key[j] > kv) {
+                    key[j + 1] = key[j];
+                    --j;
+                }
+                key[j + 1] = kv;
+            }
+        }
+        __syncthreads();
+        for (int q = b0 + (int)threadIdx.x; q < b1; q += SEG_CELLS) list[q] = (int)(unsigned)(key[q - b0] & 0xffffffffull);
+        return;
+    }
+    if (c >= c1) return;
     const int b = start[c], e = start[c + 1];
     for (int i = b + 1; i < e; ++i) {
-        const int key = list[i];
-        const int kp = pid[key];
+        const int kslot = list[i];
+        const int kp = pid[kslot];
         int j = i - 1;
         while (j >= b && pid[list[j]] > kp) {
             list[j + 1] = list[j];
             --j;
         }
-        list[j + 1] = key;
+        list[j + 1] = kslot;
     }
 }
 
@@ -543,26 +602,38 @@ __global__ void k_eos(Geom g, const double* __restrict__ p, double dtOverRho, do
 }
 
 /* v = (v + beta*duRepair[c]) + duEOS[c] for the particle's (red) cell
- * (apply_cell_kick twice, liquid_closure.cpp:379-436).  Optional per-block
- * partials: sum |kR|^2, max |kR|, sum |kE|^2, max |kE|, sum dvx, sum dvy. */
-__global__ void __launch_bounds__(TPB) k_kick(Geom g, const double* __restrict__ x, const double* __restrict__ y,
-                                              double* __restrict__ vx, double* __restrict__ vy,
-                                              const double* __restrict__ duRX, const double* __restrict__ duRY,
-                                              const double* __restrict__ duEX, const double* __restrict__ duEY,
-                                              const double* __restrict__ scal, double* __restrict__ part) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+ * (apply_cell_kick twice, liquid_closure.cpp:379-436), fused with a reorder
+ * of the state into the order of the current (red) binning: position s of
+ * the list becomes slot s, so the next step starts with particles sorted by
+ * cell and every per-cell gather is local.  Optional per-block partials:
+ * sum |kR|^2, max |kR|, sum |kE|^2, max |kE|, sum dvx, sum dvy. */
+__global__ void __launch_bounds__(TPB) k_kick_reorder(Geom g, const int* __restrict__ list,
+                                                      const double* __restrict__ x, const double* __restrict__ y,
+                                                      const double* __restrict__ vx, const double* __restrict__ vy,
+                                                      const int* __restrict__ pid, double* __restrict__ xo,
+                                                      double* __restrict__ yo, double* __restrict__ vxo,
+                                                      double* __restrict__ vyo, int* __restrict__ pido,
+                                                      const double* __restrict__ duRX, const double* __restrict__ duRY,
+                                                      const double* __restrict__ duEX, const double* __restrict__ duEY,
+                                                      const double* __restrict__ scal, double* __restrict__ part) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
     const double beta = scal[3];
     double r2 = 0.0, rm = 0.0, e2 = 0.0, em = 0.0, dvx = 0.0, dvy = 0.0;
-    if (i < g.n) {
-        const int c = cell_of(g, x[i], y[i]);
+    if (s < g.n) {
+        const int i = list[s];
+        const double px = x[i], py = y[i];
+        const int c = cell_of(g, px, py);
         const double kx = beta * duRX[c], ky = beta * duRY[c];
         const double v0x = vx[i], v0y = vy[i];
         double nx = v0x + kx, ny = v0y + ky;
         const double ex = duEX[c], ey = duEY[c];
         nx += ex;
         ny += ey;
-        vx[i] = nx;
-        vy[i] = ny;
+        xo[s] = px;
+        yo[s] = py;
+        vxo[s] = nx;
+        vyo[s] = ny;
+        pido[s] = pid[i];
         if (part) {
             const double kr = MPCD_SQRT(kx * kx + ky * ky);
             const double ke = MPCD_SQRT(ex * ex + ey * ey);
@@ -575,27 +646,27 @@ __global__ void __launch_bounds__(TPB) k_kick(Geom g, const double* __restrict__
         }
     }
     if (!part) return;
-    __shared__ double s[6][TPB];
-    s[0][threadIdx.x] = r2;
-    s[1][threadIdx.x] = rm;
-    s[2][threadIdx.x] = e2;
-    s[3][threadIdx.x] = em;
-    s[4][threadIdx.x] = dvx;
-    s[5][threadIdx.x] = dvy;
+    __shared__ double sh[6][TPB];
+    sh[0][threadIdx.x] = r2;
+    sh[1][threadIdx.x] = rm;
+    sh[2][threadIdx.x] = e2;
+    sh[3][threadIdx.x] = em;
+    sh[4][threadIdx.x] = dvx;
+    sh[5][threadIdx.x] = dvy;
     __syncthreads();
     for (int st = TPB / 2; st > 0; st >>= 1) {
         if (threadIdx.x < (unsigned)st) {
-            s[0][threadIdx.x] += s[0][threadIdx.x + st];
-            s[1][threadIdx.x] = dmax(s[1][threadIdx.x], s[1][threadIdx.x + st]);
-            s[2][threadIdx.x] += s[2][threadIdx.x + st];
-            s[3][threadIdx.x] = dmax(s[3][threadIdx.x], s[3][threadIdx.x + st]);
-            s[4][threadIdx.x] += s[4][threadIdx.x + st];
-            s[5][threadIdx.x] += s[5][threadIdx.x + st];
+            sh[0][threadIdx.x] += sh[0][threadIdx.x + st];
+            sh[1][threadIdx.x] = dmax(sh[1][threadIdx.x], sh[1][threadIdx.x + st]);
+            sh[2][threadIdx.x] += sh[2][threadIdx.x + st];
+            sh[3][threadIdx.x] = dmax(sh[3][threadIdx.x], sh[3][threadIdx.x + st]);
+            sh[4][threadIdx.x] += sh[4][threadIdx.x + st];
+            sh[5][threadIdx.x] += sh[5][threadIdx.x + st];
         }
         __syncthreads();
     }
     if (threadIdx.x == 0)
-        for (int k = 0; k < 6; ++k) part[6 * (size_t)blockIdx.x + k] = s[k][0];
+        for (int k = 0; k < 6; ++k) part[6 * (size_t)blockIdx.x + k] = sh[k][0];
 }
 
 /* per-cell closure metrics partials: sum hypot(beta*duR), sum pdriveRaw, sum pdrive */
@@ -655,8 +726,11 @@ struct mpcd_gpu_sim {
     hipStream_t stream = nullptr;
     double *x = nullptr, *y = nullptr, *vx = nullptr, *vy = nullptr;
     double *vx2 = nullptr, *vy2 = nullptr; /* zone-pass output velocities (swapped after each pass) */
+    double *x2 = nullptr, *y2 = nullptr;   /* reorder targets */
     int* pid = nullptr;
-    int* cid = nullptr;
+    int* pid2 = nullptr;
+    int* cid = nullptr;  /* unshifted cell of each slot (key of the current binning) */
+    int* cidS = nullptr; /* shifted collision cell of each slot */
     Binning bin;
     int* tileSum = nullptr;
     int nTiles = 0;
@@ -743,8 +817,12 @@ struct mpcd_gpu_sim {
         vy = dalloc<double>(n);
         vx2 = dalloc<double>(n);
         vy2 = dalloc<double>(n);
+        x2 = dalloc<double>(n);
+        y2 = dalloc<double>(n);
         pid = dalloc<int>(n);
+        pid2 = dalloc<int>(n);
         cid = dalloc<int>(n);
+        cidS = dalloc<int>(n);
         bin.cnt = dalloc<int>((size_t)Nc + 1);
         bin.start = dalloc<int>((size_t)Nc + 1);
         bin.cursor = dalloc<int>((size_t)Nc + 1);
@@ -803,7 +881,7 @@ struct mpcd_gpu_sim {
     }
 
     ~mpcd_gpu_sim() {
-        void* ptrs[] = {x, y, vx, vy, vx2, vy2, pid, cid, bin.cnt, bin.start, bin.cursor, bin.list, tileSum, next,
+        void* ptrs[] = {x, y, vx, vy, vx2, vy2, x2, y2, pid, pid2, cid, cidS, bin.cnt, bin.start, bin.cursor, bin.list, tileSum, next,
                         tmpg, slotOf, zsize, zoff, owner[0], owner[1], stage,
                         refPx, refPy, refUx, refUy, pdriveRaw, redPx, redPy, redUx, redUy, duRX, duRY, duEX, duEY,
                         pdrive, cellTmp, mask, part, scal, err, wallPart, rects[0], rects[1], zout, zscratch};
@@ -852,23 +930,24 @@ struct mpcd_gpu_sim {
         if (hv) download_xy(vx, vy, hv);
     }
 
-    /* ---- binning: cnt/start/list of the cell ids in `cid` ---- */
-    void bin_cells() {
+    /* ---- binning: cnt/start/list of the cell ids in `key` ---- */
+    void bin_cells(const int* key) {
         /* fills by kernel, not memset, so the sequence can be graph-captured */
         hipLaunchKernelGGL(k_fill_int, dim3(nblk(Nc + 1)), dim3(TPB), 0, stream, bin.cnt, Nc + 1, 0);
         hipLaunchKernelGGL(k_fill_int, dim3(nblk(Nc + 1)), dim3(TPB), 0, stream, bin.cursor, Nc + 1, 0);
-        hipLaunchKernelGGL(k_hist, dim3(nblk(n)), dim3(TPB), 0, stream, cid, n, bin.cnt);
+        hipLaunchKernelGGL(k_hist, dim3(nblk(n)), dim3(TPB), 0, stream, key, n, bin.cnt);
         hipLaunchKernelGGL(k_scan_tiles, dim3(nTiles), dim3(TPB), 0, stream, bin.cnt, Nc + 1, bin.start, tileSum);
         hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(TPB), 0, stream, tileSum, nTiles);
         hipLaunchKernelGGL(k_scan_add, dim3(nblk(Nc + 1)), dim3(TPB), 0, stream, bin.start, Nc + 1, tileSum);
-        hipLaunchKernelGGL(k_scatter, dim3(nblk(n)), dim3(TPB), 0, stream, cid, n, bin.start, bin.cursor, bin.list);
-        hipLaunchKernelGGL(k_segsort, dim3(nblk(Nc)), dim3(TPB), 0, stream, bin.start, Nc, bin.list, pid);
+        hipLaunchKernelGGL(k_scatter, dim3(nblk(n)), dim3(TPB), 0, stream, key, n, bin.start, bin.cursor, bin.list);
+        hipLaunchKernelGGL(k_segsort, dim3(nblk(Nc, SEG_CELLS)), dim3(SEG_CELLS), 0, stream, bin.start, Nc, bin.list,
+                           pid);
         check_launch();
     }
     void bin_unshifted() {
         hipLaunchKernelGGL(k_cells, dim3(nblk(n)), dim3(TPB), 0, stream, g, x, y, cid);
         check_launch();
-        bin_cells();
+        bin_cells(cid);
     }
 
     void clear_err() {
@@ -884,10 +963,12 @@ struct mpcd_gpu_sim {
     }
 
     /* ---- stages ---- */
+    /* shift != nullptr: also write the shifted cells (cidS) and the
+     * unshifted cells (cid) of the new positions */
     void stream_bc(uint64_t seed, int do_stream, const double* shift, mpcd_wall_info* wi) {
         hipLaunchKernelGGL(k_stream_bc, dim3(nblk(n)), dim3(TPB), 0, stream, g, bc, seed, do_stream, x, y, vx, vy,
-                           pid, shift ? cid : (int*)nullptr, shift ? shift[0] : 0.0, shift ? shift[1] : 0.0, xmax,
-                           ymax, wi ? wallPart : (WallAcc*)nullptr, err);
+                           pid, shift ? cidS : (int*)nullptr, shift ? cid : (int*)nullptr, shift ? shift[0] : 0.0,
+                           shift ? shift[1] : 0.0, xmax, ymax, wi ? wallPart : (WallAcc*)nullptr, err);
         check_launch();
         if (wi) {
             const int nb = (int)nblk(n);
@@ -1080,9 +1161,14 @@ struct mpcd_gpu_sim {
             hipLaunchKernelGGL(k_reduce_parts, dim3(1), dim3(TPB), 0, stream, part, nbc, 3, scal + 16);
         }
         pbeg(MPCD_PROF_KICK);
-        hipLaunchKernelGGL(k_kick, dim3(nbp), dim3(TPB), 0, stream, g, x, y, vx, vy, duRX, duRY, duEX, duEY, scal,
-                           cm ? part : (double*)nullptr);
+        hipLaunchKernelGGL(k_kick_reorder, dim3(nbp), dim3(TPB), 0, stream, g, bin.list, x, y, vx, vy, pid, x2, y2,
+                           vx2, vy2, pid2, duRX, duRY, duEX, duEY, scal, cm ? part : (double*)nullptr);
         pend(MPCD_PROF_KICK);
+        std::swap(x, x2);
+        std::swap(y, y2);
+        std::swap(vx, vx2);
+        std::swap(vy, vy2);
+        std::swap(pid, pid2); /* the binning now describes the old order: stale until rebuilt */
         check_launch();
         if (cm) {
             std::vector<double> h((size_t)nbp * 6);
@@ -1129,13 +1215,13 @@ struct mpcd_gpu_sim {
         stream_bc(104729ULL + 10007ULL * us, 1, sh, sm ? &sm->wall : nullptr);
         pend(MPCD_PROF_STREAM_BC);
         pbeg(MPCD_PROF_BIN);
-        bin_cells();
+        bin_cells(cidS);
         pend(MPCD_PROF_BIN);
         pbeg(MPCD_PROF_COLLIDE);
         collide(433494437ULL + 10007ULL * us);
         pend(MPCD_PROF_COLLIDE);
         pbeg(MPCD_PROF_BIN);
-        bin_unshifted();
+        bin_cells(cid); /* collision does not move particles */
         pend(MPCD_PROF_BIN);
         if (stages && stages[0]) get_state(stages[0], stages[1]);
         pbeg(MPCD_PROF_CELL_FIELDS);
@@ -1327,7 +1413,7 @@ int mpcd_gpu_srd_collision_step(const mpcd_params* p, double* v, const int32_t* 
         auto& s = scratch_sim(*p);
         s.set_state(nullptr, v);
         HIP_OK(hipMemcpyAsync(s.cid, cid, (size_t)s.n * sizeof(int), hipMemcpyHostToDevice, s.stream));
-        s.bin_cells();
+        s.bin_cells(s.cid);
         s.collide(rng_seed);
         s.get_state(nullptr, v);
     });
