@@ -232,11 +232,13 @@ struct CollideParams {
 
 MPCD_HD void collide_cell(const CollideParams& cp, uint64_t seed, int cell, const int* ids, int m, double* vx,
                           double* vy) {
+    /* ids == nullptr: the cell's particles are vx[0..m), vy[0..m) */
     if (m <= 0) return;
     double sx = 0.0, sy = 0.0;
     for (int k = 0; k < m; ++k) {
-        sx += vx[ids[k]];
-        sy += vy[ids[k]];
+        const int i = ids ? ids[k] : k;
+        sx += vx[i];
+        sy += vy[i];
     }
     const double ux = sx / (double)m, uy = sy / (double)m;
     double ct = 1.0, st = 0.0; /* cos(0), sin(0) for cells with fewer than 2 particles */
@@ -249,7 +251,8 @@ MPCD_HD void collide_cell(const CollideParams& cp, uint64_t seed, int cell, cons
     if (cp.thermostat) {
         double s2 = 0.0;
         for (int k = 0; k < m; ++k) {
-            const double vxr = vx[ids[k]] - ux, vyr = vy[ids[k]] - uy;
+            const int i = ids ? ids[k] : k;
+            const double vxr = vx[i] - ux, vyr = vy[i] - uy;
             const double rx = ct * vxr - st * vyr;
             const double ry = st * vxr + ct * vyr;
             s2 += rx * rx + ry * ry;
@@ -258,7 +261,7 @@ MPCD_HD void collide_cell(const CollideParams& cp, uint64_t seed, int cell, cons
         if (m > 1 && s2 > 0.0) lam = MPCD_SQRT((double)dof * cp.kBT / s2);
     }
     for (int k = 0; k < m; ++k) {
-        const int i = ids[k];
+        const int i = ids ? ids[k] : k;
         const double vxr = vx[i] - ux, vyr = vy[i] - uy;
         double rx = ct * vxr - st * vyr;
         double ry = st * vxr + ct * vyr;
@@ -280,13 +283,15 @@ struct CellStats {
 
 MPCD_HD CellStats cell_stats(const int* ids, int m, const double* vx, const double* vy, double Vc, double Kvirial,
                              double rhoTarget) {
+    /* ids == nullptr: the cell's particles are vx[0..m), vy[0..m) */
     CellStats s;
     s.N = m;
     s.Px = 0.0;
     s.Py = 0.0;
     for (int k = 0; k < m; ++k) {
-        s.Px += vx[ids[k]];
-        s.Py += vy[ids[k]];
+        const int i = ids ? ids[k] : k;
+        s.Px += vx[i];
+        s.Py += vy[i];
     }
     s.Ux = 0.0;
     s.Uy = 0.0;
@@ -297,7 +302,8 @@ MPCD_HD CellStats cell_stats(const int* ids, int m, const double* vx, const doub
     s.rho = (double)m / Vc;
     double s2 = 0.0;
     for (int k = 0; k < m; ++k) {
-        const double dvx = vx[ids[k]] - s.Ux, dvy = vy[ids[k]] - s.Uy;
+        const int i = ids ? ids[k] : k;
+        const double dvx = vx[i] - s.Ux, dvy = vy[i] - s.Uy;
         s2 += dvx * dvx + dvy * dvy;
     }
     const int dof = 2 * imax(m - 1, 0);
@@ -368,14 +374,21 @@ constexpr int ZONE_MAX_MOVE = 64; /* movers whose draws are staged at once */
 /* Working set of one zone.  Cell arrays are indexed by the zone-local cell
  * lc = (ix-xs)*zh + (iy-ys) (ascending global cell id); particles by the
  * zone-local index k (the zone's cells' runs of the binning list, in lc
- * order).  Per-particle scratch lives at [off, off+P) of next/tmpg/slotOf,
+ * order).  Per-particle scratch lives at [off, off+P) of next and tmpg,
  * private to the zone: the shifted layout's zones overlap
  * (zone_pass.cpp:191-219 starts them at 0 and T/2 with width T), so two
- * zones may hold the same particle.  Reads come from the pass-input
- * snapshot; a zone writes only the particles it owns, i.e. whose input cell
- * has this zone as its owner-map entry (the last zone covering the cell,
- * which is the zone whose result the reference's merge keeps,
- * zone_pass_openmp.cpp:145-153).  owner == nullptr: the zone owns all. */
+ * zones may hold the same particle.  Every zone reads the pass input; a
+ * zone writes only the particles it owns, i.e. whose input cell has this
+ * zone as its owner-map entry (the last zone covering the cell, which is the
+ * zone whose result the reference's merge keeps,
+ * zone_pass_openmp.cpp:145-153); owner == nullptr: the zone owns all.
+ * Positions are written in place (nobody reads them during a pass).
+ * Velocities are written in place for particles whose input cell only one
+ * zone covers; for cells several zones cover (gcover != 0) they go to
+ * vsx/vsy and are copied back after the pass, so every zone reads inputs.
+ * A cell's particle list stays implicit -- its input run of local indices
+ * k0..k0+cnt-1 -- until the first change, when its links are written
+ * (mat = 1); cells nothing touched cost nothing at the correction. */
 struct ZoneCtx {
     Geom g;
     ZoneParams zp;
@@ -389,18 +402,16 @@ struct ZoneCtx {
     const int* gpid;
     const int* gcid;  /* input cell of each slot */
     const int* owner; /* owner map of the layout, or nullptr */
-    /* state: positions updated in place (nobody reads them during a pass),
-     * velocities read from vxin/vyin and written to vxout/vyout */
-    double *x, *y;
-    const double *vxin, *vyin;
-    double *vxout, *vyout;
-    int* next;   /* [off, off+P): list links by local index */
-    int* tmpg;   /* [off, off+P): donor shuffles */
-    int* slotOf; /* [off, off+P): slot of each local index */
+    /* state */
+    double *x, *y, *vx, *vy;
+    double *vsx, *vsy;
+    const unsigned char* gcover; /* per cell: several zones cover it, or nullptr */
+    int* next; /* [off, off+P): list links by local index */
+    int* tmpg; /* [off, off+P): donor shuffles */
     /* zone cell arrays */
-    int *cnt, *head, *tail, *dl, *cntS, *jd, *colL, *colG;
-    double *lowA, *highA, *dU2; /* lowA/highA only with local fluid fractions */
-    unsigned char *bulk0, *recv, *bb, *skip;
+    int *cnt, *head, *tail, *dl, *cntS, *jd, *k0, *colL, *colG;
+    double *lowA, *highA, *dux, *duy; /* lowA/highA only with local fluid fractions */
+    unsigned char *bulk0, *recv, *bb, *skip, *mat, *corr;
     /* small scratch */
     Cand* cand;     /* 24 */
     int* ord;       /* 24 */
@@ -504,6 +515,32 @@ MPCD_HD int shift_idx(const ZoneCtx& z, int gx, int gy, int dx, int dy, int& jx,
     return 1;
 }
 
+/* slot of the zone's k-th particle: the zone's particles are its columns'
+ * runs of the binning list, in column order */
+MPCD_HD int slot_of(const ZoneCtx& z, int k) {
+    int lo = 0, hi = z.zw - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (z.colL[mid] <= k) lo = mid;
+        else hi = mid - 1;
+    }
+    return z.glist[z.colG[lo] + (k - z.colL[lo])];
+}
+
+/* successor of k in cell lc's list (implicit run or links) */
+MPCD_HD int next_k(const ZoneCtx& z, int lc, int k) {
+    return z.mat[lc] ? z.next[z.off + k] : (k < z.tail[lc] ? k + 1 : -1);
+}
+
+/* write the links of a still-implicit list before its first change */
+MPCD_HD void ensure_mat(ZoneCtx& z, int lc) {
+    if (z.mat[lc]) return;
+    const int t = z.tail[lc];
+    for (int k = z.head[lc]; k >= 0 && k <= t; ++k) z.next[z.off + k] = (k < t) ? k + 1 : -1;
+    z.mat[lc] = 1;
+}
+
+/* requires a materialised list */
 MPCD_HD void list_append(ZoneCtx& z, int lc, int k) {
     z.next[z.off + k] = -1;
     if (z.tail[lc] < 0) z.head[lc] = k;
@@ -669,12 +706,12 @@ MPCD_HD void stage_dense_draws(ZoneCtx& z, int src, int cnt, int pass, int lane,
         int k = z.head[src];
         for (int q = 0; q < cnt && k >= 0; ++q) {
             z.mvk[q] = k;
-            k = z.next[z.off + k];
+            k = next_k(z, src, k);
         }
     }
     ZBAR();
     for (int q = lane; q < cnt; q += nl) {
-        const int slot = z.slotOf[z.off + z.mvk[q]];
+        const int slot = slot_of(z, z.mvk[q]);
         mpcd_draw_dense_pos(z.seed, pass, (uint32_t)z.gpid[slot], &z.mvx[q], &z.mvy[q]);
     }
     ZBAR();
@@ -702,11 +739,13 @@ MPCD_HD int spread(ZoneCtx& z, int src, int high, int needOut, int pass, int* rv
             }
             const int quota = imax(1, (int)MPCD_CEIL((double)needOut / (double)imax(1, remaining)));
             const int nt = imax(1, imin(imin(quota, deficit), imin(needOut, avail)));
+            ensure_mat(z, src);
+            ensure_mat(z, nb);
             for (int q = 0; q < nt; ++q) {
                 const int k = z.head[src];
                 z.head[src] = z.next[z.off + k];
                 if (z.head[src] < 0) z.tail[src] = -1;
-                const int slot = z.slotOf[z.off + k];
+                const int slot = slot_of(z, k);
                 double ux, uy;
                 if (used < ZONE_MAX_MOVE) {
                     ux = z.mvx[used];
@@ -892,20 +931,21 @@ MPCD_HD void zone_sparse_pass(ZoneCtx& z, int lane, int nl) {
                 int kk = z.head[d];
                 for (int q = 0; q < md; ++q) {
                     tmp[q] = kk;
-                    kk = z.next[z.off + kk];
+                    kk = next_k(z, d, kk);
                 }
             }
             const int dgc = lc_gc(z, d);
             zone_shuffle(z, tmp, md, lane, nl,
                          [&](uint32_t i) { return mpcd_draw_particle_shuffle(z.seed, (uint32_t)cgc, (uint32_t)dgc, i); });
             for (int q = lane; q < nt && q < ZONE_MAX_MOVE; q += nl) {
-                const int slot = z.slotOf[z.off + tmp[q]];
+                const int slot = slot_of(z, tmp[q]);
                 mpcd_draw_sparse_pos(z.seed, (uint32_t)z.gpid[slot], &z.mvx[q], &z.mvy[q]);
             }
             ZBAR();
             if (lane == 0) {
+                ensure_mat(z, c);
                 for (int q = 0; q < nt; ++q) {
-                    const int slot = z.slotOf[z.off + tmp[q]];
+                    const int slot = slot_of(z, tmp[q]);
                     double ux, uy;
                     if (q < ZONE_MAX_MOVE) {
                         ux = z.mvx[q];
@@ -916,6 +956,7 @@ MPCD_HD void zone_sparse_pass(ZoneCtx& z, int lane, int nl) {
                     move_to(z, c, slot, ux, uy);
                     list_append(z, c, tmp[q]);
                 }
+                z.mat[d] = 1;
                 z.head[d] = -1;
                 z.tail[d] = -1;
                 for (int q = nt; q < md; ++q) list_append(z, d, tmp[q]);
@@ -933,9 +974,9 @@ MPCD_HD void zone_sparse_pass(ZoneCtx& z, int lane, int nl) {
     }
 }
 
-/* Zone setup (zone_pass.cpp:680-685), all lanes: counts, list heads from a
- * wave prefix sum over the zone's cells, the column map, then per cell the
- * slots and links. */
+/* Zone setup (zone_pass.cpp:680-685), all lanes: counts, implicit list of
+ * each cell (its run of local indices, from a wave prefix sum over the
+ * zone's cells) and the column map. */
 MPCD_HD void zone_setup(ZoneCtx& z, int lane, int nl) {
     int base = 0;
     for (int b0 = 0; b0 < z.C; b0 += nl) {
@@ -945,21 +986,23 @@ MPCD_HD void zone_setup(ZoneCtx& z, int lane, int nl) {
         if (lc < z.C) {
             const int k0 = base + incl - m;
             z.cnt[lc] = m;
+            z.k0[lc] = k0;
             z.head[lc] = m > 0 ? k0 : -1;
             z.tail[lc] = m > 0 ? k0 + m - 1 : -1;
-            z.dl[lc] = k0; /* first local index of the cell */
             z.recv[lc] = 0;
             z.bb[lc] = 0;
             z.skip[lc] = 0;
-            z.dU2[lc] = 0.0;
+            z.mat[lc] = 0;
+            z.corr[lc] = 0;
+            z.dux[lc] = 0.0;
+            z.duy[lc] = 0.0;
             if (lc % z.zh == 0) {
                 z.colL[lc / z.zh] = k0;
                 z.colG[lc / z.zh] = z.gstart[lc_gc(z, lc)];
             }
         }
         /* lanes past the zone add 0, so the last lane holds the chunk total */
-        const int chunk = wave_bcast(incl, nl - 1);
-        base += chunk;
+        base += wave_bcast(incl, nl - 1);
     }
     if (lane == 0) {
         z.P = base;
@@ -973,56 +1016,75 @@ MPCD_HD void zone_setup(ZoneCtx& z, int lane, int nl) {
     }
 }
 
-MPCD_HD void zone_setup_links(ZoneCtx& z, int lane, int nl) {
-    for (int lc = lane; lc < z.C; lc += nl) {
-        const int gc = lc_gc(z, lc);
-        const int s0 = z.gstart[gc], m = z.gcnt[gc];
-        const int k0 = z.dl[lc];
-        for (int j = 0; j < m; ++j) {
-            z.slotOf[z.off + k0 + j] = z.glist[s0 + j];
-            z.next[z.off + k0 + j] = (j + 1 < m) ? k0 + j + 1 : -1;
-        }
-    }
-}
-
-/* Momentum correction of one zone cell (zone_pass.cpp:870-909); writes the
- * output velocity of every particle of the cell this zone owns.  The
- * momentum before is the sum over the cell's input run in particle order
- * (the input velocities are never written during the pass). */
-MPCD_HD void zone_correct_cell(ZoneCtx& z, int lc) {
+/* Momentum correction (zone_pass.cpp:870-909), first half: the velocity
+ * change of a changed cell, from the momentum of its input run (particle
+ * order) and of its final list.  Reads only; a cell nothing changed has
+ * the same list before and after, so its change is exactly zero. */
+MPCD_HD void zone_correct_sums(ZoneCtx& z, int lc) {
     const int m = z.cnt[lc];
-    if (m == 0) return;
+    if (!z.mat[lc] || m == 0) return;
     const int gc = lc_gc(z, lc);
     const int s0 = z.gstart[gc], m0 = z.gcnt[gc];
     double pbx = 0.0, pby = 0.0;
     for (int j = 0; j < m0; ++j) {
         const int slot = z.glist[s0 + j];
-        pbx += z.vxin[slot];
-        pby += z.vyin[slot];
+        pbx += z.vx[slot];
+        pby += z.vy[slot];
     }
     double px = 0.0, py = 0.0;
     for (int k = z.head[lc]; k >= 0; k = z.next[z.off + k]) {
-        const int slot = z.slotOf[z.off + k];
-        px += z.vxin[slot];
-        py += z.vyin[slot];
+        const int slot = slot_of(z, k);
+        px += z.vx[slot];
+        py += z.vy[slot];
     }
     const double dPx = pbx - px, dPy = pby - py;
     const double dux = dPx / (double)m, duy = dPy / (double)m;
-    z.dU2[lc] = dux * dux + duy * duy;
+    z.dux[lc] = dux;
+    z.duy[lc] = duy;
     bool corr = z.zp.corrEnabled != 0;
     if (corr && z.skip[lc]) corr = false;
     if (corr && z.recv[lc] && !z.bb[lc]) corr = false;
     if (corr && !(dux != 0.0 || duy != 0.0)) corr = false;
-    for (int k = z.head[lc]; k >= 0; k = z.next[z.off + k]) {
-        const int slot = z.slotOf[z.off + k];
-        if (!zone_owns(z, slot)) continue;
-        double vx = z.vxin[slot], vy = z.vyin[slot];
-        if (corr) {
-            vx += dux;
-            vy += duy;
+    z.corr[lc] = corr ? 1 : 0;
+}
+
+MPCD_HD void put_velocity(ZoneCtx& z, int slot, double vx, double vy) {
+    if (z.gcover && z.gcover[z.gcid[slot]]) {
+        z.vsx[slot] = vx;
+        z.vsy[slot] = vy;
+    } else {
+        z.vx[slot] = vx;
+        z.vy[slot] = vy;
+    }
+}
+
+/* Second half: write the velocities of the owned particles of a changed
+ * cell; for an unchanged cell of the overlap band, pass its owned particles'
+ * input velocities through to the side buffer. */
+MPCD_HD void zone_correct_write(ZoneCtx& z, int lc) {
+    if (z.mat[lc]) {
+        const bool corr = z.corr[lc] != 0;
+        const double dux = z.dux[lc], duy = z.duy[lc];
+        for (int k = z.head[lc]; k >= 0; k = z.next[z.off + k]) {
+            const int slot = slot_of(z, k);
+            if (!zone_owns(z, slot)) continue;
+            double vx = z.vx[slot], vy = z.vy[slot];
+            if (corr) {
+                vx += dux;
+                vy += duy;
+            }
+            put_velocity(z, slot, vx, vy);
         }
-        z.vxout[slot] = vx;
-        z.vyout[slot] = vy;
+        return;
+    }
+    const int gc = lc_gc(z, lc);
+    if (!z.gcover || !z.gcover[gc]) return;
+    const int s0 = z.gstart[gc], m0 = z.gcnt[gc];
+    for (int j = 0; j < m0; ++j) {
+        const int slot = z.glist[s0 + j];
+        if (!zone_owns(z, slot)) continue;
+        z.vsx[slot] = z.vx[slot];
+        z.vsy[slot] = z.vy[slot];
     }
 }
 
@@ -1031,7 +1093,7 @@ MPCD_HD void zone_final_stats(ZoneCtx& z) {
     double acc = 0.0, mx = 0.0;
     int nDU = 0;
     for (int lc = 0; lc < z.C; ++lc) {
-        const double d = z.dU2[lc];
+        const double d = z.dux[lc] * z.dux[lc] + z.duy[lc] * z.duy[lc];
         mx = dmax(mx, MPCD_SQRT(d));
         if (d > 0.0) {
             acc += d;
@@ -1056,10 +1118,10 @@ struct ZoneOut {
 /* bytes of the per-zone working set for C cells and W columns */
 MPCD_HD size_t zone_bytes(int C, int W, int useLocal) {
     size_t b = 0;
-    b += (size_t)C * sizeof(double) * (useLocal ? 3 : 1); /* dU2 (+ lowA highA) */
-    b += (size_t)C * (6 * sizeof(int));                    /* cnt head tail dl cntS jd */
+    b += (size_t)C * sizeof(double) * (useLocal ? 4 : 2); /* dux duy (+ lowA highA) */
+    b += (size_t)C * (7 * sizeof(int));                    /* cnt head tail dl cntS jd k0 */
     b += (size_t)W * (2 * sizeof(int));                    /* colL colG */
-    b += (size_t)C * 4;                                    /* bulk0 recv bb skip */
+    b += (size_t)C * 6;                                    /* bulk0 recv bb skip mat corr */
     b = (b + 15) & ~(size_t)15;
     b += 24 * sizeof(Cand) + 24 * 4 * sizeof(int) + 48 * sizeof(int) + 8 * sizeof(double) + 3 * 8 * sizeof(int) +
          4 * sizeof(double) + ZONE_MAX_MOVE * (2 * sizeof(double) + sizeof(int));
@@ -1068,7 +1130,8 @@ MPCD_HD size_t zone_bytes(int C, int W, int useLocal) {
 
 MPCD_HD void zone_carve(ZoneCtx& z, unsigned char* base, int C, int W, int useLocal) {
     unsigned char* p = base;
-    z.dU2 = (double*)p; p += (size_t)C * sizeof(double);
+    z.dux = (double*)p; p += (size_t)C * sizeof(double);
+    z.duy = (double*)p; p += (size_t)C * sizeof(double);
     if (useLocal) {
         z.lowA = (double*)p; p += (size_t)C * sizeof(double);
         z.highA = (double*)p; p += (size_t)C * sizeof(double);
@@ -1082,12 +1145,15 @@ MPCD_HD void zone_carve(ZoneCtx& z, unsigned char* base, int C, int W, int useLo
     z.dl = (int*)p; p += (size_t)C * sizeof(int);
     z.cntS = (int*)p; p += (size_t)C * sizeof(int);
     z.jd = (int*)p; p += (size_t)C * sizeof(int);
+    z.k0 = (int*)p; p += (size_t)C * sizeof(int);
     z.colL = (int*)p; p += (size_t)W * sizeof(int);
     z.colG = (int*)p; p += (size_t)W * sizeof(int);
     z.bulk0 = p; p += C;
     z.recv = p; p += C;
     z.bb = p; p += C;
     z.skip = p; p += C;
+    z.mat = p; p += C;
+    z.corr = p; p += C;
     p = base + ((size_t)(p - base + 15) & ~(size_t)15);
     z.cand = (Cand*)p; p += 24 * sizeof(Cand);
     z.ord = (int*)p; p += 24 * sizeof(int);

@@ -306,12 +306,44 @@ __global__ void __launch_bounds__(SEG_CELLS) k_segsort(const int* __restrict__ s
 
 /* ---- collision ---- */
 
-__global__ void k_collide(CollideParams cp, uint64_t seed, const int* __restrict__ start,
-                          const int* __restrict__ list, int Nc, double* __restrict__ vx, double* __restrict__ vy) {
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= Nc) return;
-    const int b = start[c];
-    collide_cell(cp, seed, c, list + b, start[c + 1] - b, vx, vy);
+/* Per-cell kernels stage a block's particles in LDS: CELL_BLK consecutive
+ * cells own one contiguous range of the binning list, read with coalesced
+ * loads; each thread then runs its cell's sequential sums (particle order,
+ * so the same rounding as one pass over global memory) on LDS.  A range
+ * larger than `cap` entries falls back to the global-memory loop. */
+constexpr int CELL_BLK = 128;
+
+__global__ void __launch_bounds__(CELL_BLK) k_collide(CollideParams cp, uint64_t seed, const int* __restrict__ start,
+                                                      const int* __restrict__ list, int Nc, double* __restrict__ vx,
+                                                      double* __restrict__ vy, int cap) {
+    extern __shared__ double sv[];
+    const int c0 = blockIdx.x * CELL_BLK;
+    const int c1 = min(c0 + CELL_BLK, Nc);
+    const int b0 = start[c0], b1 = start[c1];
+    const int c = c0 + (int)threadIdx.x;
+    if (b1 - b0 <= cap) {
+        double* svx = sv;
+        double* svy = sv + cap;
+        for (int q = (int)threadIdx.x; q < b1 - b0; q += CELL_BLK) {
+            const int slot = list[b0 + q];
+            svx[q] = vx[slot];
+            svy[q] = vy[slot];
+        }
+        __syncthreads();
+        if (c < c1) {
+            const int o = start[c] - b0;
+            collide_cell(cp, seed, c, nullptr, start[c + 1] - start[c], svx + o, svy + o);
+        }
+        __syncthreads();
+        for (int q = (int)threadIdx.x; q < b1 - b0; q += CELL_BLK) {
+            const int slot = list[b0 + q];
+            vx[slot] = svx[q];
+            vy[slot] = svy[q];
+        }
+        return;
+    }
+    if (c >= c1) return;
+    collide_cell(cp, seed, c, list + start[c], start[c + 1] - start[c], vx, vy);
 }
 
 /* deterministic block partial sums of (vx, vy) */
@@ -369,13 +401,32 @@ struct FieldsOut {
     double *Ux, *Uy, *Px, *Py, *T, *rho, *Pkin, *Pvir, *P;
 };
 
-__global__ void k_cell_fields(const int* __restrict__ start, const int* __restrict__ list, int Nc,
-                              const double* __restrict__ vx, const double* __restrict__ vy, double Vc, double K,
-                              double rhoT, FieldsOut f) {
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= Nc) return;
-    const int b = start[c];
-    const CellStats s = cell_stats(list + b, start[c + 1] - b, vx, vy, Vc, K, rhoT);
+__global__ void __launch_bounds__(CELL_BLK) k_cell_fields(const int* __restrict__ start, const int* __restrict__ list,
+                                                          int Nc, const double* __restrict__ vx,
+                                                          const double* __restrict__ vy, double Vc, double K,
+                                                          double rhoT, FieldsOut f, int cap) {
+    extern __shared__ double sv[];
+    const int c0 = blockIdx.x * CELL_BLK;
+    const int c1 = min(c0 + CELL_BLK, Nc);
+    const int b0 = start[c0], b1 = start[c1];
+    const int c = c0 + (int)threadIdx.x;
+    CellStats s;
+    if (b1 - b0 <= cap) {
+        double* svx = sv;
+        double* svy = sv + cap;
+        for (int q = (int)threadIdx.x; q < b1 - b0; q += CELL_BLK) {
+            const int slot = list[b0 + q];
+            svx[q] = vx[slot];
+            svy[q] = vy[slot];
+        }
+        __syncthreads();
+        if (c >= c1) return;
+        const int o = start[c] - b0;
+        s = cell_stats(nullptr, start[c + 1] - start[c], svx + o, svy + o, Vc, K, rhoT);
+    } else {
+        if (c >= c1) return;
+        s = cell_stats(list + start[c], start[c + 1] - start[c], vx, vy, Vc, K, rhoT);
+    }
     if (f.N) f.N[c] = s.N;
     if (f.Ux) f.Ux[c] = s.Ux;
     if (f.Uy) f.Uy[c] = s.Uy;
@@ -405,10 +456,10 @@ struct ZoneArgs {
     const int* owner; /* nullptr: every zone owns all its particles */
     const int* zoff;  /* per-zone scratch offsets (exclusive scan of sizes) */
     long long scratchCap;
-    double *x, *y;
-    const double *vxin, *vyin;
-    double *vxout, *vyout;
-    int *next, *tmpg, *slotOf;
+    double *x, *y, *vx, *vy;
+    double *vsx, *vsy;
+    const unsigned char* gcover;
+    int *next, *tmpg;
     ZoneOut* out;
     int* err;
     unsigned char* gscratch; /* used instead of LDS when the zone arrays do not fit */
@@ -459,13 +510,13 @@ __global__ void __launch_bounds__(ZONE_TPB) k_zone(ZoneArgs a) {
         z.owner = a.owner;
         z.x = a.x;
         z.y = a.y;
-        z.vxin = a.vxin;
-        z.vyin = a.vyin;
-        z.vxout = a.vxout;
-        z.vyout = a.vyout;
+        z.vx = a.vx;
+        z.vy = a.vy;
+        z.vsx = a.vsx;
+        z.vsy = a.vsy;
+        z.gcover = a.gcover;
         z.next = a.next;
         z.tmpg = a.tmpg;
-        z.slotOf = a.slotOf;
         unsigned char* base = a.gscratch ? a.gscratch + (size_t)zi * a.gstride : lds;
         zone_carve(z, base, z.C, z.zw, a.zp.useLocal);
     }
@@ -478,12 +529,12 @@ __global__ void __launch_bounds__(ZONE_TPB) k_zone(ZoneArgs a) {
     }
     __syncthreads();
     if (!ok) return; /* uniform across the workgroup */
-    zone_setup_links(z, lane, nl);
-    __syncthreads();
     zone_dense_passes(z, lane, nl);
     zone_sparse_pass(z, lane, nl);
     __syncthreads();
-    for (int lc = lane; lc < z.C; lc += nl) zone_correct_cell(z, lc);
+    for (int lc = lane; lc < z.C; lc += nl) zone_correct_sums(z, lc);
+    __syncthreads();
+    for (int lc = lane; lc < z.C; lc += nl) zone_correct_write(z, lc);
     __syncthreads();
     if (lane == 0) {
         zone_final_stats(z);
@@ -495,6 +546,21 @@ __global__ void __launch_bounds__(ZONE_TPB) k_zone(ZoneArgs a) {
         o.stats4 = z.stats4;
         o.stats5 = z.stats5;
         a.out[zi] = o;
+    }
+}
+
+/* velocities of the particles of cells several zones cover, from the side
+ * buffer (their input runs in the pass-input binning) */
+__global__ void k_overlap_copyback(const int* __restrict__ cells, int ncells, const int* __restrict__ start,
+                                   const int* __restrict__ list, const double* __restrict__ vsx,
+                                   const double* __restrict__ vsy, double* __restrict__ vx, double* __restrict__ vy) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ncells) return;
+    const int c = cells[t];
+    for (int j = start[c]; j < start[c + 1]; ++j) {
+        const int slot = list[j];
+        vx[slot] = vsx[slot];
+        vy[slot] = vsy[slot];
     }
 }
 
@@ -734,7 +800,7 @@ struct mpcd_gpu_sim {
     Binning bin;
     int* tileSum = nullptr;
     int nTiles = 0;
-    int *next = nullptr, *tmpg = nullptr, *slotOf = nullptr;
+    int *next = nullptr, *tmpg = nullptr;
     long long scratchCap = 0;
     int *zsize = nullptr, *zoff = nullptr;
     double* stage = nullptr; /* 2n doubles, host<->device transfers */
@@ -751,6 +817,9 @@ struct mpcd_gpu_sim {
     WallAcc* wallPart = nullptr;
     ZoneRect* rects[2] = {nullptr, nullptr};
     int* owner[2] = {nullptr, nullptr};
+    unsigned char* cover[2] = {nullptr, nullptr}; /* per cell: several zones cover it (nullptr: none do) */
+    int* overCells[2] = {nullptr, nullptr};
+    int nOver[2] = {0, 0};
     int nz[2] = {0, 0};
     int maxCells[2] = {0, 0}, maxCols[2] = {0, 0};
     ZoneOut* zout = nullptr;
@@ -761,6 +830,7 @@ struct mpcd_gpu_sim {
     CollideParams cp{};
     BCParams bc{};
     double xmax = 0, ymax = 0;
+    int cellCap = 0; /* LDS entries of a per-cell block (k_collide, k_cell_fields) */
 
     /* per-class kernel timing with events on this stream (MPCD_PROF_*) */
     bool prof = false;
@@ -846,10 +916,15 @@ struct mpcd_gpu_sim {
         ymax = std::nextafter(p.Ly, 0.0);
         cp = make_collide_params(p);
         zp = make_zone_params(p);
+        {
+            /* 1.25x the mean particles of a block plus slack, at most 64 KB */
+            const double mean = (double)n / (double)std::max(Nc, 1) * CELL_BLK;
+            cellCap = (int)std::min(4096.0, std::max(256.0, 1.25 * mean + 8.0 * CELL_BLK));
+        }
         /* layouts (zones of the shifted layout overlap: a cell is covered by
          * up to four zones, so per-zone particle scratch is sized by the
          * largest coverage) and the interzone mask */
-        int cover = 1;
+        int coverMax = 1;
         for (int L = 0; L < 2; ++L) {
             std::vector<int32_t> own;
             const auto r = layout_rects(p, L == 0 ? MPCD_LAYOUT_BASE : MPCD_LAYOUT_SHIFTED, &own);
@@ -859,17 +934,30 @@ struct mpcd_gpu_sim {
                 maxCells[L] = std::max(maxCells[L], (q.xe - q.xs) * (q.ye - q.ys));
                 maxCols[L] = std::max(maxCols[L], q.xe - q.xs);
                 for (int ix = q.xs; ix < q.xe; ++ix)
-                    for (int iy = q.ys; iy < q.ye; ++iy) cover = std::max(cover, ++cov[(size_t)(iy + p.Ny * ix)]);
+                    for (int iy = q.ys; iy < q.ye; ++iy) coverMax = std::max(coverMax, ++cov[(size_t)(iy + p.Ny * ix)]);
+            }
+            std::vector<unsigned char> ov((size_t)Nc, 0);
+            std::vector<int> oc;
+            for (int c = 0; c < Nc; ++c)
+                if (cov[(size_t)c] > 1) {
+                    ov[(size_t)c] = 1;
+                    oc.push_back(c);
+                }
+            nOver[L] = (int)oc.size();
+            if (!oc.empty()) {
+                cover[L] = dalloc<unsigned char>((size_t)Nc);
+                HIP_OK(hipMemcpy(cover[L], ov.data(), ov.size(), hipMemcpyHostToDevice));
+                overCells[L] = dalloc<int>(oc.size());
+                HIP_OK(hipMemcpy(overCells[L], oc.data(), oc.size() * sizeof(int), hipMemcpyHostToDevice));
             }
             rects[L] = dalloc<ZoneRect>(r.size());
             HIP_OK(hipMemcpy(rects[L], r.data(), r.size() * sizeof(ZoneRect), hipMemcpyHostToDevice));
             owner[L] = dalloc<int>((size_t)Nc);
             HIP_OK(hipMemcpy(owner[L], own.data(), own.size() * sizeof(int), hipMemcpyHostToDevice));
         }
-        scratchCap = (long long)cover * (long long)n;
+        scratchCap = (long long)coverMax * (long long)n;
         next = dalloc<int>((size_t)scratchCap + 1);
         tmpg = dalloc<int>((size_t)scratchCap + 1);
-        slotOf = dalloc<int>((size_t)scratchCap + 1);
         zsize = dalloc<int>((size_t)std::max(nz[0], nz[1]) + 2);
         zoff = dalloc<int>((size_t)std::max(nz[0], nz[1]) + 2);
         ensure_zout(std::max(nz[0], nz[1]));
@@ -882,7 +970,7 @@ struct mpcd_gpu_sim {
 
     ~mpcd_gpu_sim() {
         void* ptrs[] = {x, y, vx, vy, vx2, vy2, x2, y2, pid, pid2, cid, cidS, bin.cnt, bin.start, bin.cursor, bin.list, tileSum, next,
-                        tmpg, slotOf, zsize, zoff, owner[0], owner[1], stage,
+                        tmpg, zsize, zoff, owner[0], owner[1], cover[0], cover[1], overCells[0], overCells[1], stage,
                         refPx, refPy, refUx, refUy, pdriveRaw, redPx, redPy, redUx, redUy, duRX, duRY, duEX, duEY,
                         pdrive, cellTmp, mask, part, scal, err, wallPart, rects[0], rects[1], zout, zscratch};
         for (void* q : ptrs)
@@ -995,7 +1083,8 @@ struct mpcd_gpu_sim {
     }
 
     void collide(uint64_t seed) {
-        hipLaunchKernelGGL(k_collide, dim3(nblk(Nc)), dim3(TPB), 0, stream, cp, seed, bin.start, bin.list, Nc, vx, vy);
+        hipLaunchKernelGGL(k_collide, dim3(nblk(Nc, CELL_BLK)), dim3(CELL_BLK), cellCap * 2 * sizeof(double), stream, cp,
+                           seed, bin.start, bin.list, Nc, vx, vy, cellCap);
         check_launch();
         if (p.keepMeanFlow) {
             const int nb = (int)nblk(n);
@@ -1017,8 +1106,8 @@ struct mpcd_gpu_sim {
         f.Px = refPx;
         f.Py = refPy;
         f.P = pdriveRaw; /* P = Pkin + Pvir = pdriveRaw (liquid_closure.cpp:401-405) */
-        hipLaunchKernelGGL(k_cell_fields, dim3(nblk(Nc)), dim3(TPB), 0, stream, bin.start, bin.list, Nc, vx, vy, vc(),
-                           p.Kvirial, rho_target(), f);
+        hipLaunchKernelGGL(k_cell_fields, dim3(nblk(Nc, CELL_BLK)), dim3(CELL_BLK), cellCap * 2 * sizeof(double), stream,
+                           bin.start, bin.list, Nc, vx, vy, vc(), p.Kvirial, rho_target(), f, cellCap);
         check_launch();
     }
 
@@ -1032,10 +1121,11 @@ struct mpcd_gpu_sim {
     }
 
     /* One snapshot zone pass over the current unshifted binning (whose cid
-     * holds the input cells).  single = 1: one zone that owns all its
+     * holds the input cells).  coverDev/overDev: cells several zones cover
+     * (nullptr when none do).  single = 1: one zone that owns all its
      * particles, seeded with seedBase itself (run_zone_kernel_inplace). */
     void zone_pass(uint64_t seedBase, const ZoneRect* rectsDev, int nZ, int maxC, int maxW, const int* ownerDev,
-                   int single) {
+                   const unsigned char* coverDev, const int* overDev, int nOverCells, int single) {
         if (!single) {
             hipLaunchKernelGGL(k_zone_sizes, dim3(nblk(nZ + 1)), dim3(TPB), 0, stream, rectsDev, nZ, bin.start, p.Ny,
                                zsize);
@@ -1044,10 +1134,6 @@ struct mpcd_gpu_sim {
             hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(TPB), 0, stream, tileSum, nt);
             hipLaunchKernelGGL(k_scan_add, dim3(nblk(nZ + 1)), dim3(TPB), 0, stream, zoff, nZ + 1, tileSum);
             check_launch();
-        } else {
-            /* particles outside the zone keep their velocity */
-            HIP_OK(hipMemcpyAsync(vx2, vx, (size_t)n * sizeof(double), hipMemcpyDeviceToDevice, stream));
-            HIP_OK(hipMemcpyAsync(vy2, vy, (size_t)n * sizeof(double), hipMemcpyDeviceToDevice, stream));
         }
         ZoneArgs a{};
         a.g = g;
@@ -1066,13 +1152,13 @@ struct mpcd_gpu_sim {
         a.scratchCap = scratchCap;
         a.x = x;
         a.y = y;
-        a.vxin = vx;
-        a.vyin = vy;
-        a.vxout = vx2;
-        a.vyout = vy2;
+        a.vx = vx;
+        a.vy = vy;
+        a.vsx = vx2; /* the side buffer: free during a pass */
+        a.vsy = vy2;
+        a.gcover = single ? nullptr : coverDev;
         a.next = next;
         a.tmpg = tmpg;
-        a.slotOf = slotOf;
         a.out = zout;
         a.err = err;
         const size_t bytes = zone_bytes(maxC, maxW, zp.useLocal);
@@ -1092,12 +1178,15 @@ struct mpcd_gpu_sim {
         hipLaunchKernelGGL(k_zone, dim3(nZ), dim3(ZONE_TPB), lds, stream, a);
         pend(MPCD_PROF_ZONE_KERNEL);
         check_launch();
-        std::swap(vx, vx2);
-        std::swap(vy, vy2);
+        if (!single && coverDev && nOverCells > 0) {
+            hipLaunchKernelGGL(k_overlap_copyback, dim3(nblk(nOverCells)), dim3(TPB), 0, stream, overDev, nOverCells,
+                               bin.start, bin.list, vx2, vy2, vx, vy);
+            check_launch();
+        }
     }
 
     void layout_pass(int L, uint64_t seedBase) {
-        zone_pass(seedBase, rects[L], nz[L], maxCells[L], maxCols[L], owner[L], 0);
+        zone_pass(seedBase, rects[L], nz[L], maxCells[L], maxCols[L], owner[L], cover[L], overCells[L], nOver[L], 0);
     }
 
     void pass_metrics(int nZ, mpcd_zone_pass_metrics* m) {
@@ -1129,8 +1218,8 @@ struct mpcd_gpu_sim {
         f.Uy = redUy;
         f.Px = redPx;
         f.Py = redPy;
-        hipLaunchKernelGGL(k_cell_fields, dim3(nblk(Nc)), dim3(TPB), 0, stream, bin.start, bin.list, Nc, vx, vy, vc(),
-                           p.Kvirial, rho_target(), f);
+        hipLaunchKernelGGL(k_cell_fields, dim3(nblk(Nc, CELL_BLK)), dim3(CELL_BLK), cellCap * 2 * sizeof(double), stream,
+                           bin.start, bin.list, Nc, vx, vy, vc(), p.Kvirial, rho_target(), f, cellCap);
         ClosureArgs ca{};
         ca.Nc = Nc;
         ca.gamma = p.gamma;
@@ -1430,8 +1519,9 @@ int mpcd_gpu_compute_cell_fields(const mpcd_params* p, const double* x, const do
         for (auto& q : d) q = dalloc<double>((size_t)s.Nc);
         int* dN = dalloc<int>((size_t)s.Nc);
         FieldsOut f{dN, d[0], d[1], d[2], d[3], d[4], d[5], d[6], d[7], d[8]};
-        hipLaunchKernelGGL(k_cell_fields, dim3(nblk(s.Nc)), dim3(TPB), 0, s.stream, s.bin.start, s.bin.list, s.Nc,
-                           s.vx, s.vy, s.vc(), p->Kvirial, rhoTarget, f);
+        hipLaunchKernelGGL(k_cell_fields, dim3(nblk(s.Nc, CELL_BLK)), dim3(CELL_BLK), s.cellCap * 2 * sizeof(double),
+                           s.stream, s.bin.start, s.bin.list, s.Nc, s.vx, s.vy, s.vc(), p->Kvirial, rhoTarget, f,
+                           s.cellCap);
         s.check_launch();
         const size_t nb = (size_t)s.Nc * sizeof(double);
         if (out->N) HIP_OK(hipMemcpyAsync(out->N, dN, (size_t)s.Nc * sizeof(int), hipMemcpyDeviceToHost, s.stream));
@@ -1476,7 +1566,7 @@ int mpcd_gpu_run_zone_kernel(const mpcd_params* p, double* x, double* v, int xs,
         ZoneRect* d = dalloc<ZoneRect>(1);
         HIP_OK(hipMemcpyAsync(d, &r, sizeof(r), hipMemcpyHostToDevice, s.stream));
         s.clear_err();
-        s.zone_pass(rng_seed, d, 1, (xe - xs) * (ye - ys), xe - xs, nullptr, 1);
+        s.zone_pass(rng_seed, d, 1, (xe - xs) * (ye - ys), xe - xs, nullptr, nullptr, nullptr, 0, 1);
         ZoneOut o{};
         HIP_OK(hipMemcpyAsync(&o, s.zout, sizeof(o), hipMemcpyDeviceToHost, s.stream));
         s.get_state(x, v);
@@ -1514,19 +1604,38 @@ int mpcd_gpu_zone_pass(const mpcd_params* p, double* x, double* v, int layoutMod
             maxC = std::max(maxC, (q.xe - q.xs) * (q.ye - q.ys));
             maxW = std::max(maxW, q.xe - q.xs);
         }
+        std::vector<int> cov((size_t)p->Nc, 0);
+        for (const auto& q : r)
+            for (int ix = q.xs; ix < q.xe; ++ix)
+                for (int iy = q.ys; iy < q.ye; ++iy) cov[(size_t)(iy + p->Ny * ix)]++;
+        std::vector<unsigned char> ov((size_t)p->Nc, 0);
+        std::vector<int> oc;
+        for (int c = 0; c < p->Nc; ++c)
+            if (cov[(size_t)c] > 1) {
+                ov[(size_t)c] = 1;
+                oc.push_back(c);
+            }
         ZoneRect* d = dalloc<ZoneRect>(r.size());
         int* dOwn = dalloc<int>(own.size());
+        unsigned char* dCov = oc.empty() ? nullptr : dalloc<unsigned char>(ov.size());
+        int* dOver = oc.empty() ? nullptr : dalloc<int>(oc.size());
         HIP_OK(hipMemcpyAsync(d, r.data(), r.size() * sizeof(ZoneRect), hipMemcpyHostToDevice, s.stream));
         HIP_OK(hipMemcpyAsync(dOwn, own.data(), own.size() * sizeof(int), hipMemcpyHostToDevice, s.stream));
+        if (dCov) {
+            HIP_OK(hipMemcpyAsync(dCov, ov.data(), ov.size(), hipMemcpyHostToDevice, s.stream));
+            HIP_OK(hipMemcpyAsync(dOver, oc.data(), oc.size() * sizeof(int), hipMemcpyHostToDevice, s.stream));
+        }
         s.ensure_zout((int)r.size());
         s.clear_err();
-        s.zone_pass(rngSeedBase, d, (int)r.size(), maxC, maxW, dOwn, 0);
+        s.zone_pass(rngSeedBase, d, (int)r.size(), maxC, maxW, dOwn, dCov, dOver, (int)oc.size(), 0);
         s.check_err();
         mpcd_zone_pass_metrics mm{};
         s.pass_metrics((int)r.size(), &mm);
         s.get_state(x, v);
         HIP_OK(hipFree(d));
         HIP_OK(hipFree(dOwn));
+        if (dCov) HIP_OK(hipFree(dCov));
+        if (dOver) HIP_OK(hipFree(dOver));
         if (m) *m = mm;
         if (ownerMap) std::memcpy(ownerMap, own.data(), own.size() * sizeof(int32_t));
     });

@@ -94,7 +94,8 @@ extern "C" int core_collide(const mpcd_params* p, double* va, const int32_t* cid
 }
 
 /* One snapshot zone pass as mpcd_engine.hip runs it: per-zone scratch at
- * exact offsets, velocities from the pass input, owned particles written. */
+ * exact offsets, owned particles written, velocities in place except for
+ * cells several zones cover (side buffer, copied back after the pass). */
 extern "C" int core_zone_pass(const mpcd_params* p, double* xa, double* va, int layout, uint64_t seedBase,
                               int32_t* moved2, double* corr2) {
     const Geom g = make_geom(*p);
@@ -102,14 +103,24 @@ extern "C" int core_zone_pass(const mpcd_params* p, double* xa, double* va, int 
     std::vector<double> x, y, vx, vy;
     split(xa, p->n, x, y);
     split(va, p->n, vx, vy);
-    std::vector<double> vxo(vx.size(), -777.0), vyo(vy.size(), -777.0);
+    std::vector<double> vsx(vx.size(), -777.0), vsy(vy.size(), -777.0);
     std::vector<int> cid((size_t)p->n), pid((size_t)p->n);
     std::iota(pid.begin(), pid.end(), 0);
     for (int i = 0; i < p->n; ++i) cid[(size_t)i] = cell_of(g, x[(size_t)i], y[(size_t)i]);
     const Bin b = bin_by(cid, p->Nc);
     std::vector<int32_t> owner;
     const auto rects = layout_rects(*p, layout, &owner);
-    /* per-zone particle counts and offsets */
+    std::vector<int> cov((size_t)p->Nc, 0);
+    for (const auto& r : rects)
+        for (int gx = r.xs; gx < r.xe; ++gx)
+            for (int gy = r.ys; gy < r.ye; ++gy) cov[(size_t)(gy + p->Ny * gx)]++;
+    std::vector<unsigned char> over((size_t)p->Nc, 0);
+    bool anyOver = false;
+    for (int c = 0; c < p->Nc; ++c)
+        if (cov[(size_t)c] > 1) {
+            over[(size_t)c] = 1;
+            anyOver = true;
+        }
     std::vector<int> off(rects.size() + 1, 0);
     for (size_t zi = 0; zi < rects.size(); ++zi) {
         int P = 0;
@@ -117,7 +128,7 @@ extern "C" int core_zone_pass(const mpcd_params* p, double* xa, double* va, int 
             P += b.start[(size_t)(rects[zi].ye - 1 + p->Ny * gx) + 1] - b.start[(size_t)(rects[zi].ys + p->Ny * gx)];
         off[zi + 1] = off[zi] + P;
     }
-    std::vector<int> next((size_t)off.back() + 1), tmpg((size_t)off.back() + 1), slotOf((size_t)off.back() + 1);
+    std::vector<int> next((size_t)off.back() + 1, -999), tmpg((size_t)off.back() + 1, -999);
     int md = 0, ms = 0;
     double acc = 0.0, mx = 0.0;
     int nc = 0;
@@ -144,21 +155,21 @@ extern "C" int core_zone_pass(const mpcd_params* p, double* xa, double* va, int 
         z.owner = owner.data();
         z.x = x.data();
         z.y = y.data();
-        z.vxin = vx.data();
-        z.vyin = vy.data();
-        z.vxout = vxo.data();
-        z.vyout = vyo.data();
+        z.vx = vx.data();
+        z.vy = vy.data();
+        z.vsx = vsx.data();
+        z.vsy = vsy.data();
+        z.gcover = anyOver ? over.data() : nullptr;
         z.next = next.data();
         z.tmpg = tmpg.data();
-        z.slotOf = slotOf.data();
         std::vector<unsigned char> buf(zone_bytes(z.C, z.zw, zp.useLocal) + 16);
         zone_carve(z, buf.data(), z.C, z.zw, zp.useLocal);
         zone_setup(z, 0, 1);
         if (z.P != off[zi + 1] - off[zi]) return -1;
-        zone_setup_links(z, 0, 1);
         zone_dense_passes(z, 0, 1);
         zone_sparse_pass(z, 0, 1);
-        for (int lc = 0; lc < z.C; ++lc) zone_correct_cell(z, lc);
+        for (int lc = 0; lc < z.C; ++lc) zone_correct_sums(z, lc);
+        for (int lc = 0; lc < z.C; ++lc) zone_correct_write(z, lc);
         zone_final_stats(z);
         md += z.movedDense;
         ms += z.movedSparse;
@@ -168,10 +179,17 @@ extern "C" int core_zone_pass(const mpcd_params* p, double* xa, double* va, int 
         }
         mx = std::max(mx, z.stats5);
     }
-    for (size_t i = 0; i < vxo.size(); ++i)
-        if (vxo[i] == -777.0) return -2; /* a particle no zone owns */
+    for (int c = 0; c < p->Nc; ++c) {
+        if (!over[(size_t)c]) continue;
+        for (int j = b.start[(size_t)c]; j < b.start[(size_t)c + 1]; ++j) {
+            const int slot = b.list[(size_t)j];
+            if (vsx[(size_t)slot] == -777.0) return -2; /* an overlap particle no zone wrote */
+            vx[(size_t)slot] = vsx[(size_t)slot];
+            vy[(size_t)slot] = vsy[(size_t)slot];
+        }
+    }
     join(x, y, xa);
-    join(vxo, vyo, va);
+    join(vx, vy, va);
     if (moved2) {
         moved2[0] = md;
         moved2[1] = ms;
