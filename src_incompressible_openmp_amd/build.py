@@ -21,7 +21,12 @@ CXX_SOURCES = [os.path.join(CSRC, "mpcd_api.cpp"), os.path.join(CSRC, "host_io.c
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 COMMON = ["-std=c++17", "-O3", "-fPIC", "-ffp-contract=off", "-I" + os.path.join(ROOT, "include"),
           "-I" + os.path.join(ROOT, "include", "mpcd"), "-I" + CSRC]
-HIP_FLAGS = ["--offload-arch=gfx950", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
+# -simplifycfg-sink-common=false: merging the two arms of a branch that write
+# through different fields of a kernel's per-lane context turns the field
+# choice into a computed offset, which keeps the context in private memory and
+# makes the zone kernel's working-set accesses generic (flat) ones.
+HIP_FLAGS = ["--offload-arch=gfx950", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+             "-mllvm", "-simplifycfg-sink-common=false"]
 
 
 def build(verbose: bool = False) -> str:

@@ -42,6 +42,9 @@ struct HipError : std::runtime_error {
 constexpr int TPB = 256;
 constexpr int ZONE_TPB = 64;
 constexpr int ZONE_LDS_CAP = 64 * 1024;
+/* LDS of one zone workgroup on the uint16_t path: four fit a CU's 160 KB with
+ * the static shared scalars and allocation rounding */
+constexpr int ZONE_LDS_FAST = 38 * 1024;
 constexpr int SCAN_TILE = 1024; /* 256 threads x 4 */
 
 inline unsigned nblk(long long n, int tpb = TPB) { return (unsigned)((n + tpb - 1) / tpb); }
@@ -442,28 +445,24 @@ __global__ void __launch_bounds__(CELL_BLK) k_cell_fields(const int* __restrict_
 /* ---- zone pass: one workgroup (one wavefront) per zone ---- */
 
 struct ZoneArgs {
-    Geom g;
-    ZoneParams zp;
+    ZoneIn in;
     const ZoneRect* rects;
     int nZones;
     uint64_t seedBase;
     int singleSeed; /* 1: seedBase is the zone's own seed (single-zone call) */
-    const int* gcnt;
-    const int* gstart;
-    const int* glist;
-    const int* gpid;
-    const int* gcid;
-    const int* owner; /* nullptr: every zone owns all its particles */
-    const int* zoff;  /* per-zone scratch offsets (exclusive scan of sizes) */
+    const int* zsize; /* particles of each zone */
+    const int* zoff;  /* per-zone offsets into the particle regions (exclusive scan of zsize) */
     long long scratchCap;
-    double *x, *y, *vx, *vy;
-    double *vsx, *vsy;
-    const unsigned char* gcover;
-    int *next, *tmpg;
+    void* nextG;    /* links beyond LDS, IT per particle */
+    void* tmpG;     /* long donor lists, IT per particle */
+    double* duG;    /* 2 per cell per zone */
+    int duStride;   /* doubles per zone */
+    int pMin, pMax; /* zones with pMin <= P <= pMax run in this launch */
+    size_t ldsBytes;
+    unsigned char* gscratch; /* working sets in global memory when they do not fit LDS */
+    size_t gstride;
     ZoneOut* out;
     int* err;
-    unsigned char* gscratch; /* used instead of LDS when the zone arrays do not fit */
-    size_t gstride;
 };
 
 /* particles of each zone: its columns' runs of the binning list */
@@ -481,70 +480,43 @@ __global__ void k_zone_sizes(const ZoneRect* __restrict__ rects, int nZ, const i
     sizes[zi] = P;
 }
 
+/* One zone per workgroup of one wavefront.  IT = uint16_t keeps the working
+ * set in LDS at four zones per CU; IT = int serves zones of 65535 particles
+ * or more (and the single-zone layout), in LDS or in global scratch. */
+template <class IT, bool kLds>
 __global__ void __launch_bounds__(ZONE_TPB) k_zone(ZoneArgs a) {
     extern __shared__ __align__(16) unsigned char lds[];
+    __shared__ ZoneShared sh;
     const int zi = blockIdx.x;
     if (zi >= a.nZones) return;
-    const int lane = threadIdx.x, nl = ZONE_TPB;
-    __shared__ ZoneCtx z;
-    __shared__ int ok;
-    if (lane == 0) {
-        const ZoneRect r = a.rects[zi];
-        z.g = a.g;
-        z.zp = a.zp;
-        z.xs = r.xs;
-        z.xe = r.xe;
-        z.ys = r.ys;
-        z.ye = r.ye;
-        z.zw = r.xe - r.xs;
-        z.zh = r.ye - r.ys;
-        z.C = z.zw * z.zh;
-        z.zid = zi + 1;
-        z.off = a.zoff ? a.zoff[zi] : 0;
-        z.seed = a.singleSeed ? a.seedBase : a.seedBase + 104729ULL * (uint64_t)(zi + 1);
-        z.gcnt = a.gcnt;
-        z.gstart = a.gstart;
-        z.glist = a.glist;
-        z.gpid = a.gpid;
-        z.gcid = a.gcid;
-        z.owner = a.owner;
-        z.x = a.x;
-        z.y = a.y;
-        z.vx = a.vx;
-        z.vy = a.vy;
-        z.vsx = a.vsx;
-        z.vsy = a.vsy;
-        z.gcover = a.gcover;
-        z.next = a.next;
-        z.tmpg = a.tmpg;
-        unsigned char* base = a.gscratch ? a.gscratch + (size_t)zi * a.gstride : lds;
-        zone_carve(z, base, z.C, z.zw, a.zp.useLocal);
+    const int P = a.zsize[zi];
+    if (P < a.pMin || P > a.pMax) return; /* uniform: another launch takes this zone */
+    const int lane = threadIdx.x;
+    const long long off = a.zoff[zi];
+    if (off + (long long)P > a.scratchCap) {
+        if (lane == 0) atomicOr(a.err, 2);
+        return;
     }
+    ZoneCtxT<IT> z;
+    const uint64_t seed = a.singleSeed ? a.seedBase : a.seedBase + 104729ULL * (uint64_t)(zi + 1);
+    zone_init(z, a.in, a.rects[zi], zi + 1, seed, &sh);
+    z.nxG = (IT*)a.nextG + off;
+    z.tmpG = (IT*)a.tmpG + off;
+    z.duG = a.duG + (size_t)zi * (size_t)a.duStride;
+    if (kLds) zone_carve(z, lds, a.ldsBytes);
+    else zone_carve(z, a.gscratch + (size_t)zi * a.gstride, a.gstride);
+    zone_setup(z, a.in, lane, ZONE_TPB);
     __syncthreads();
-    zone_setup(z, lane, nl);
-    __syncthreads();
-    if (lane == 0) {
-        ok = ((long long)z.off + (long long)z.P <= a.scratchCap) ? 1 : 0;
-        if (!ok) atomicOr(a.err, 2);
-    }
-    __syncthreads();
-    if (!ok) return; /* uniform across the workgroup */
-    zone_dense_passes(z, lane, nl);
-    zone_sparse_pass(z, lane, nl);
-    __syncthreads();
-    for (int lc = lane; lc < z.C; lc += nl) zone_correct_sums(z, lc);
-    __syncthreads();
-    for (int lc = lane; lc < z.C; lc += nl) zone_correct_write(z, lc);
+    zone_run(z, lane, ZONE_TPB);
     __syncthreads();
     if (lane == 0) {
-        zone_final_stats(z);
         ZoneOut o;
-        o.nHigh = z.nHigh;
-        o.nLow = z.nLow;
-        o.movedDense = z.movedDense;
-        o.movedSparse = z.movedSparse;
-        o.stats4 = z.stats4;
-        o.stats5 = z.stats5;
+        o.nHigh = sh.nHigh;
+        o.nLow = sh.nLow;
+        o.movedDense = sh.movedDense;
+        o.movedSparse = sh.movedSparse;
+        o.stats4 = sh.stats4;
+        o.stats5 = sh.stats5;
         a.out[zi] = o;
     }
 }
@@ -824,6 +796,8 @@ struct mpcd_gpu_sim {
     int maxCells[2] = {0, 0}, maxCols[2] = {0, 0};
     ZoneOut* zout = nullptr;
     int zoutCap = 0;
+    double* duG = nullptr;
+    size_t duCap = 0;
     unsigned char* zscratch = nullptr;
     size_t zscratchCap = 0;
     ZoneParams zp{};
@@ -961,6 +935,7 @@ struct mpcd_gpu_sim {
         zsize = dalloc<int>((size_t)std::max(nz[0], nz[1]) + 2);
         zoff = dalloc<int>((size_t)std::max(nz[0], nz[1]) + 2);
         ensure_zout(std::max(nz[0], nz[1]));
+        ensure_du(2 * (size_t)std::max((size_t)nz[0] * (size_t)maxCells[0], (size_t)nz[1] * (size_t)maxCells[1]));
         const auto m = interzone_mask_host(p);
         nMasked = (int)std::count(m.begin(), m.end(), 1);
         anyMask = nMasked > 0;
@@ -972,7 +947,7 @@ struct mpcd_gpu_sim {
         void* ptrs[] = {x, y, vx, vy, vx2, vy2, x2, y2, pid, pid2, cid, cidS, bin.cnt, bin.start, bin.cursor, bin.list, tileSum, next,
                         tmpg, zsize, zoff, owner[0], owner[1], cover[0], cover[1], overCells[0], overCells[1], stage,
                         refPx, refPy, refUx, refUy, pdriveRaw, redPx, redPy, redUx, redUy, duRX, duRY, duEX, duEY,
-                        pdrive, cellTmp, mask, part, scal, err, wallPart, rects[0], rects[1], zout, zscratch};
+                        pdrive, cellTmp, mask, part, scal, err, wallPart, rects[0], rects[1], zout, zscratch, duG};
         for (void* q : ptrs)
             if (q) (void)hipFree(q);
         for (hipEvent_t e : evPool) (void)hipEventDestroy(e);
@@ -984,6 +959,12 @@ struct mpcd_gpu_sim {
         if (part) HIP_OK(hipFree(part));
         part = dalloc<double>(m);
         partCap = m;
+    }
+    void ensure_du(size_t m) {
+        if (m <= duCap) return;
+        if (duG) HIP_OK(hipFree(duG));
+        duG = dalloc<double>(m);
+        duCap = m;
     }
     void ensure_zout(int m) {
         if (m <= zoutCap) return;
@@ -1126,56 +1107,70 @@ struct mpcd_gpu_sim {
      * particles, seeded with seedBase itself (run_zone_kernel_inplace). */
     void zone_pass(uint64_t seedBase, const ZoneRect* rectsDev, int nZ, int maxC, int maxW, const int* ownerDev,
                    const unsigned char* coverDev, const int* overDev, int nOverCells, int single) {
-        if (!single) {
-            hipLaunchKernelGGL(k_zone_sizes, dim3(nblk(nZ + 1)), dim3(TPB), 0, stream, rectsDev, nZ, bin.start, p.Ny,
-                               zsize);
-            const int nt = (nZ + 1 + SCAN_TILE - 1) / SCAN_TILE;
-            hipLaunchKernelGGL(k_scan_tiles, dim3(nt), dim3(TPB), 0, stream, zsize, nZ + 1, zoff, tileSum);
-            hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(TPB), 0, stream, tileSum, nt);
-            hipLaunchKernelGGL(k_scan_add, dim3(nblk(nZ + 1)), dim3(TPB), 0, stream, zoff, nZ + 1, tileSum);
-            check_launch();
-        }
+        hipLaunchKernelGGL(k_zone_sizes, dim3(nblk(nZ + 1)), dim3(TPB), 0, stream, rectsDev, nZ, bin.start, p.Ny,
+                           zsize);
+        const int nt = (nZ + 1 + SCAN_TILE - 1) / SCAN_TILE;
+        hipLaunchKernelGGL(k_scan_tiles, dim3(nt), dim3(TPB), 0, stream, zsize, nZ + 1, zoff, tileSum);
+        hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(TPB), 0, stream, tileSum, nt);
+        hipLaunchKernelGGL(k_scan_add, dim3(nblk(nZ + 1)), dim3(TPB), 0, stream, zoff, nZ + 1, tileSum);
+        check_launch();
+        ensure_du((size_t)nZ * 2 * (size_t)maxC);
         ZoneArgs a{};
-        a.g = g;
-        a.zp = zp;
+        a.in.g = g;
+        a.in.zp = zp;
+        a.in.gcnt = bin.cnt;
+        a.in.gstart = bin.start;
+        a.in.glist = bin.list;
+        a.in.gpid = pid;
+        a.in.owner = single ? nullptr : ownerDev;
+        a.in.gcover = single ? nullptr : coverDev;
+        a.in.x = x;
+        a.in.y = y;
+        a.in.vx = vx;
+        a.in.vy = vy;
+        a.in.vsx = vx2; /* the side buffer: free during a pass */
+        a.in.vsy = vy2;
         a.rects = rectsDev;
         a.nZones = nZ;
         a.seedBase = seedBase;
         a.singleSeed = single;
-        a.gcnt = bin.cnt;
-        a.gstart = bin.start;
-        a.glist = bin.list;
-        a.gpid = pid;
-        a.gcid = cid;
-        a.owner = single ? nullptr : ownerDev;
-        a.zoff = single ? nullptr : zoff;
+        a.zsize = zsize;
+        a.zoff = zoff;
         a.scratchCap = scratchCap;
-        a.x = x;
-        a.y = y;
-        a.vx = vx;
-        a.vy = vy;
-        a.vsx = vx2; /* the side buffer: free during a pass */
-        a.vsy = vy2;
-        a.gcover = single ? nullptr : coverDev;
-        a.next = next;
-        a.tmpg = tmpg;
+        a.nextG = next;
+        a.tmpG = tmpg;
+        a.duG = duG;
+        a.duStride = 2 * maxC;
         a.out = zout;
         a.err = err;
-        const size_t bytes = zone_bytes(maxC, maxW, zp.useLocal);
-        size_t lds = bytes;
-        if (bytes > (size_t)ZONE_LDS_CAP) {
-            const size_t need = bytes * (size_t)nZ;
+        pbeg(MPCD_PROF_ZONE_KERNEL);
+        const size_t fix16 = zone_fixed_bytes<uint16_t>(maxC, maxW, zp.useLocal);
+        const bool fast = maxC < 65535 && fix16 + 2048 <= (size_t)ZONE_LDS_FAST;
+        if (fast) {
+            a.pMin = 0;
+            a.pMax = 65534;
+            a.ldsBytes = ZONE_LDS_FAST;
+            hipLaunchKernelGGL((k_zone<uint16_t, true>), dim3(nZ), dim3(ZONE_TPB), ZONE_LDS_FAST, stream, a);
+            check_launch();
+        }
+        /* zones the uint16_t path does not take (none, as a rule) */
+        a.pMin = fast ? 65535 : 0;
+        a.pMax = 0x7fffffff;
+        const size_t fix32 = zone_fixed_bytes<int>(maxC, maxW, zp.useLocal);
+        if (fix32 <= (size_t)ZONE_LDS_CAP) {
+            a.ldsBytes = ZONE_LDS_CAP;
+            hipLaunchKernelGGL((k_zone<int, true>), dim3(nZ), dim3(ZONE_TPB), ZONE_LDS_CAP, stream, a);
+        } else {
+            const size_t need = fix32 * (size_t)nZ;
             if (need > zscratchCap) {
                 if (zscratch) HIP_OK(hipFree(zscratch));
                 zscratch = dalloc<unsigned char>(need);
                 zscratchCap = need;
             }
             a.gscratch = zscratch;
-            a.gstride = bytes;
-            lds = 0;
+            a.gstride = fix32;
+            hipLaunchKernelGGL((k_zone<int, false>), dim3(nZ), dim3(ZONE_TPB), 0, stream, a);
         }
-        pbeg(MPCD_PROF_ZONE_KERNEL);
-        hipLaunchKernelGGL(k_zone, dim3(nZ), dim3(ZONE_TPB), lds, stream, a);
         pend(MPCD_PROF_ZONE_KERNEL);
         check_launch();
         if (!single && coverDev && nOverCells > 0) {

@@ -11,6 +11,7 @@
 #include <vector>
 
 #include "mpcd_core.h"
+#include "mpcd_zone.h"
 
 namespace mpcdk {
 
@@ -145,6 +146,10 @@ inline ZoneParams make_zone_params(const mpcd_params& p) {
     zp.useLocal = p.useLocalFluidFractionThresholds;
     zp.maxPasses = p.maxRedistribPasses;
     zp.corrEnabled = p.enableMomentumCorrectionPostRedistribution;
+    zp.dedupe = ((p.bc_left == MPCD_BC_PERIODIC && p.bc_right == MPCD_BC_PERIODIC && p.Nx < 5) ||
+                 (p.bc_bottom == MPCD_BC_PERIODIC && p.bc_top == MPCD_BC_PERIODIC && p.Ny < 5))
+                    ? 1
+                    : 0;
     { /* empty-cell thresholds at fluid fraction 1 (zone_pass.cpp:334-342) */
         const double gl = p.gamma * 1.0;
         const double lo = zp.lowFrac * gl, hi = zp.highFrac * gl;
@@ -157,7 +162,9 @@ inline ZoneParams make_zone_params(const mpcd_params& p) {
             const double ang = std::atan2((double)dy, (double)dx);
             int ib = (int)std::floor((ang + M_PI) / (2.0 * M_PI / 8.0));
             ib = ((ib % 8) + 8) % 8;
-            zp.binTab[(dy + 2) * 5 + (dx + 2)] = (signed char)ib;
+            const int idx = (dy + 2) * 5 + (dx + 2);
+            if (idx < 21) zp.binLo |= (unsigned long long)ib << (3 * idx);
+            else zp.binHi |= (unsigned long long)ib << (3 * (idx - 21));
         }
     return zp;
 }

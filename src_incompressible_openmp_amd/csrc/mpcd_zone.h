@@ -1,0 +1,1281 @@
+/*
+ * mpcd_zone.h -- one zone of the snapshot zone redistribution
+ * (zone_pass.cpp:62-973) on one wavefront, shared by the gfx950 kernel in
+ * mpcd_engine.hip and the host logic check in tests/ (ZHD code).
+ *
+ * The reference's zone kernel is sequential: a shuffled list of dense cells,
+ * each of which mutates the counts its successors read.  That order is kept;
+ * inside each step the wavefront works together wherever the work is
+ * independent by construction: all counter-based draws of a step (shuffle
+ * partners, tie breaks, score noise, new positions), candidate evaluation and
+ * ranking, order-preserving compactions and prefix sums, the momentum
+ * correction.  Only list bookkeeping runs on lane 0, and it touches LDS only:
+ * the slots, ownership and new positions of the particles it moves are
+ * staged by all lanes beforehand.  The same code runs on the host with one
+ * lane (nl = 1), where the wave primitives reduce to scalars and the barrier
+ * to nothing.
+ *
+ * Memory.  Cell arrays are indexed by the zone-local cell
+ * lc = (ix-xs)*zh + (iy-ys) (ascending global cell id); particles by the
+ * zone-local index k (the zone's cells' runs of the pass-input binning list,
+ * in lc order).  Index arrays use the type IT: uint16_t when the zone has
+ * fewer than 65535 cells and particles (the cell arrays and most list links
+ * then fit in LDS next to three other zones on a CU), int otherwise.  List
+ * links of k < nL live in LDS, the rest in the zone's private global region.
+ *
+ * Snapshot semantics.  The shifted layout's zones overlap
+ * (zone_pass.cpp:191-219 starts them at 0 and T/2 with width T), so two zones
+ * may hold the same particle.  Every zone reads the pass input; a zone writes
+ * only the particles it owns, i.e. whose input cell has this zone as its
+ * owner-map entry (the last zone covering the cell, which is the zone whose
+ * result the reference's merge keeps, zone_pass_openmp.cpp:145-153).
+ * Positions are written in place (nobody reads them during a pass).
+ * Velocities are written in place for particles whose input cell only one
+ * zone covers; for cells several zones cover they go to vsx/vsy and are
+ * copied back after the pass, so every zone reads inputs.
+ *
+ * Lists.  A cell's list stays implicit -- a run of consecutive local
+ * indices head..tail, initially its input run -- while only its head is
+ * popped; the first append or rewrite writes its links (ZF_MAT).  ZF_CHG
+ * marks every cell whose list changed; the others cost nothing at the
+ * correction.
+ */
+#pragma once
+
+#include <stdint.h>
+
+#include "mpcd_core.h"
+
+/* every zone function is inlined into the kernel, so the per-lane zone
+ * context stays in registers */
+#if defined(__HIPCC__)
+#define ZHD __host__ __device__ inline __attribute__((always_inline))
+#else
+#define ZHD inline
+#endif
+
+namespace mpcdk {
+
+#if defined(__HIP_DEVICE_COMPILE__)
+#define ZBAR() __syncthreads()
+ZHD unsigned long long wave_ballot(bool f) { return __ballot(f); }
+ZHD int wave_popc(unsigned long long m) { return __popcll(m); }
+ZHD int wave_shfl_up_add(int v, int lane) {
+    for (int o = 1; o < 64; o <<= 1) {
+        const int t = __shfl_up(v, o, 64);
+        if (lane >= o) v += t;
+    }
+    return v;
+}
+ZHD int wave_bcast(int v, int src) { return __shfl(v, src, 64); }
+/* value of lane l (l uniform across the wavefront) */
+ZHD int wave_readlane(int v, int l) { return __builtin_amdgcn_readlane(v, l); }
+ZHD double wave_readlane_d(double v, int l) {
+    const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)b, l);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(b >> 32), l);
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+ZHD int wave_sum(int v) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+ZHD double wave_max(double v) {
+    for (int o = 32; o > 0; o >>= 1) {
+        const double t = __shfl_xor(v, o, 64);
+        v = (v < t) ? t : v;
+    }
+    return v;
+}
+#else
+#define ZBAR() ((void)0)
+ZHD unsigned long long wave_ballot(bool f) { return f ? 1ull : 0ull; }
+ZHD int wave_popc(unsigned long long m) { return __builtin_popcountll(m); }
+ZHD int wave_shfl_up_add(int v, int) { return v; }
+ZHD int wave_bcast(int v, int) { return v; }
+ZHD int wave_readlane(int v, int) { return v; }
+ZHD double wave_readlane_d(double v, int) { return v; }
+ZHD int wave_sum(int v) { return v; }
+ZHD double wave_max(double v) { return v; }
+#endif
+ZHD unsigned long long lanes_below(int lane) { return (lane >= 64) ? ~0ull : ((1ull << lane) - 1ull); }
+
+struct ZoneParams {
+    double highThr, lowThrBulk;
+    double lowFrac, highFrac; /* 1/(1+coef), 1+coef */
+    double gamma, fluidFracGain, lowThrFloor, highThrFloor;
+    double lowLoc1, highLoc1; /* empty-cell thresholds for fluid fraction 1 */
+    int useLocal, maxPasses, corrEnabled;
+    int dedupe; /* a periodic axis narrower than 5 cells: wrapped neighbours can repeat */
+    /* angular bin (3 bits) of (dx,dy) in [-2,2]^2 at index (dy+2)*5+(dx+2):
+     * indices 0..20 in binLo, 21..24 in binHi (registers, not an array that
+     * a dynamic index would send to private memory) */
+    unsigned long long binLo, binHi;
+};
+
+ZHD int zone_bin(const ZoneParams& zp, int idx) {
+    return idx < 21 ? (int)((zp.binLo >> (3 * idx)) & 7ull) : (int)((zp.binHi >> (3 * (idx - 21))) & 7ull);
+}
+
+/* Candidate of a dense cell's target ranking. */
+struct Cand {
+    int lc, gc;
+    double deficitNeg;
+    int cnt, cheb, dist2, bin;
+    double tie;
+};
+
+struct ZoneRect {
+    int xs, xe, ys, ye;
+};
+
+struct ZoneOut {
+    int nHigh, nLow, movedDense, movedSparse;
+    double stats4, stats5;
+};
+
+/* (overridable so that a host build can exercise the paths past them) */
+#ifndef MPCD_ZONE_MAX_MOVE
+#define MPCD_ZONE_MAX_MOVE 64
+#endif
+#ifndef MPCD_ZONE_TMP_L
+#define MPCD_ZONE_TMP_L 256
+#endif
+constexpr int ZONE_MAX_MOVE = MPCD_ZONE_MAX_MOVE; /* dense movers staged per source cell */
+constexpr int ZONE_TMP_L = MPCD_ZONE_TMP_L;       /* donor lists up to this length are shuffled in LDS */
+
+/* per-cell flag bits */
+enum : unsigned short {
+    ZF_BULK0 = 1,   /* occupied when this pass's maps were built */
+    ZF_RECV = 2,    /* received particles */
+    ZF_BB = 4,      /* bulk-to-bulk transfer */
+    ZF_SKIP = 8,    /* transfer touching a non-bulk cell */
+    ZF_MAT = 16,    /* list links written */
+    ZF_CORR = 32,   /* momentum correction applies */
+    ZF_CHG = 64,    /* list changed */
+    ZF_OWN = 128,   /* this zone owns the cell's input particles */
+    ZF_COVER = 256  /* several zones cover the cell */
+};
+
+/* Inputs of a zone pass common to all its zones. */
+struct ZoneIn {
+    Geom g;
+    ZoneParams zp;
+    const int* gcnt;    /* pass-input binning: counts, run starts, slots */
+    const int* gstart;
+    const int* glist;
+    const int* gpid;    /* particle id of each slot */
+    const int* owner;   /* owner map of the layout, or nullptr: each zone owns its cells */
+    const unsigned char* gcover; /* per cell: several zones cover it, or nullptr */
+    double *x, *y, *vx, *vy;
+    double *vsx, *vsy;  /* side buffer for covered cells */
+};
+
+/* Scalars the lanes share (LDS on the device). */
+struct ZoneShared {
+    int nOrd, startBin, nr, stage, P, allOwned;
+    int nHigh, nLow, movedDense, movedSparse;
+    double stats4, stats5;
+};
+
+template <class IT>
+struct ZoneCtxT {
+    typedef IT idx_t;
+    Geom g;
+    ZoneParams zp;
+    int xs, xe, ys, ye, zw, zh, C;
+    int zid, P, nL;
+    int allOwned; /* copied from the shared set-up result */
+    uint64_t seed;
+    const int *gcnt, *gstart, *glist, *gpid;
+    double *x, *y, *vx, *vy, *vsx, *vsy;
+    IT* nxL;     /* links of k < nL */
+    IT* nxG;     /* links of k >= nL (zone's global region, indexed by k) */
+    IT* tmpG;    /* donor lists longer than ZONE_TMP_L (zone's global region) */
+    double* duG; /* velocity change per cell (zone's global region, 2 per cell) */
+    /* cell arrays */
+    IT *cnt, *head, *tail, *dl, *cntS, *k0;
+    unsigned short* fl;
+    int *colL, *colG; /* per column: first local index, start in the binning list */
+    double *lowA, *highA; /* local fluid-fraction thresholds (only with useLocal) */
+    /* small scratch */
+    Cand* cand;     /* 24 */
+    int* ord;       /* 24 */
+    double* ordThr; /* 24 */
+    int* nbGc;      /* 24 */
+    double* nbTie;  /* 24 */
+    int* rnk;       /* 24 */
+    int* recvT;     /* 24 */
+    double* dsc;    /* 8 */
+    int* dord;      /* 8 */
+    int* dord2;     /* 8 */
+    double* nsm4;   /* 4 */
+    double *mvx, *mvy; /* ZONE_MAX_MOVE: staged dense movers' new-position draws */
+    int* mvk;          /* local index */
+    int* mvs;          /* slot */
+    unsigned short* mvo; /* ZF_OWN / ZF_COVER of the input cell */
+    IT* tmpL;          /* ZONE_TMP_L */
+    ZoneShared* sh;
+};
+
+/* ---- index and flag access ---- */
+
+template <class IT>
+ZHD int iget(IT v) {
+    return v == (IT)-1 ? -1 : (int)v;
+}
+template <class Z>
+ZHD bool fget(const Z& z, int lc, unsigned short b) {
+    return (z.fl[lc] & b) != 0;
+}
+template <class Z>
+ZHD void fset(Z& z, int lc, unsigned short b) {
+    z.fl[lc] = (unsigned short)(z.fl[lc] | b);
+}
+template <class Z>
+ZHD void fclr(Z& z, int lc, unsigned short b) {
+    z.fl[lc] = (unsigned short)(z.fl[lc] & ~b);
+}
+template <class Z>
+ZHD int cnt_of(const Z& z, int lc) {
+    return (int)z.cnt[lc];
+}
+template <class Z>
+ZHD void cnt_set(Z& z, int lc, int v) {
+    z.cnt[lc] = (typename Z::idx_t)v;
+}
+template <class Z>
+ZHD int head_of(const Z& z, int lc) {
+    return iget(z.head[lc]);
+}
+template <class Z>
+ZHD int tail_of(const Z& z, int lc) {
+    return iget(z.tail[lc]);
+}
+template <class Z>
+ZHD void head_set(Z& z, int lc, int k) {
+    z.head[lc] = (typename Z::idx_t)k;
+}
+template <class Z>
+ZHD void tail_set(Z& z, int lc, int k) {
+    z.tail[lc] = (typename Z::idx_t)k;
+}
+/* Accesses to the zone's global regions.  On the device they are
+ * non-temporal operations: distinct from the working-set accesses, so the
+ * compiler never merges the two into one generic (flat) access whose wait
+ * would also stall on every outstanding global load. */
+#if defined(__HIP_DEVICE_COMPILE__)
+#define ZG_LOAD(p) __builtin_nontemporal_load(p)
+#define ZG_STORE(p, v) __builtin_nontemporal_store((v), (p))
+#else
+#define ZG_LOAD(p) (*(p))
+#define ZG_STORE(p, v) (*(p) = (v))
+#endif
+
+/* The link accessors read both base pointers before choosing: a choice
+ * between two fields of the context (rather than two loaded values) would
+ * keep the context out of registers. */
+template <class Z>
+ZHD int nx_get(const Z& z, int k) {
+    const typename Z::idx_t* const L = z.nxL;
+    const typename Z::idx_t* const G = z.nxG;
+    if (k < z.nL) return iget(L[k]);
+    return iget(ZG_LOAD(G + k));
+}
+template <class Z>
+ZHD void nx_set(Z& z, int k, int v) {
+    typename Z::idx_t* const L = z.nxL;
+    typename Z::idx_t* const G = z.nxG;
+    if (k < z.nL) L[k] = (typename Z::idx_t)v;
+    else ZG_STORE(G + k, (typename Z::idx_t)v);
+}
+
+/* ---- geometry ---- */
+
+template <class Z>
+ZHD int zone_lc(const Z& z, int gx, int gy) {
+    if (gx >= z.xs && gx < z.xe && gy >= z.ys && gy < z.ye) return (gx - z.xs) * z.zh + (gy - z.ys);
+    return -1;
+}
+template <class Z>
+ZHD int lc_gx(const Z& z, int lc) {
+    return z.xs + lc / z.zh;
+}
+template <class Z>
+ZHD int lc_gy(const Z& z, int lc) {
+    return z.ys + lc % z.zh;
+}
+template <class Z>
+ZHD int lc_gc(const Z& z, int lc) {
+    return lc_gy(z, lc) + z.g.Ny * lc_gx(z, lc);
+}
+
+/* count at a global cell: live zone count (or the sparse-phase copy), else
+ * the pass-input snapshot. */
+template <class Z>
+ZHD int count_at(const Z& z, int gx, int gy, int use_copy) {
+    const int lc = zone_lc(z, gx, gy);
+    if (lc >= 0) return use_copy ? (int)z.cntS[lc] : (int)z.cnt[lc];
+    return z.gcnt[gy + z.g.Ny * gx];
+}
+
+/* 3x3 binomial smoothing without wrap (zone_pass.cpp:296-319). */
+template <class Z>
+ZHD double nsm_at(const Z& z, int gx, int gy, int use_copy) {
+    double acc = 0.0, ws = 0.0;
+    for (int dy = -1; dy <= 1; ++dy)
+        for (int dx = -1; dx <= 1; ++dx) {
+            const int jy = gy + dy, jx = gx + dx;
+            if (jy < 0 || jy >= z.g.Ny || jx < 0 || jx >= z.g.Nx) continue;
+            const double K = (double)((dx == 0 ? 2 : 1) * (dy == 0 ? 2 : 1)) / 16.0; /* 1 2 1 / 2 4 2 / 1 2 1 */
+            acc += K * (double)count_at(z, jx, jy, use_copy);
+            ws += K;
+        }
+    return acc / dmax(ws, 1e-30);
+}
+
+/* Active thresholds of a zone cell (zone_pass.cpp:321-345, :715-725): the
+ * bulk flag is taken when the maps are built; with fluid fraction 1 the
+ * empty-cell values are constants, otherwise they are stored per cell. */
+template <class Z>
+ZHD void zone_maps(Z& z, int lc, int use_copy) {
+    const int bulk = z.cnt[lc] > 0;
+    if (bulk) fset(z, lc, ZF_BULK0);
+    else fclr(z, lc, ZF_BULK0);
+    if (!z.zp.useLocal) return;
+    const double fRaw = nsm_at(z, lc_gx(z, lc), lc_gy(z, lc), use_copy) / dmax(z.zp.gamma, 1e-12);
+    const double t = z.zp.fluidFracGain * fRaw;
+    const double f = (t < 0.0) ? 0.0 : ((1.0 < t) ? 1.0 : t); /* std::clamp(t, 0, 1) */
+    const double gl = z.zp.gamma * f;
+    z.lowA[lc] = bulk ? z.zp.lowThrBulk : dmax(z.zp.lowFrac * gl, z.zp.lowThrFloor);
+    z.highA[lc] = bulk ? z.zp.highThr : dmax(z.zp.highFrac * gl, z.zp.highThrFloor);
+}
+template <class Z>
+ZHD double low_thr(const Z& z, int lc) {
+    if (z.zp.useLocal) return z.lowA[lc];
+    const double b = z.zp.lowThrBulk, e = z.zp.lowLoc1;
+    return fget(z, lc, ZF_BULK0) ? b : e;
+}
+template <class Z>
+ZHD double high_thr(const Z& z, int lc) {
+    if (z.zp.useLocal) return z.highA[lc];
+    const double b = z.zp.highThr, e = z.zp.highLoc1;
+    return fget(z, lc, ZF_BULK0) ? b : e;
+}
+template <class Z>
+ZHD double thr(const Z& z, int lc, int high) {
+    return high ? high_thr(z, lc) : low_thr(z, lc);
+}
+
+/* zone_pass.cpp:229-244 */
+template <class Z>
+ZHD void min_step(const Z& z, int sx, int sy, int tx, int ty, int& ddx, int& ddy) {
+    ddx = tx - sx;
+    ddy = ty - sy;
+    if (z.g.perx) {
+        if (ddx > z.g.Nx / 2) ddx -= z.g.Nx;
+        if (ddx < -z.g.Nx / 2) ddx += z.g.Nx;
+    }
+    if (z.g.pery) {
+        if (ddy > z.g.Ny / 2) ddy -= z.g.Ny;
+        if (ddy < -z.g.Ny / 2) ddy += z.g.Ny;
+    }
+}
+
+/* zone_pass.cpp:246-261 */
+template <class Z>
+ZHD int shift_idx(const Z& z, int gx, int gy, int dx, int dy, int& jx, int& jy) {
+    jy = gy + dy;
+    if (z.g.pery) jy = wrapi(jy, z.g.Ny);
+    else if (jy < 0 || jy >= z.g.Ny) return 0;
+    jx = gx + dx;
+    if (z.g.perx) jx = wrapi(jx, z.g.Nx);
+    else if (jx < 0 || jx >= z.g.Nx) return 0;
+    return 1;
+}
+
+/* ---- particles ---- */
+
+/* column of the zone's k-th particle (the zone's particles are its columns'
+ * runs of the binning list, in column order) */
+template <class Z>
+ZHD int col_of(const Z& z, int k) {
+    int lo = 0, hi = z.zw - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (z.colL[mid] <= k) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+template <class Z>
+ZHD int slot_in(const Z& z, int k, int col) {
+    return z.glist[z.colG[col] + (k - z.colL[col])];
+}
+template <class Z>
+ZHD int slot_of(const Z& z, int k) {
+    return slot_in(z, k, col_of(z, k));
+}
+/* ZF_OWN / ZF_COVER of the input cell of k: the last cell of k's column
+ * whose input run starts at or before k */
+template <class Z>
+ZHD unsigned short own_bits(const Z& z, int k, int col) {
+    if (z.allOwned) return ZF_OWN;
+    int lo = col * z.zh, hi = lo + z.zh - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if ((int)z.k0[mid] <= k) lo = mid;
+        else hi = mid - 1;
+    }
+    return (unsigned short)(z.fl[lo] & (ZF_OWN | ZF_COVER));
+}
+
+/* successor of k in cell lc's list (implicit run or links) */
+template <class Z>
+ZHD int next_k(const Z& z, int lc, int k) {
+    return fget(z, lc, ZF_MAT) ? nx_get(z, k) : (k < tail_of(z, lc) ? k + 1 : -1);
+}
+
+/* write the links of a still-implicit list before its first append */
+template <class Z>
+ZHD void ensure_mat(Z& z, int lc) {
+    if (fget(z, lc, ZF_MAT)) return;
+    const int t = tail_of(z, lc);
+    for (int k = head_of(z, lc); k >= 0 && k <= t; ++k) nx_set(z, k, (k < t) ? k + 1 : -1);
+    fset(z, lc, ZF_MAT);
+}
+
+/* requires a materialised list */
+template <class Z>
+ZHD void list_append(Z& z, int lc, int k) {
+    typedef typename Z::idx_t IT;
+    IT* const H = z.head;
+    IT* const T = z.tail;
+    fset(z, lc, ZF_CHG);
+    nx_set(z, k, -1);
+    const int t = iget(T[lc]);
+    if (t < 0) H[lc] = (IT)k;
+    else nx_set(z, t, k);
+    T[lc] = (IT)k;
+}
+
+/* zone_pass.cpp:280-294: new position uniform in the target cell, kept
+ * only where this zone owns the particle */
+template <class Z>
+ZHD void move_to(Z& z, int lc, int slot, unsigned short own, double ux, double uy) {
+    if (!(own & ZF_OWN)) return;
+    const int ix = lc / z.zh;
+    const double nx = (double)(z.xs + ix) * z.g.dxc + z.g.dxc * ux;
+    const double ny = (double)(z.ys + (lc - ix * z.zh)) * z.g.dyc + z.g.dyc * uy;
+    z.x[slot] = nx;
+    z.y[slot] = ny;
+}
+
+/* the bulk/skip masks of a transfer (zone_pass.cpp:492-525) */
+template <class Z>
+ZHD void transfer_masks(Z& z, int src, const int* rv, int nr) {
+    bool allBulk = fget(z, src, ZF_BULK0);
+    for (int k = 0; k < nr && allBulk; ++k)
+        if (!fget(z, rv[k], ZF_BULK0)) allBulk = false;
+    if (allBulk) {
+        fset(z, src, ZF_BB);
+        for (int k = 0; k < nr; ++k) fset(z, rv[k], ZF_BB);
+    } else {
+        fset(z, src, ZF_SKIP);
+        fclr(z, src, ZF_BB);
+        for (int k = 0; k < nr; ++k) {
+            fset(z, rv[k], ZF_SKIP);
+            fclr(z, rv[k], ZF_BB);
+        }
+    }
+}
+
+ZHD bool cand_less(const Cand& a, const Cand& b) {
+    /* (-deficit, -inward (all equal: weight 0), cnt, cheb, dist2, tie) */
+    if (a.deficitNeg != b.deficitNeg) return a.deficitNeg < b.deficitNeg;
+    if (a.cnt != b.cnt) return a.cnt < b.cnt;
+    if (a.cheb != b.cheb) return a.cheb < b.cheb;
+    if (a.dist2 != b.dist2) return a.dist2 < b.dist2;
+    return a.tie < b.tie;
+}
+
+enum { ZC_DENSE = 0, ZC_LOW = 1 };
+
+/* Order-preserving compaction into z.dl of the zone cells above their high
+ * threshold (ZC_DENSE) or occupied and below their low threshold (ZC_LOW);
+ * all lanes; returns the count (same on every lane). */
+template <class Z>
+ZHD int zone_compact(Z& z, int lane, int nl, int mode) {
+    int base = 0;
+    for (int b0 = 0; b0 < z.C; b0 += nl) {
+        const int lc = b0 + lane;
+        bool f = false;
+        if (lc < z.C) {
+            const double c = (double)z.cnt[lc];
+            f = (mode == ZC_DENSE) ? c > high_thr(z, lc) : (z.cnt[lc] > 0 && c < low_thr(z, lc));
+        }
+        const unsigned long long msk = wave_ballot(f);
+        if (f) z.dl[base + wave_popc(msk & lanes_below(lane))] = (typename Z::idx_t)lc;
+        base += wave_popc(msk);
+    }
+    return base;
+}
+
+/* The shuffles' partner draws, j in [0, i]. */
+enum { ZS_DENSE = 0, ZS_LOW = 1, ZS_DONOR = 2, ZS_PARTICLE = 3 };
+struct ShufDraw {
+    uint64_t seed;
+    int kind;
+    uint32_t a, b;
+    ZHD uint32_t operator()(uint32_t i) const {
+        switch (kind) {
+            case ZS_DENSE: return mpcd_draw_dense_shuffle(seed, (int)a, i);
+            case ZS_LOW: return mpcd_draw_low_shuffle(seed, i);
+            case ZS_DONOR: return mpcd_draw_donor_shuffle(seed, a, i);
+            default: return mpcd_draw_particle_shuffle(seed, a, b, i);
+        }
+    }
+};
+
+/* Fisher-Yates of a[0..m), i = m-1 down to 1, swapping a[i] with a[j(i)],
+ * `draw(i)` giving j in [0, i].  The partner draws of a chunk of nl indices
+ * are computed by the lanes at once and kept in registers; lane 0 applies
+ * the chunk's swaps in order, reading each partner from its lane. */
+template <class T>
+ZHD void zone_shuffle(T* a, int m, int lane, int nl, ShufDraw draw) {
+    for (int c0 = ((m - 1) / nl) * nl; c0 >= 0 && m > 1; c0 -= nl) {
+        const int i = c0 + lane;
+        const int j = (i >= 1 && i < m) ? (int)draw((uint32_t)i) : 0;
+        const int hi = imin(m - 1, c0 + nl - 1);
+        for (int ii = hi; ii >= imax(1, c0); --ii) {
+            const int jj = wave_readlane(j, ii - c0);
+            if (lane == 0) {
+                const T t = a[ii];
+                a[ii] = a[jj];
+                a[jj] = t;
+            }
+        }
+    }
+    ZBAR();
+}
+
+/* ---- dense phase ---- */
+
+/* Admissible targets of dense cell src within Chebyshev range 2 inside the
+ * zone, ranked into the round-robin order of rank_dense_targets_unified
+ * (zone_pass.cpp:413-490): 8 angular bins from a random start bin, each bin
+ * sorted by the candidate key, then one per bin per round.  All lanes; the
+ * order lands in z.ord[0..nOrd) with each target's threshold in z.ordThr.
+ * Every draw is taken in one round: the tie breaks (t = 0..23, one per
+ * neighbour offset), the start bin (t = 24) and, when nStage > 0, the first
+ * nStage particles of src's list, the only ones its spreads can move
+ * (t = 25..): their slot, ownership and new-position draw. */
+template <class Z>
+ZHD void dense_targets(Z& z, int src, int high, int pass, int phase, int lane, int nl, int nStage) {
+    const int sgx = lc_gx(z, src), sgy = lc_gy(z, src), sgc = lc_gc(z, src);
+    const bool srcMat = fget(z, src, ZF_MAT);
+    if (nStage > 0 && srcMat) {
+        if (lane == 0) {
+            int k = head_of(z, src);
+            for (int q = 0; q < nStage && k >= 0; ++q) {
+                z.mvk[q] = k;
+                k = nx_get(z, k);
+            }
+        }
+        ZBAR();
+    }
+    const int h0 = head_of(z, src);
+    for (int t = lane; t < 25 + nStage; t += nl) {
+        if (t < 24) {
+            /* neighbour offsets in (dy outer, dx inner) order, centre skipped */
+            const int f = t < 12 ? t : t + 1;
+            const int dy = f / 5 - 2, dx = f % 5 - 2;
+            int jx, jy, gc = -1;
+            if (shift_idx(z, sgx, sgy, dx, dy, jx, jy)) {
+                const int lc = zone_lc(z, jx, jy);
+                if (lc >= 0 && (double)z.cnt[lc] < thr(z, lc, high)) gc = jy + z.g.Ny * jx;
+            }
+            z.nbGc[t] = gc;
+            if (gc >= 0) z.nbTie[t] = mpcd_draw_tie(z.seed, pass, phase, (uint32_t)sgc, (uint32_t)gc);
+        } else if (t == 24) {
+            z.sh->startBin = mpcd_draw_start_bin(z.seed, pass, phase, (uint32_t)sgc);
+        } else {
+            const int q = t - 25;
+            const int k = srcMat ? z.mvk[q] : h0 + q;
+            const int col = col_of(z, k);
+            const int slot = slot_in(z, k, col);
+            z.mvs[q] = slot;
+            z.mvo[q] = own_bits(z, k, col);
+            mpcd_draw_dense_pos(z.seed, pass, (uint32_t)z.gpid[slot], &z.mvx[q], &z.mvy[q]);
+        }
+    }
+    ZBAR();
+    int base = 0;
+    for (int b0 = 0; b0 < 24; b0 += nl) {
+        const int o = b0 + lane;
+        bool keep = false;
+        int gc = -1;
+        if (o < 24) {
+            gc = z.nbGc[o];
+            keep = gc >= 0;
+            if (z.zp.dedupe)
+                for (int q = 0; q < o && keep; ++q)
+                    if (z.nbGc[q] == gc) keep = false;
+        }
+        const unsigned long long msk = wave_ballot(keep);
+        if (keep) {
+            Cand& c = z.cand[base + wave_popc(msk & lanes_below(lane))];
+            const int jx = gc / z.g.Ny, jy = gc - jx * z.g.Ny;
+            const int lc = zone_lc(z, jx, jy);
+            int ddx, ddy;
+            min_step(z, sgx, sgy, jx, jy, ddx, ddy);
+            c.lc = lc;
+            c.gc = gc;
+            c.bin = zone_bin(z.zp, (ddy + 2) * 5 + (ddx + 2));
+            c.deficitNeg = -dmax(0.0, thr(z, lc, high) - (double)z.cnt[lc]);
+            c.cnt = z.cnt[lc];
+            c.cheb = imax(ddx < 0 ? -ddx : ddx, ddy < 0 ? -ddy : ddy);
+            c.dist2 = ddx * ddx + ddy * ddy;
+            c.tie = z.nbTie[o];
+        }
+        base += wave_popc(msk);
+    }
+    const int m = base;
+    if (lane == 0) z.sh->nOrd = m;
+    ZBAR();
+    if (m == 0) return;
+    /* rank inside the bin: stable order of the candidate key */
+    for (int i = lane; i < m; i += nl) {
+        const Cand ci = z.cand[i];
+        int r = 0;
+        for (int j = 0; j < m; ++j) {
+            if (j == i || z.cand[j].bin != ci.bin) continue;
+            if (cand_less(z.cand[j], ci) || (j < i && !cand_less(ci, z.cand[j]))) ++r;
+        }
+        z.rnk[i] = r;
+    }
+    ZBAR();
+    /* round robin position: earlier rounds, then earlier bins from startBin */
+    const int sb = z.sh->startBin;
+    for (int i = lane; i < m; i += nl) {
+        const int ri = z.rnk[i];
+        const int si = (z.cand[i].bin - sb + 8) & 7;
+        int pos = 0;
+        for (int j = 0; j < m; ++j) {
+            const int rj = z.rnk[j];
+            if (rj < ri || (rj == ri && ((z.cand[j].bin - sb + 8) & 7) < si)) ++pos;
+        }
+        const int lc = z.cand[i].lc;
+        z.ord[pos] = lc;
+        z.ordThr[pos] = thr(z, lc, high);
+    }
+    ZBAR();
+}
+
+/* spread_particles_from_dense_source_round_robin (zone_pass.cpp:527-589),
+ * lane 0.  `used` counts the movers taken so far; the first nStaged were
+ * staged by dense_targets.  The number of candidates at or after position kk
+ * that still have a deficit is kept incrementally: within a sweep only the
+ * candidate being served changes, and each target appears once. */
+template <class Z>
+ZHD int spread(Z& z, int src, int high, int needOut, int pass, int nStaged, int* rv, int& nr, int& used) {
+    int moved = 0;
+    const int nOrd = z.sh->nOrd;
+    if (needOut <= 0 || nOrd == 0) return 0;
+    int avail = cnt_of(z, src);
+    if (avail <= 0) return 0;
+    while (needOut > 0 && avail > 0) {
+        bool progressed = false;
+        int R = 0;
+        for (int jj = 0; jj < nOrd; ++jj)
+            if (imax(0, (int)MPCD_FLOOR(z.ordThr[jj] - (double)cnt_of(z, z.ord[jj]))) > 0) ++R;
+        for (int kk = 0; kk < nOrd; ++kk) {
+            const int nb = z.ord[kk];
+            const int deficit = imax(0, (int)MPCD_FLOOR(z.ordThr[kk] - (double)cnt_of(z, nb)));
+            if (deficit <= 0) continue;
+            const int remaining = R;
+            --R;
+            const int quota = imax(1, (int)MPCD_CEIL((double)needOut / (double)imax(1, remaining)));
+            const int nt = imax(1, imin(imin(quota, deficit), imin(needOut, avail)));
+            /* pops from the head leave an implicit run implicit */
+            ensure_mat(z, nb);
+            fset(z, src, ZF_CHG);
+            for (int q = 0; q < nt; ++q) {
+                const int k = head_of(z, src);
+                const int nk = next_k(z, src, k);
+                head_set(z, src, nk);
+                if (nk < 0) tail_set(z, src, -1);
+                int slot;
+                unsigned short own;
+                double ux, uy;
+                if (used < nStaged) {
+                    slot = z.mvs[used];
+                    own = z.mvo[used];
+                    ux = z.mvx[used];
+                    uy = z.mvy[used];
+                } else {
+                    const int col = col_of(z, k);
+                    slot = slot_in(z, k, col);
+                    own = own_bits(z, k, col);
+                    mpcd_draw_dense_pos(z.seed, pass, (uint32_t)z.gpid[slot], &ux, &uy);
+                }
+                move_to(z, nb, slot, own, ux, uy);
+                ++used;
+                list_append(z, nb, k);
+            }
+            avail -= nt;
+            cnt_set(z, nb, cnt_of(z, nb) + nt);
+            bool seen = false;
+            for (int k = 0; k < nr; ++k)
+                if (rv[k] == nb) seen = true;
+            if (!seen) rv[nr++] = nb;
+            moved += nt;
+            needOut -= nt;
+            progressed = true;
+            if (needOut <= 0 || avail <= 0) break;
+        }
+        if (!progressed) break;
+    }
+    if (moved > 0) cnt_set(z, src, avail);
+    return moved;
+}
+
+/* Dense passes (zone_pass.cpp:708-781), all lanes. */
+template <class Z>
+ZHD void zone_dense_passes(Z& z, int lane, int nl) {
+    for (int pass = 1; pass <= z.zp.maxPasses; ++pass) {
+        for (int lc = lane; lc < z.C; lc += nl) zone_maps(z, lc, 0);
+        ZBAR();
+        const int m = zone_compact(z, lane, nl, ZC_DENSE);
+        ZBAR();
+        zone_shuffle(z.dl, m, lane, nl, ShufDraw{z.seed, ZS_DENSE, (uint32_t)pass, 0u});
+        for (int k = 0; k < m; ++k) {
+            const int c = (int)z.dl[k];
+            const int nc = cnt_of(z, c);
+            if (nc <= 0) continue;
+            const double hl = high_thr(z, c);
+            if ((double)nc <= hl) continue;
+            const int needOut = imax(0, (int)MPCD_CEIL((double)nc - hl));
+            int moved = 0;
+            if (lane == 0) {
+                ++z.sh->nHigh;
+                z.sh->nr = 0;
+            }
+            if (needOut > 0) {
+                /* the spreads move at most needOut particles, all from the
+                 * head of the list: they are staged with the first target
+                 * ranking's draws */
+                const int nStage = imin(imin(needOut, nc), ZONE_MAX_MOVE);
+                dense_targets(z, c, 0, pass, 0, lane, nl, nStage);
+                if (lane == 0) {
+                    int used = 0, nr = 0;
+                    moved += spread(z, c, 0, needOut, pass, nStage, z.recvT, nr, used);
+                    z.sh->stage = used;
+                    z.sh->nr = nr;
+                }
+                ZBAR();
+                const int remaining = imax(0, (int)MPCD_CEIL((double)cnt_of(z, c) - hl));
+                if (remaining > 0) {
+                    dense_targets(z, c, 1, pass, 1, lane, nl, 0);
+                    if (lane == 0) {
+                        int used = z.sh->stage, nr = z.sh->nr;
+                        moved += spread(z, c, 1, remaining, pass, nStage, z.recvT, nr, used);
+                        z.sh->nr = nr;
+                    }
+                }
+            }
+            if (lane == 0) {
+                const int nr = z.sh->nr;
+                for (int r = 0; r < nr; ++r) fset(z, z.recvT[r], ZF_RECV);
+                transfer_masks(z, c, z.recvT, nr);
+                z.sh->movedDense += moved;
+            }
+            ZBAR();
+        }
+    }
+}
+
+/* ---- sparse phase ---- */
+
+/* inward_direction_from_local_gradient (zone_pass.cpp:375-411) on the
+ * sparse-phase count copy; the four smoothed counts in parallel. */
+template <class Z>
+ZHD void inward_dir(Z& z, int gx, int gy, int lane, int nl, double& dxo, double& dyo) {
+    const int Nx = z.g.Nx, Ny = z.g.Ny;
+    /* x pair (xa - xb) * fx, y pair (ya - yb) * fy */
+    int xa, xb, ya, yb;
+    double fx, fy;
+    if (z.g.perx) { xa = wrapi(gx + 1, Nx); xb = wrapi(gx - 1, Nx); fx = 0.5; }
+    else if (gx == 0) { xa = gx + 1; xb = gx; fx = 1.0; }
+    else if (gx == Nx - 1) { xa = gx; xb = gx - 1; fx = 1.0; }
+    else { xa = gx + 1; xb = gx - 1; fx = 0.5; }
+    if (z.g.pery) { ya = wrapi(gy + 1, Ny); yb = wrapi(gy - 1, Ny); fy = 0.5; }
+    else if (gy == 0) { ya = gy + 1; yb = gy; fy = 1.0; }
+    else if (gy == Ny - 1) { ya = gy; yb = gy - 1; fy = 1.0; }
+    else { ya = gy + 1; yb = gy - 1; fy = 0.5; }
+    for (int t = lane; t < 4; t += nl) {
+        const int qx = (t == 0) ? xa : (t == 1) ? xb : gx;
+        const int qy = (t == 2) ? ya : (t == 3) ? yb : gy;
+        z.nsm4[t] = nsm_at(z, qx, qy, 1);
+    }
+    ZBAR();
+    dxo = (fx == 0.5) ? 0.5 * (z.nsm4[0] - z.nsm4[1]) : (z.nsm4[0] - z.nsm4[1]);
+    dyo = (fy == 0.5) ? 0.5 * (z.nsm4[2] - z.nsm4[3]) : (z.nsm4[2] - z.nsm4[3]);
+}
+
+/* One donor transfer of the sparse phase (zone_pass.cpp:818-850): copy
+ * donor d's list into tmp (LDS or the zone's global region), Fisher-Yates
+ * it, move the first nt to low cell c; all lanes. */
+template <class Z, class T>
+ZHD void donor_transfer(Z& z, int c, int d, int md, int nt, int cgc, T* tmp, int lane, int nl) {
+    typedef typename Z::idx_t IT;
+    /* copy the donor list, Fisher-Yates it, move the first nt */
+    if (!fget(z, d, ZF_MAT)) {
+        const int h = head_of(z, d);
+        for (int q = lane; q < md; q += nl) tmp[q] = (IT)(h + q);
+    } else if (lane == 0) {
+        int kk = head_of(z, d);
+        for (int q = 0; q < md; ++q) {
+            tmp[q] = (IT)kk;
+            kk = nx_get(z, kk);
+        }
+    }
+    ZBAR();
+    const int dgc = lc_gc(z, d);
+    zone_shuffle(tmp, md, lane, nl,
+                 ShufDraw{z.seed, ZS_PARTICLE, (uint32_t)cgc, (uint32_t)dgc});
+    /* the movers: new positions, links of the moved run and of the
+     * donor's remaining run, all lanes */
+    for (int q = lane; q < md; q += nl) {
+        const int kq = (int)tmp[q];
+        if (q < nt) {
+            const int col = col_of(z, kq);
+            const int slot = slot_in(z, kq, col);
+            double ux, uy;
+            mpcd_draw_sparse_pos(z.seed, (uint32_t)z.gpid[slot], &ux, &uy);
+            move_to(z, c, slot, own_bits(z, kq, col), ux, uy);
+            nx_set(z, kq, q + 1 < nt ? (int)tmp[q + 1] : -1);
+        } else {
+            nx_set(z, kq, q + 1 < md ? (int)tmp[q + 1] : -1);
+        }
+    }
+    if (lane == 0) {
+        ensure_mat(z, c);
+        const int tc = tail_of(z, c);
+        if (tc < 0) head_set(z, c, (int)tmp[0]);
+        else nx_set(z, tc, (int)tmp[0]);
+        tail_set(z, c, (int)tmp[nt - 1]);
+        fset(z, c, ZF_CHG);
+        fset(z, d, ZF_MAT | ZF_CHG);
+        head_set(z, d, md > nt ? (int)tmp[nt] : -1);
+        tail_set(z, d, md > nt ? (int)tmp[md - 1] : -1);
+        cnt_set(z, d, md - nt);
+        cnt_set(z, c, cnt_of(z, c) + nt);
+        fset(z, c, ZF_RECV);
+        const int rv1[1] = {c};
+        transfer_masks(z, d, rv1, 1);
+        z.sh->movedSparse += nt;
+    }
+    ZBAR();
+}
+
+/* Sparse pass (zone_pass.cpp:783-854), all lanes. */
+template <class Z>
+ZHD void zone_sparse_pass(Z& z, int lane, int nl) {
+    typedef typename Z::idx_t IT;
+    for (int lc = lane; lc < z.C; lc += nl) {
+        zone_maps(z, lc, 0);
+        z.cntS[lc] = z.cnt[lc];
+    }
+    ZBAR();
+    const int m = zone_compact(z, lane, nl, ZC_LOW);
+    ZBAR();
+    zone_shuffle(z.dl, m, lane, nl, ShufDraw{z.seed, ZS_LOW, 0u, 0u});
+    const int lowInt = (int)MPCD_CEIL(z.zp.lowThrBulk);
+    for (int k = 0; k < m; ++k) {
+        const int c = (int)z.dl[k];
+        const int nc = cnt_of(z, c);
+        if (nc <= 0) continue;
+        if (lane == 0) ++z.sh->nLow;
+        const int cgx = lc_gx(z, c), cgy = lc_gy(z, c), cgc = lc_gc(z, c);
+        /* zone neighbours at range 1 (local_neighbors_in_zone :263-278) */
+        for (int o = lane; o < 8; o += nl) {
+            const int f = o < 4 ? o : o + 1;
+            const int dy = f / 3 - 1, dx = f % 3 - 1;
+            int jx, jy, q = -1;
+            if (shift_idx(z, cgx, cgy, dx, dy, jx, jy)) q = zone_lc(z, jx, jy);
+            z.nbGc[o] = q;
+        }
+        ZBAR();
+        int nn = 0, nd = 0;
+        for (int b0 = 0; b0 < 8; b0 += nl) {
+            const int o = b0 + lane;
+            bool keep = false;
+            if (o < 8) {
+                const int q = z.nbGc[o];
+                keep = q >= 0;
+                if (z.zp.dedupe)
+                    for (int t = 0; t < o && keep; ++t)
+                        if (z.nbGc[t] == q) keep = false;
+            }
+            const unsigned long long mk = wave_ballot(keep);
+            nn += wave_popc(mk);
+            const bool don = keep && cnt_of(z, z.nbGc[o]) > lowInt;
+            const unsigned long long mdn = wave_ballot(don);
+            if (don) z.dord[nd + wave_popc(mdn & lanes_below(lane))] = z.nbGc[o];
+            nd += wave_popc(mdn);
+        }
+        ZBAR();
+        if (nn == 0) continue;
+        int need = imax(0, lowInt - nc);
+        if (need <= 0 || nd == 0) continue;
+        double gxd, gyd;
+        inward_dir(z, cgx, cgy, lane, nl, gxd, gyd);
+        /* rank_admissible_neighbors_inward (zone_pass.cpp:636-667) */
+        zone_shuffle(z.dord, nd, lane, nl,
+                     ShufDraw{z.seed, ZS_DONOR, (uint32_t)cgc, 0u});
+        const double ndir = mpcd_hypot(gxd, gyd);
+        for (int t = lane; t < nd; t += nl) {
+            const int q = z.dord[t];
+            int ddx, ddy;
+            min_step(z, cgx, cgy, lc_gx(z, q), lc_gy(z, q), ddx, ddy);
+            const double nstep = mpcd_hypot((double)ddx, (double)ddy);
+            double align = 0.0;
+            if (ndir >= 1e-14 && nstep >= 1e-14) align = ((double)ddx * gxd + (double)ddy * gyd) / (ndir * nstep);
+            const double noise = 1e-9 * mpcd_draw_donor_noise(z.seed, (uint32_t)cgc, (uint32_t)lc_gc(z, q));
+            z.dsc[t] = 100.0 * align + nsm_at(z, lc_gx(z, q), lc_gy(z, q), 1) + noise;
+        }
+        ZBAR();
+        for (int t = lane; t < nd; t += nl) { /* stable, descending score */
+            int r = 0;
+            for (int u = 0; u < nd; ++u)
+                if (z.dsc[u] > z.dsc[t] || (u < t && !(z.dsc[t] > z.dsc[u]))) ++r;
+            z.dord2[r] = z.dord[t];
+        }
+        ZBAR();
+        for (int t = 0; t < nd; ++t) {
+            const int d = z.dord2[t];
+            const int md = cnt_of(z, d);
+            if (md <= 0) continue;
+            const int avail = md - lowInt;
+            if (avail <= 0) continue;
+            const int nt = imin(imin(need, avail), md);
+            if (nt <= 0) continue;
+            if (md <= ZONE_TMP_L) donor_transfer(z, c, d, md, nt, cgc, z.tmpL, lane, nl);
+            else donor_transfer(z, c, d, md, nt, cgc, z.tmpG, lane, nl);
+            need -= nt;
+            if (need <= 0) break;
+        }
+    }
+}
+
+/* ---- set-up, correction, statistics ---- */
+
+/* Zone setup (zone_pass.cpp:680-685), all lanes: counts, implicit list of
+ * each cell (its run of local indices, from a wave prefix sum over the
+ * zone's cells), the column map and the ownership flags. */
+template <class Z>
+ZHD void zone_setup(Z& z, const ZoneIn& in, int lane, int nl) {
+    int base = 0;
+    bool mixed = false;
+    for (int b0 = 0; b0 < z.C; b0 += nl) {
+        const int lc = b0 + lane;
+        const int gc = lc < z.C ? lc_gc(z, lc) : 0;
+        const int m = lc < z.C ? z.gcnt[gc] : 0;
+        const int incl = wave_shfl_up_add(m, lane);
+        if (lc < z.C) {
+            const int k0 = base + incl - m;
+            cnt_set(z, lc, m);
+            z.k0[lc] = (typename Z::idx_t)k0;
+            head_set(z, lc, m > 0 ? k0 : -1);
+            tail_set(z, lc, m > 0 ? k0 + m - 1 : -1);
+            unsigned short f = 0;
+            if (!in.owner || in.owner[gc] == z.zid) f |= ZF_OWN;
+            if (in.gcover && in.gcover[gc]) f |= ZF_COVER;
+            if (f != ZF_OWN) mixed = true;
+            z.fl[lc] = f;
+            if (lc % z.zh == 0) {
+                z.colL[lc / z.zh] = k0;
+                z.colG[lc / z.zh] = z.gstart[gc];
+            }
+        }
+        /* lanes past the zone add 0, so the last lane holds the chunk total */
+        base += wave_bcast(incl, nl - 1);
+    }
+    const bool anyMixed = wave_ballot(mixed) != 0ull;
+    if (lane == 0) {
+        z.sh->P = base;
+        z.sh->allOwned = anyMixed ? 0 : 1;
+        z.sh->nHigh = 0;
+        z.sh->nLow = 0;
+        z.sh->movedDense = 0;
+        z.sh->movedSparse = 0;
+        z.sh->stats4 = 0.0;
+        z.sh->stats5 = 0.0;
+    }
+}
+
+/* sum over the slots glist[s0 .. s0+n) in order, loads issued 8 at a time */
+template <class Z>
+ZHD void run_sum(const Z& z, int s0, int n, double& sx, double& sy) {
+    for (int j0 = 0; j0 < n; j0 += 8) {
+        int sl[8];
+        double ax[8], ay[8];
+        for (int u = 0; u < 8; ++u) sl[u] = (j0 + u < n) ? z.glist[s0 + j0 + u] : -1;
+        for (int u = 0; u < 8; ++u) {
+            ax[u] = sl[u] >= 0 ? z.vx[sl[u]] : 0.0;
+            ay[u] = sl[u] >= 0 ? z.vy[sl[u]] : 0.0;
+        }
+        for (int u = 0; u < 8; ++u)
+            if (sl[u] >= 0) {
+                sx += ax[u];
+                sy += ay[u];
+            }
+    }
+}
+
+/* Momentum correction (zone_pass.cpp:870-909), first half, one lane per
+ * cell: the velocity change of a changed cell from the momentum of its
+ * input run and of its final list (both in list order).  Reads only; a cell
+ * nothing changed has the same list before and after, so its change is
+ * exactly zero. */
+template <class Z>
+ZHD void zone_correct_sums(Z& z, int lc) {
+    const int m = cnt_of(z, lc);
+    if (!fget(z, lc, ZF_CHG) || m == 0) return;
+    const int gc = lc_gc(z, lc);
+    double pbx = 0.0, pby = 0.0;
+    run_sum(z, z.gstart[gc], z.gcnt[gc], pbx, pby);
+    double px = 0.0, py = 0.0;
+    int k = head_of(z, lc);
+    while (k >= 0) {
+        int sl[8];
+        double ax[8], ay[8];
+        for (int u = 0; u < 8; ++u) {
+            sl[u] = -1;
+            if (k >= 0) {
+                sl[u] = slot_of(z, k);
+                k = next_k(z, lc, k);
+            }
+        }
+        for (int u = 0; u < 8; ++u) {
+            ax[u] = sl[u] >= 0 ? z.vx[sl[u]] : 0.0;
+            ay[u] = sl[u] >= 0 ? z.vy[sl[u]] : 0.0;
+        }
+        for (int u = 0; u < 8; ++u)
+            if (sl[u] >= 0) {
+                px += ax[u];
+                py += ay[u];
+            }
+    }
+    const double dPx = pbx - px, dPy = pby - py;
+    const double dux = dPx / (double)m, duy = dPy / (double)m;
+    z.duG[2 * lc] = dux;
+    z.duG[2 * lc + 1] = duy;
+    bool corr = z.zp.corrEnabled != 0;
+    if (corr && fget(z, lc, ZF_SKIP)) corr = false;
+    if (corr && fget(z, lc, ZF_RECV) && !fget(z, lc, ZF_BB)) corr = false;
+    if (corr && !(dux != 0.0 || duy != 0.0)) corr = false;
+    if (corr) fset(z, lc, ZF_CORR);
+}
+
+/* Second half, one lane per cell: write the velocities of the owned
+ * particles of a changed cell; for an unchanged cell that several zones
+ * cover, pass its owned particles' input velocities through to the side
+ * buffer.  Returns the cell's |du|^2 (0 for an unchanged cell). */
+template <class Z>
+ZHD double zone_correct_write(Z& z, int lc) {
+    if (fget(z, lc, ZF_CHG)) {
+        if (cnt_of(z, lc) == 0) return 0.0;
+        const bool corr = fget(z, lc, ZF_CORR);
+        const double dux = z.duG[2 * lc], duy = z.duG[2 * lc + 1];
+        int k = head_of(z, lc);
+        while (k >= 0) {
+            int sl[8];
+            unsigned short ow[8];
+            double ax[8], ay[8];
+            for (int u = 0; u < 8; ++u) {
+                sl[u] = -1;
+                ow[u] = 0;
+                if (k >= 0) {
+                    const int col = col_of(z, k);
+                    sl[u] = slot_in(z, k, col);
+                    ow[u] = own_bits(z, k, col);
+                    k = next_k(z, lc, k);
+                }
+            }
+            for (int u = 0; u < 8; ++u) {
+                const bool w = sl[u] >= 0 && (ow[u] & ZF_OWN);
+                ax[u] = w ? z.vx[sl[u]] : 0.0;
+                ay[u] = w ? z.vy[sl[u]] : 0.0;
+            }
+            for (int u = 0; u < 8; ++u) {
+                if (sl[u] < 0 || !(ow[u] & ZF_OWN)) continue;
+                double vx = ax[u], vy = ay[u];
+                if (corr) {
+                    vx += dux;
+                    vy += duy;
+                }
+                if (ow[u] & ZF_COVER) {
+                    z.vsx[sl[u]] = vx;
+                    z.vsy[sl[u]] = vy;
+                } else {
+                    z.vx[sl[u]] = vx;
+                    z.vy[sl[u]] = vy;
+                }
+            }
+        }
+        return dux * dux + duy * duy;
+    }
+    if (!fget(z, lc, ZF_COVER) || !fget(z, lc, ZF_OWN)) return 0.0;
+    const int gc = lc_gc(z, lc);
+    const int s0 = z.gstart[gc], m0 = z.gcnt[gc];
+    for (int j0 = 0; j0 < m0; j0 += 8) {
+        int sl[8];
+        double ax[8], ay[8];
+        for (int u = 0; u < 8; ++u) sl[u] = (j0 + u < m0) ? z.glist[s0 + j0 + u] : -1;
+        for (int u = 0; u < 8; ++u) {
+            ax[u] = sl[u] >= 0 ? z.vx[sl[u]] : 0.0;
+            ay[u] = sl[u] >= 0 ? z.vy[sl[u]] : 0.0;
+        }
+        for (int u = 0; u < 8; ++u)
+            if (sl[u] >= 0) {
+                z.vsx[sl[u]] = ax[u];
+                z.vsy[sl[u]] = ay[u];
+            }
+    }
+    return 0.0;
+}
+
+/* Correction of all cells and stats[4], stats[5] of the zone
+ * (zone_pass.cpp:870-922), all lanes.  The rms sums |du|^2 over the cells in
+ * ascending order: each chunk of nl cells is added lane by lane. */
+template <class Z>
+ZHD void zone_correct(Z& z, int lane, int nl) {
+    for (int lc = lane; lc < z.C; lc += nl) zone_correct_sums(z, lc);
+    ZBAR();
+    double acc = 0.0, mx = 0.0;
+    int nDU = 0;
+    for (int b0 = 0; b0 < z.C; b0 += nl) {
+        const int lc = b0 + lane;
+        const double d = lc < z.C ? zone_correct_write(z, lc) : 0.0;
+        mx = dmax(mx, MPCD_SQRT(d));
+        unsigned long long pos = wave_ballot(d > 0.0);
+        nDU += wave_popc(pos);
+        while (pos) {
+            const int l = __builtin_ctzll(pos);
+            pos &= pos - 1ull;
+            acc += wave_readlane_d(d, l);
+        }
+    }
+    mx = wave_max(mx);
+    if (lane == 0 && nDU > 0) {
+        z.sh->stats4 = MPCD_SQRT(acc / (double)nDU);
+        z.sh->stats5 = mx;
+    }
+}
+
+/* ---- working set ---- */
+
+/* bytes of the per-zone cell arrays and scratch (links excluded) */
+template <class IT>
+ZHD size_t zone_fixed_bytes(int C, int W, int useLocal) {
+    size_t b = 0;
+    if (useLocal) b += (size_t)C * 2 * sizeof(double);         /* lowA highA */
+    b += 24 * sizeof(Cand) + 2 * 24 * sizeof(double) + 8 * sizeof(double) + 4 * sizeof(double) +
+         (size_t)ZONE_MAX_MOVE * (2 * sizeof(double) + 2 * sizeof(int)); /* 8-byte items first */
+    b += (size_t)W * 2 * sizeof(int);                           /* colL colG */
+    b += (5 * 24 + 3 * 8) * sizeof(int);                        /* ord nbGc rnk recvT (+pad) dord dord2 (+pad) */
+    b += (size_t)C * 6 * sizeof(IT);                            /* cnt head tail dl cntS k0 */
+    b += (size_t)C * sizeof(unsigned short) + ZONE_MAX_MOVE * sizeof(unsigned short); /* fl mvo */
+    b += (size_t)ZONE_TMP_L * sizeof(IT);                       /* tmpL */
+    return (b + 15) & ~(size_t)15;
+}
+
+/* Carves the working set out of `base` (16-byte aligned); links of k < nL
+ * follow the fixed part, nL = what the remaining `bytes` hold. */
+template <class IT>
+ZHD void zone_carve(ZoneCtxT<IT>& z, unsigned char* base, size_t bytes) {
+    const int C = z.C, W = z.zw;
+    unsigned char* p = base;
+    if (z.zp.useLocal) {
+        z.lowA = (double*)p; p += (size_t)C * sizeof(double);
+        z.highA = (double*)p; p += (size_t)C * sizeof(double);
+    } else {
+        z.lowA = nullptr;
+        z.highA = nullptr;
+    }
+    z.cand = (Cand*)p; p += 24 * sizeof(Cand);
+    z.ordThr = (double*)p; p += 24 * sizeof(double);
+    z.nbTie = (double*)p; p += 24 * sizeof(double);
+    z.dsc = (double*)p; p += 8 * sizeof(double);
+    z.nsm4 = (double*)p; p += 4 * sizeof(double);
+    z.mvx = (double*)p; p += ZONE_MAX_MOVE * sizeof(double);
+    z.mvy = (double*)p; p += ZONE_MAX_MOVE * sizeof(double);
+    z.mvk = (int*)p; p += ZONE_MAX_MOVE * sizeof(int);
+    z.mvs = (int*)p; p += ZONE_MAX_MOVE * sizeof(int);
+    z.colL = (int*)p; p += (size_t)W * sizeof(int);
+    z.colG = (int*)p; p += (size_t)W * sizeof(int);
+    z.ord = (int*)p; p += 24 * sizeof(int);
+    z.nbGc = (int*)p; p += 24 * sizeof(int);
+    z.rnk = (int*)p; p += 24 * sizeof(int);
+    z.recvT = (int*)p; p += 48 * sizeof(int);
+    z.dord = (int*)p; p += 8 * sizeof(int);
+    z.dord2 = (int*)p; p += 16 * sizeof(int);
+    z.cnt = (IT*)p; p += (size_t)C * sizeof(IT);
+    z.head = (IT*)p; p += (size_t)C * sizeof(IT);
+    z.tail = (IT*)p; p += (size_t)C * sizeof(IT);
+    z.dl = (IT*)p; p += (size_t)C * sizeof(IT);
+    z.cntS = (IT*)p; p += (size_t)C * sizeof(IT);
+    z.k0 = (IT*)p; p += (size_t)C * sizeof(IT);
+    z.tmpL = (IT*)p; p += (size_t)ZONE_TMP_L * sizeof(IT);
+    z.fl = (unsigned short*)p; p += (size_t)C * sizeof(unsigned short);
+    z.mvo = (unsigned short*)p; p += ZONE_MAX_MOVE * sizeof(unsigned short);
+    const size_t fixed = zone_fixed_bytes<IT>(C, W, z.zp.useLocal);
+    p = base + fixed;
+    z.nxL = (IT*)p;
+    const size_t rest = bytes > fixed ? bytes - fixed : 0;
+    const size_t nl = rest / sizeof(IT);
+    z.nL = nl > (size_t)0x7fffffff ? 0x7fffffff : (int)nl;
+}
+
+/* Per-lane context of zone zid (1-based) with rectangle r. */
+template <class IT>
+ZHD void zone_init(ZoneCtxT<IT>& z, const ZoneIn& in, ZoneRect r, int zid, uint64_t seed, ZoneShared* sh) {
+    z.g = in.g;
+    z.zp = in.zp;
+    z.xs = r.xs;
+    z.xe = r.xe;
+    z.ys = r.ys;
+    z.ye = r.ye;
+    z.zw = r.xe - r.xs;
+    z.zh = r.ye - r.ys;
+    z.C = z.zw * z.zh;
+    z.zid = zid;
+    z.seed = seed;
+    z.gcnt = in.gcnt;
+    z.gstart = in.gstart;
+    z.glist = in.glist;
+    z.gpid = in.gpid;
+    z.x = in.x;
+    z.y = in.y;
+    z.vx = in.vx;
+    z.vy = in.vy;
+    z.vsx = in.vsx;
+    z.vsy = in.vsy;
+    z.sh = sh;
+}
+
+/* The zone after zone_setup and a barrier: both phases, the correction and
+ * the stats. */
+template <class Z>
+ZHD void zone_run(Z& z, int lane, int nl) {
+    z.P = z.sh->P;
+    z.allOwned = z.sh->allOwned;
+    zone_dense_passes(z, lane, nl);
+    zone_sparse_pass(z, lane, nl);
+    ZBAR();
+    zone_correct(z, lane, nl);
+}
+
+}  // namespace mpcdk

@@ -93,11 +93,31 @@ extern "C" int core_collide(const mpcd_params* p, double* va, const int32_t* cid
     return 0;
 }
 
-/* One snapshot zone pass as mpcd_engine.hip runs it: per-zone scratch at
- * exact offsets, owned particles written, velocities in place except for
- * cells several zones cover (side buffer, copied back after the pass). */
+/* One zone of a pass, as k_zone runs it, with working-set index type IT and
+ * `bytes` of working set (links beyond it go to the zone's global region). */
+template <class IT>
+void run_zone(const ZoneIn& in, ZoneRect r, int zid, uint64_t seed, size_t bytes, std::vector<IT>& nxG,
+              std::vector<IT>& tmpG, std::vector<double>& duG, ZoneShared& sh) {
+    ZoneCtxT<IT> z{};
+    zone_init(z, in, r, zid, seed, &sh);
+    z.nxG = nxG.data();
+    z.tmpG = tmpG.data();
+    z.duG = duG.data();
+    const size_t fixed = zone_fixed_bytes<IT>(z.C, z.zw, in.zp.useLocal);
+    std::vector<unsigned char> buf(std::max(bytes, fixed) + 16);
+    unsigned char* base = buf.data() + ((16 - ((uintptr_t)buf.data() & 15)) & 15);
+    zone_carve(z, base, std::max(bytes, fixed));
+    zone_setup(z, in, 0, 1);
+    zone_run(z, 0, 1);
+}
+
+/* One snapshot zone pass as mpcd_engine.hip runs it: per-zone scratch,
+ * owned particles written, velocities in place except for cells several
+ * zones cover (side buffer, copied back after the pass).  wide = 1: int
+ * working set, else uint16_t; ldsBytes: the working set's size (small values
+ * push list links to the global region). */
 extern "C" int core_zone_pass(const mpcd_params* p, double* xa, double* va, int layout, uint64_t seedBase,
-                              int32_t* moved2, double* corr2) {
+                              int32_t* moved2, double* corr2, int wide, long long ldsBytes) {
     const Geom g = make_geom(*p);
     const ZoneParams zp = make_zone_params(*p);
     std::vector<double> x, y, vx, vy;
@@ -121,63 +141,49 @@ extern "C" int core_zone_pass(const mpcd_params* p, double* xa, double* va, int 
             over[(size_t)c] = 1;
             anyOver = true;
         }
-    std::vector<int> off(rects.size() + 1, 0);
-    for (size_t zi = 0; zi < rects.size(); ++zi) {
-        int P = 0;
-        for (int gx = rects[zi].xs; gx < rects[zi].xe; ++gx)
-            P += b.start[(size_t)(rects[zi].ye - 1 + p->Ny * gx) + 1] - b.start[(size_t)(rects[zi].ys + p->Ny * gx)];
-        off[zi + 1] = off[zi] + P;
-    }
-    std::vector<int> next((size_t)off.back() + 1, -999), tmpg((size_t)off.back() + 1, -999);
+    ZoneIn in{};
+    in.g = g;
+    in.zp = zp;
+    in.gcnt = b.cnt.data();
+    in.gstart = b.start.data();
+    in.glist = b.list.data();
+    in.gpid = pid.data();
+    in.owner = owner.data();
+    in.gcover = anyOver ? over.data() : nullptr;
+    in.x = x.data();
+    in.y = y.data();
+    in.vx = vx.data();
+    in.vy = vy.data();
+    in.vsx = vsx.data();
+    in.vsy = vsy.data();
     int md = 0, ms = 0;
     double acc = 0.0, mx = 0.0;
     int nc = 0;
     for (size_t zi = 0; zi < rects.size(); ++zi) {
         const ZoneRect r = rects[zi];
-        ZoneCtx z{};
-        z.g = g;
-        z.zp = zp;
-        z.xs = r.xs;
-        z.xe = r.xe;
-        z.ys = r.ys;
-        z.ye = r.ye;
-        z.zw = r.xe - r.xs;
-        z.zh = r.ye - r.ys;
-        z.C = z.zw * z.zh;
-        z.zid = (int)zi + 1;
-        z.off = off[zi];
-        z.seed = seedBase + 104729ULL * (uint64_t)(zi + 1);
-        z.gcnt = b.cnt.data();
-        z.gstart = b.start.data();
-        z.glist = b.list.data();
-        z.gpid = pid.data();
-        z.gcid = cid.data();
-        z.owner = owner.data();
-        z.x = x.data();
-        z.y = y.data();
-        z.vx = vx.data();
-        z.vy = vy.data();
-        z.vsx = vsx.data();
-        z.vsy = vsy.data();
-        z.gcover = anyOver ? over.data() : nullptr;
-        z.next = next.data();
-        z.tmpg = tmpg.data();
-        std::vector<unsigned char> buf(zone_bytes(z.C, z.zw, zp.useLocal) + 16);
-        zone_carve(z, buf.data(), z.C, z.zw, zp.useLocal);
-        zone_setup(z, 0, 1);
-        if (z.P != off[zi + 1] - off[zi]) return -1;
-        zone_dense_passes(z, 0, 1);
-        zone_sparse_pass(z, 0, 1);
-        for (int lc = 0; lc < z.C; ++lc) zone_correct_sums(z, lc);
-        for (int lc = 0; lc < z.C; ++lc) zone_correct_write(z, lc);
-        zone_final_stats(z);
-        md += z.movedDense;
-        ms += z.movedSparse;
-        if (z.stats4 > 0.0) {
-            acc += z.stats4 * z.stats4;
+        int P = 0;
+        for (int gx = r.xs; gx < r.xe; ++gx)
+            P += b.start[(size_t)(r.ye - 1 + p->Ny * gx) + 1] - b.start[(size_t)(r.ys + p->Ny * gx)];
+        const int C = (r.xe - r.xs) * (r.ye - r.ys);
+        std::vector<double> duG(2 * (size_t)C, -555.0);
+        ZoneShared sh{};
+        const uint64_t seed = seedBase + 104729ULL * (uint64_t)(zi + 1);
+        if (wide) {
+            std::vector<int> nxG((size_t)P + 1, -999), tmpG((size_t)P + 1, -999);
+            run_zone<int>(in, r, (int)zi + 1, seed, (size_t)ldsBytes, nxG, tmpG, duG, sh);
+        } else {
+            if (P > 65534 || C > 65534) return -3;
+            std::vector<uint16_t> nxG((size_t)P + 1, 0xABCD), tmpG((size_t)P + 1, 0xABCD);
+            run_zone<uint16_t>(in, r, (int)zi + 1, seed, (size_t)ldsBytes, nxG, tmpG, duG, sh);
+        }
+        if (sh.P != P) return -1;
+        md += sh.movedDense;
+        ms += sh.movedSparse;
+        if (sh.stats4 > 0.0) {
+            acc += sh.stats4 * sh.stats4;
             ++nc;
         }
-        mx = std::max(mx, z.stats5);
+        mx = std::max(mx, sh.stats5);
     }
     for (int c = 0; c < p->Nc; ++c) {
         if (!over[(size_t)c]) continue;

@@ -13,25 +13,33 @@ SRC = os.path.join(ROOT, "tests", "core_host.cpp")
 OUT_DIR = os.path.join(ROOT, "tests", "_build")
 SO = os.path.join(OUT_DIR, "libcore_host.so")
 DEPS = [SRC, os.path.join(ROOT, "src_incompressible_openmp_amd", "csrc", "mpcd_core.h"),
+        os.path.join(ROOT, "src_incompressible_openmp_amd", "csrc", "mpcd_zone.h"),
         os.path.join(ROOT, "src_incompressible_openmp_amd", "csrc", "mpcd_host.h"),
         os.path.join(ROOT, "include", "mpcd_rng.h"), os.path.join(ROOT, "include", "mpcd_gpu.h")]
 
 _dp = ctypes.POINTER(ctypes.c_double)
 
 
-def build() -> str:
+SO_SMALL = os.path.join(OUT_DIR, "libcore_host_small.so")
+# staging limits small enough that the golden cases run past them
+SMALL_DEFS = ["-DMPCD_ZONE_MAX_MOVE=2", "-DMPCD_ZONE_TMP_L=4"]
+
+
+def build(small: bool = False) -> str:
     os.makedirs(OUT_DIR, exist_ok=True)
-    if not os.path.exists(SO) or any(os.path.getmtime(d) > os.path.getmtime(SO) for d in DEPS):
+    so = SO_SMALL if small else SO
+    if not os.path.exists(so) or any(os.path.getmtime(d) > os.path.getmtime(so) for d in DEPS):
         subprocess.run(["g++", "-std=c++17", "-O2", "-fPIC", "-shared", "-ffp-contract=off",
+                        *(SMALL_DEFS if small else []),
                         "-I" + os.path.join(ROOT, "include"),
-                        "-I" + os.path.join(ROOT, "src_incompressible_openmp_amd", "csrc"), SRC, "-o", SO],
+                        "-I" + os.path.join(ROOT, "src_incompressible_openmp_amd", "csrc"), SRC, "-o", so],
                        check=True)
-    return SO
+    return so
 
 
 class CoreHost:
-    def __init__(self):
-        self.lib = ctypes.CDLL(build())
+    def __init__(self, small: bool = False):
+        self.lib = ctypes.CDLL(build(small))
 
     @staticmethod
     def _p(a, ct=ctypes.c_double):
@@ -45,11 +53,14 @@ class CoreHost:
         cid = np.ascontiguousarray(cid, np.int32)
         return self.lib.core_collide(ctypes.byref(p), self._p(v), self._p(cid, ctypes.c_int32), ctypes.c_uint64(seed))
 
-    def zone_pass(self, p, x, v, layout, seed_base):
+    def zone_pass(self, p, x, v, layout, seed_base, wide=False, ws_bytes=38 * 1024):
+        """wide: int working set (else uint16_t); ws_bytes: working-set bytes,
+        list links beyond them go to the zone's global region."""
         moved = np.zeros(2, np.int32)
         corr = np.zeros(2)
         rc = self.lib.core_zone_pass(ctypes.byref(p), self._p(x), self._p(v), layout, ctypes.c_uint64(seed_base),
-                                     self._p(moved, ctypes.c_int32), self._p(corr))
+                                     self._p(moved, ctypes.c_int32), self._p(corr), int(wide),
+                                     ctypes.c_longlong(ws_bytes))
         if rc != 0:
             raise RuntimeError(f"core_zone_pass failed: {rc}")
         return moved, corr
