@@ -179,6 +179,12 @@ int mpcd_gpu_run_zone_kernel(const mpcd_params* p, double* x, double* v, int xs,
 int mpcd_gpu_zone_pass(const mpcd_params* p, double* x, double* v, int layoutMode, uint64_t rngSeedBase,
                        mpcd_zone_pass_metrics* m, int32_t* ownerMap);
 
+/* replaces mpcd::run_zone_pass (zone_pass.h:71-74): the zones of the layout
+ * applied one after another to the evolving state (Gauss-Seidel), each
+ * seeded rngSeedBase + 104729 (zone index + 1) */
+int mpcd_gpu_zone_pass_gauss_seidel(const mpcd_params* p, double* x, double* v, int layoutMode,
+                                    uint64_t rngSeedBase, mpcd_zone_pass_metrics* m, int32_t* ownerMap);
+
 /* replaces mpcd::run_liquid_closure (liquid_closure.h:46-48) */
 int mpcd_gpu_liquid_closure(const mpcd_params* p, const double* xRef, const double* vRef, const double* xRed,
                             const double* vRed, double* xOut, double* vOut, mpcd_closure_metrics* m,

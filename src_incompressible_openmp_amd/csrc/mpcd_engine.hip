@@ -162,21 +162,37 @@ __global__ void k_cells(Geom g, const double* __restrict__ x, const double* __re
 
 /* ---- counting sort ---- */
 
-/* Cell histogram.  Lanes of a wavefront that hit the same cell add once:
- * the state is kept sorted by cell, so a wavefront covers a few cells. */
+/* Groups of lanes of a wavefront holding the same key: each active lane
+ * gets its group's lowest lane (the leader), its rank in the group and the
+ * group size.  ALU only (one readlane and one ballot per group); the state
+ * is kept sorted by cell, so a wavefront holds a few keys. */
+__device__ inline void wave_groups(int c, bool active, int lane, int& leader, int& rank, int& size) {
+    unsigned long long act = __ballot(active);
+    leader = -1;
+    rank = 0;
+    size = 0;
+    while (act) {
+        const int l = __ffsll((long long)act) - 1;
+        const int lc = __builtin_amdgcn_readlane(c, l);
+        const bool mine = active && c == lc;
+        const unsigned long long grp = __ballot(mine);
+        if (mine) {
+            leader = l;
+            rank = __popcll(grp & ((1ull << lane) - 1ull));
+            size = __popcll(grp);
+        }
+        act &= ~grp;
+    }
+}
+
+/* Cell histogram: one atomic per key group, all groups in one instruction. */
 __global__ void __launch_bounds__(TPB) k_hist(const int* __restrict__ cid, int n, int* __restrict__ cnt) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int lane = __lane_id();
     const int c = (i < n) ? cid[i] : -1;
-    bool active = c >= 0;
-    for (unsigned long long act = __ballot(active); act; act = __ballot(active)) {
-        const int leader = __ffsll((long long)act) - 1;
-        const int lc = __shfl(c, leader, 64);
-        const bool mine = active && c == lc;
-        const unsigned long long grp = __ballot(mine);
-        if (lane == leader) atomicAdd(&cnt[lc], __popcll(grp));
-        if (mine) active = false;
-    }
+    int leader, rank, size;
+    wave_groups(c, c >= 0, lane, leader, rank, size);
+    if (c >= 0 && lane == leader) atomicAdd(&cnt[c], size);
 }
 
 /* exclusive scan, level 1: each block scans SCAN_TILE entries */
@@ -236,75 +252,71 @@ __global__ void k_scan_add(int* __restrict__ out, int m, const int* __restrict__
     if (i < m) out[i] += tileSum[i / SCAN_TILE];
 }
 
-/* Scatter slots into their cell's run (one atomic per cell per wavefront). */
-__global__ void __launch_bounds__(TPB) k_scatter(const int* __restrict__ cid, int n, const int* __restrict__ start,
-                                                 int* __restrict__ cursor, int* __restrict__ list) {
+/* Scatter (pid, slot) keys into their cell's run: one returning atomic per
+ * key group, all groups in one instruction, the group base passed back from
+ * its leader. */
+__global__ void __launch_bounds__(TPB) k_scatter(const int* __restrict__ cid, const int* __restrict__ pid, int n,
+                                                 const int* __restrict__ start, int* __restrict__ cursor,
+                                                 unsigned long long* __restrict__ key) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int lane = __lane_id();
-    const int c = (i < n) ? cid[i] : -1;
-    bool active = c >= 0;
-    for (unsigned long long act = __ballot(active); act; act = __ballot(active)) {
-        const int leader = __ffsll((long long)act) - 1;
-        const int lc = __shfl(c, leader, 64);
-        const bool mine = active && c == lc;
-        const unsigned long long grp = __ballot(mine);
-        int base = 0;
-        if (lane == leader) base = atomicAdd(&cursor[lc], __popcll(grp));
-        base = __shfl(base, leader, 64);
-        if (mine) {
-            list[start[c] + base + __popcll(grp & ((1ull << lane) - 1ull))] = i;
-            active = false;
-        }
-    }
+    const bool in = i < n;
+    const int c = in ? cid[i] : -1;
+    const unsigned p = in ? (unsigned)pid[i] : 0u;
+    const int s0 = (c >= 0) ? start[c] : 0;
+    int leader, rank, size;
+    wave_groups(c, c >= 0, lane, leader, rank, size);
+    int old = 0;
+    if (c >= 0 && lane == leader) old = atomicAdd(&cursor[c], size);
+    const int base = __shfl(old, leader < 0 ? lane : leader, 64);
+    if (c >= 0) key[s0 + base + rank] = ((unsigned long long)p << 32) | (unsigned)i;
 }
 
-/* Order each cell's run by pid.  A block takes SEG_CELLS consecutive cells,
- * whose runs are one contiguous range of the list, stages (pid, slot) keys
- * in LDS and insertion-sorts each run (runs are short and nearly sorted);
- * a range too large for LDS is sorted in place in global memory. */
+/* Order each cell's run by pid and write the slots.  A block takes
+ * SEG_CELLS consecutive cells, whose runs are one contiguous range of the
+ * keys, stages the range in LDS and insertion-sorts each run (runs are short
+ * and nearly sorted); a range too large for LDS is sorted in place in
+ * global memory. */
 constexpr int SEG_CELLS = 256;
 constexpr int SEG_CAP = 6144;
-__global__ void __launch_bounds__(SEG_CELLS) k_segsort(const int* __restrict__ start, int Nc, int* __restrict__ list,
-                                                       const int* __restrict__ pid) {
-    __shared__ unsigned long long key[SEG_CAP];
+__global__ void __launch_bounds__(SEG_CELLS) k_segsort(const int* __restrict__ start, int Nc,
+                                                       unsigned long long* __restrict__ key, int* __restrict__ list) {
+    __shared__ unsigned long long sk[SEG_CAP];
     const int c0 = blockIdx.x * SEG_CELLS;
     const int c1 = min(c0 + SEG_CELLS, Nc);
     const int b0 = start[c0], b1 = start[c1];
     const int c = c0 + (int)threadIdx.x;
     if (b1 - b0 <= SEG_CAP) {
-        for (int q = b0 + (int)threadIdx.x; q < b1; q += SEG_CELLS) {
-            const int slot = list[q];
-            key[q - b0] = ((unsigned long long)(unsigned)pid[slot] << 32) | (unsigned)slot;
-        }
+        for (int q = b0 + (int)threadIdx.x; q < b1; q += SEG_CELLS) sk[q - b0] = key[q];
         __syncthreads();
         if (c < c1) {
             const int b = start[c] - b0, e = start[c + 1] - b0;
             for (int i = b + 1; i < e; ++i) {
-                const unsigned long long kv = key[i];
+                const unsigned long long kv = sk[i];
                 int j = i - 1;
-                while (j >= b && key[j] > kv) {
-                    key[j + 1] = key[j];
+                while (j >= b && sk[j] > kv) {
+                    sk[j + 1] = sk[j];
                     --j;
                 }
-                key[j + 1] = kv;
+                sk[j + 1] = kv;
             }
         }
         __syncthreads();
-        for (int q = b0 + (int)threadIdx.x; q < b1; q += SEG_CELLS) list[q] = (int)(unsigned)(key[q - b0] & 0xffffffffull);
+        for (int q = b0 + (int)threadIdx.x; q < b1; q += SEG_CELLS) list[q] = (int)(unsigned)(sk[q - b0] & 0xffffffffull);
         return;
     }
     if (c >= c1) return;
     const int b = start[c], e = start[c + 1];
     for (int i = b + 1; i < e; ++i) {
-        const int kslot = list[i];
-        const int kp = pid[kslot];
+        const unsigned long long kv = key[i];
         int j = i - 1;
-        while (j >= b && pid[list[j]] > kp) {
-            list[j + 1] = list[j];
+        while (j >= b && key[j] > kv) {
+            key[j + 1] = key[j];
             --j;
         }
-        list[j + 1] = kslot;
+        key[j + 1] = kv;
     }
+    for (int i = b; i < e; ++i) list[i] = (int)(unsigned)(key[i] & 0xffffffffull);
 }
 
 /* ---- collision ---- */
@@ -753,6 +765,7 @@ T* dalloc(size_t n) {
 
 struct Binning {
     int *cnt = nullptr, *start = nullptr, *cursor = nullptr, *list = nullptr;
+    unsigned long long* key = nullptr; /* (pid << 32 | slot) in cell runs, before the pid sort */
 };
 
 }  // namespace
@@ -871,6 +884,7 @@ struct mpcd_gpu_sim {
         bin.start = dalloc<int>((size_t)Nc + 1);
         bin.cursor = dalloc<int>((size_t)Nc + 1);
         bin.list = dalloc<int>(n);
+        bin.key = dalloc<unsigned long long>(n);
         nTiles = (int)((Nc + 1 + SCAN_TILE - 1) / SCAN_TILE);
         tileSum = dalloc<int>(nTiles);
         stage = dalloc<double>(2 * (size_t)n);
@@ -944,7 +958,7 @@ struct mpcd_gpu_sim {
     }
 
     ~mpcd_gpu_sim() {
-        void* ptrs[] = {x, y, vx, vy, vx2, vy2, x2, y2, pid, pid2, cid, cidS, bin.cnt, bin.start, bin.cursor, bin.list, tileSum, next,
+        void* ptrs[] = {x, y, vx, vy, vx2, vy2, x2, y2, pid, pid2, cid, cidS, bin.cnt, bin.start, bin.cursor, bin.list, bin.key, tileSum, next,
                         tmpg, zsize, zoff, owner[0], owner[1], cover[0], cover[1], overCells[0], overCells[1], stage,
                         refPx, refPy, refUx, refUy, pdriveRaw, redPx, redPy, redUx, redUy, duRX, duRY, duEX, duEY,
                         pdrive, cellTmp, mask, part, scal, err, wallPart, rects[0], rects[1], zout, zscratch, duG};
@@ -1008,9 +1022,10 @@ struct mpcd_gpu_sim {
         hipLaunchKernelGGL(k_scan_tiles, dim3(nTiles), dim3(TPB), 0, stream, bin.cnt, Nc + 1, bin.start, tileSum);
         hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(TPB), 0, stream, tileSum, nTiles);
         hipLaunchKernelGGL(k_scan_add, dim3(nblk(Nc + 1)), dim3(TPB), 0, stream, bin.start, Nc + 1, tileSum);
-        hipLaunchKernelGGL(k_scatter, dim3(nblk(n)), dim3(TPB), 0, stream, key, n, bin.start, bin.cursor, bin.list);
-        hipLaunchKernelGGL(k_segsort, dim3(nblk(Nc, SEG_CELLS)), dim3(SEG_CELLS), 0, stream, bin.start, Nc, bin.list,
-                           pid);
+        hipLaunchKernelGGL(k_scatter, dim3(nblk(n)), dim3(TPB), 0, stream, key, pid, n, bin.start, bin.cursor,
+                           bin.key);
+        hipLaunchKernelGGL(k_segsort, dim3(nblk(Nc, SEG_CELLS)), dim3(SEG_CELLS), 0, stream, bin.start, Nc, bin.key,
+                           bin.list);
         check_launch();
     }
     void bin_unshifted() {
@@ -1631,6 +1646,54 @@ int mpcd_gpu_zone_pass(const mpcd_params* p, double* x, double* v, int layoutMod
         HIP_OK(hipFree(dOwn));
         if (dCov) HIP_OK(hipFree(dCov));
         if (dOver) HIP_OK(hipFree(dOver));
+        if (m) *m = mm;
+        if (ownerMap) std::memcpy(ownerMap, own.data(), own.size() * sizeof(int32_t));
+    });
+}
+
+int mpcd_gpu_zone_pass_gauss_seidel(const mpcd_params* p, double* x, double* v, int layoutMode,
+                                    uint64_t rngSeedBase, mpcd_zone_pass_metrics* m, int32_t* ownerMap) {
+    return guarded([&] {
+        std::lock_guard<std::mutex> lk(g_cache_mu);
+        if (layoutMode != MPCD_LAYOUT_BASE && layoutMode != MPCD_LAYOUT_SHIFTED && layoutMode != MPCD_LAYOUT_SINGLE_FULL)
+            throw ApiError(MPCD_E_INVALID, "unknown layout mode");
+        auto& s = scratch_sim(*p);
+        s.check_pass_modes();
+        s.set_state(x, v);
+        std::vector<int32_t> own;
+        const auto r = layout_rects(*p, layoutMode, &own);
+        ZoneRect* d = dalloc<ZoneRect>(r.size());
+        HIP_OK(hipMemcpyAsync(d, r.data(), r.size() * sizeof(ZoneRect), hipMemcpyHostToDevice, s.stream));
+        s.ensure_zout(1);
+        s.clear_err();
+        /* zones one after another, each on the state its predecessors left
+         * (zone_pass.cpp:1131-1141): fresh binning, one zone owning all
+         * its particles, seed rngSeedBase + 104729 (iz+1) */
+        mpcd_zone_pass_metrics mm{};
+        mm.nZones = (int)r.size();
+        double acc = 0.0, mx = 0.0;
+        int cnt = 0;
+        for (size_t iz = 0; iz < r.size(); ++iz) {
+            const ZoneRect q = r[iz];
+            s.bin_unshifted();
+            s.zone_pass(rngSeedBase + 104729ULL * (uint64_t)(iz + 1), d + iz, 1, (q.xe - q.xs) * (q.ye - q.ys),
+                        q.xe - q.xs, nullptr, nullptr, nullptr, 0, 1);
+            ZoneOut o{};
+            HIP_OK(hipMemcpyAsync(&o, s.zout, sizeof(o), hipMemcpyDeviceToHost, s.stream));
+            HIP_OK(hipStreamSynchronize(s.stream));
+            mm.nParticlesMovedDense += o.movedDense;
+            mm.nParticlesMovedSparse += o.movedSparse;
+            if (o.stats4 > 0.0) {
+                acc += o.stats4 * o.stats4;
+                ++cnt;
+            }
+            mx = std::max(mx, o.stats5);
+        }
+        s.check_err();
+        mm.corrRmsDU = cnt > 0 ? std::sqrt(acc / (double)cnt) : 0.0;
+        mm.corrMaxDU = mx;
+        s.get_state(x, v);
+        HIP_OK(hipFree(d));
         if (m) *m = mm;
         if (ownerMap) std::memcpy(ownerMap, own.data(), own.size() * sizeof(int32_t));
     });

@@ -184,11 +184,15 @@ def run_zone_kernel(p: MpcdParams, x: np.ndarray, v: np.ndarray, rect, seed: int
     return stats, moved
 
 
-def zone_pass(p: MpcdParams, x: np.ndarray, v: np.ndarray, layout: int, seed_base: int):
+def zone_pass(p: MpcdParams, x: np.ndarray, v: np.ndarray, layout: int, seed_base: int,
+              gauss_seidel: bool = False):
+    """Snapshot pass (run_zone_pass_openmp), or with gauss_seidel the zones
+    applied one after another (run_zone_pass)."""
     m = PassMetrics()
     owner = np.zeros(p.Nc, np.int32)
-    _check(load().mpcd_gpu_zone_pass(ctypes.byref(p), _ptr(_f64(x)), _ptr(_f64(v)), layout,
-                                     ctypes.c_uint64(seed_base), ctypes.byref(m), _ptr(owner, ctypes.c_int32)))
+    fn = load().mpcd_gpu_zone_pass_gauss_seidel if gauss_seidel else load().mpcd_gpu_zone_pass
+    _check(fn(ctypes.byref(p), _ptr(_f64(x)), _ptr(_f64(v)), layout, ctypes.c_uint64(seed_base), ctypes.byref(m),
+              _ptr(owner, ctypes.c_int32)))
     return struct_to_dict(m), owner
 
 

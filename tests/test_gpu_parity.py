@@ -113,6 +113,22 @@ def test_zone_pass(gpu, oracle, case, layout):
     assert mg["corrRmsDU"] == mo["corrRmsDU"] and mg["corrMaxDU"] == mo["corrMaxDU"]
 
 
+@pytest.mark.parametrize("case", CASES)
+@pytest.mark.parametrize("layout", [LAYOUT_BASE, LAYOUT_SHIFTED])
+def test_zone_pass_gauss_seidel(gpu, oracle, case, layout):
+    """run_zone_pass: zones applied one after another to the evolving state."""
+    p, x, v = post_collision(oracle, case)
+    xo, vo = x.copy(), v.copy()
+    mo, owo = oracle.zone_pass(CTR, p, xo, vo, layout, 88172645463325252, gauss_seidel=True)
+    xg, vg = x.copy(), v.copy()
+    mg, owg = gpu.zone_pass(p, xg, vg, layout, 88172645463325252, gauss_seidel=True)
+    assert np.array_equal(owg, owo)
+    assert mg["nParticlesMovedDense"] == mo["nParticlesMovedDense"]
+    assert mg["nParticlesMovedSparse"] == mo["nParticlesMovedSparse"]
+    assert bits(xg, xo) and bits(vg, vo)
+    assert mg["corrRmsDU"] == mo["corrRmsDU"] and mg["corrMaxDU"] == mo["corrMaxDU"]
+
+
 @pytest.mark.parametrize("case", ["small", "bcmix", "override"])
 def test_single_zone_kernel(gpu, oracle, case):
     """run_zone_kernel_inplace on individual zones of both layouts."""
