@@ -115,6 +115,13 @@ typedef struct mpcd_closure_grids {
     int32_t* interzoneMask;
 } mpcd_closure_grids;
 
+/* Occupancy statistics of a pass (ZonePassMetrics' before/after fields,
+ * zone_pass.cpp:975-1044). */
+typedef struct mpcd_occupancy_stats {
+    double occStd, outBand;
+    int32_t nDense, nSparse, nEmptyActive, _pad;
+} mpcd_occupancy_stats;
+
 /* Metrics of one resident time step. */
 typedef struct mpcd_step_metrics {
     mpcd_wall_info wall;
@@ -185,6 +192,11 @@ int mpcd_gpu_zone_pass(const mpcd_params* p, double* x, double* v, int layoutMod
 int mpcd_gpu_zone_pass_gauss_seidel(const mpcd_params* p, double* x, double* v, int layoutMode,
                                     uint64_t rngSeedBase, mpcd_zone_pass_metrics* m, int32_t* ownerMap);
 
+/* occupancy statistics from Nc cell counts (compute_occ_std_from_counts,
+ * compute_out_band_from_counts, count_dense/sparse_from_counts,
+ * count_empty_active: zone_pass.cpp:975-1044); host computation */
+int mpcd_gpu_occupancy_stats(const mpcd_params* p, const int32_t* counts, mpcd_occupancy_stats* out);
+
 /* replaces mpcd::run_liquid_closure (liquid_closure.h:46-48) */
 int mpcd_gpu_liquid_closure(const mpcd_params* p, const double* xRef, const double* vRef, const double* xRed,
                             const double* vRed, double* xOut, double* vOut, mpcd_closure_metrics* m,
@@ -216,6 +228,49 @@ int mpcd_gpu_sim_profile(mpcd_gpu_sim* s, int enable);
 int mpcd_gpu_sim_profile_read(mpcd_gpu_sim* s, double* total_ms, int32_t* launches);
 /* the hipStream_t the simulation launches on */
 int mpcd_gpu_sim_stream(mpcd_gpu_sim* s, void** stream);
+
+/* ---- closure surrogate: a 4-layer 3x3 convolutional network
+ * (4 -> 32 -> 32 -> 32 -> 2 channels, ReLU after the first three) over the
+ * cell grid, bfloat16 weights and activations on the matrix cores with
+ * float32 sums.  No counterpart in the reference: it stands in for the
+ * run_liquid_closure call of the benchmark driver
+ * (main_benchmark_poiseuille_openmp_profiled_shifted_profiled.cpp:277) on
+ * the steps the plain closure skips.  Grids: B samples of Nx*Ny cells,
+ * cell index iy + Ny*ix; features [sample][cell][4], outputs and targets
+ * [sample][cell][2]; parameters (float64) per layer W[co][tap][ci] with
+ * tap = 3*(dx+1) + (dy+1), then b[co]. ---- */
+typedef struct mpcd_gpu_net mpcd_gpu_net;
+int mpcd_gpu_net_num_params(void);
+int mpcd_gpu_net_create(int Nx, int Ny, int perx, int pery, int maxBatch, mpcd_gpu_net** out);
+int mpcd_gpu_net_destroy(mpcd_gpu_net* net);
+/* sets the parameters and resets the optimiser state */
+int mpcd_gpu_net_set_params(mpcd_gpu_net* net, const double* theta);
+int mpcd_gpu_net_get_params(mpcd_gpu_net* net, double* theta);
+int mpcd_gpu_net_forward(mpcd_gpu_net* net, const float* feat, int B, float* out);
+/* mean squared error over B samples and its gradient (either may be NULL) */
+int mpcd_gpu_net_loss_grad(mpcd_gpu_net* net, const float* feat, const float* target, int B, double* loss,
+                           double* grad);
+/* one Adam step (lr; betas 0.9, 0.999; eps 1e-8) with the given gradient, or
+ * with the last computed one when grad is NULL */
+int mpcd_gpu_net_adam_step(mpcd_gpu_net* net, const double* grad, double lr);
+/* iters Adam steps on mini-batches of `batch` samples drawn (counter-based,
+ * seeded) from nSamples samples; losses[iters] (nullable) */
+int mpcd_gpu_net_train(mpcd_gpu_net* net, const float* feat, const float* target, int nSamples, int batch,
+                       int iters, double lr, uint64_t seed, double* losses);
+/* the resident loop with the surrogate: steps s with s % K != 0 take the
+ * surrogate's correction (outScale * network output) instead of the plain
+ * closure; a prediction beyond duCap (or not finite) falls back to the plain
+ * closure for that step.  K <= 0 or net NULL: plain closure every step. */
+int mpcd_gpu_sim_set_surrogate(mpcd_gpu_sim* s, mpcd_gpu_net* net, int K, double outScale, double duCap);
+/* steps taken with the surrogate, with the plain closure, and fallbacks */
+int mpcd_gpu_sim_surrogate_counts(mpcd_gpu_sim* s, int32_t* counts3);
+/* training samples: with enable, every plain closure also stores its
+ * features [cell][4] and its applied correction / outScale [cell][2];
+ * feat/target (nullable) receive the last stored sample */
+int mpcd_gpu_sim_closure_sample(mpcd_gpu_sim* s, int enable, double outScale, float* feat, float* target);
+/* one 16x16x32 bfloat16 matrix-core product of 64 lanes x 8 operands each
+ * (a, b: 512 raw bfloat16 values, d: 256 floats), to check the lane layout */
+int mpcd_gpu_net_mfma_probe(const uint16_t* a, const uint16_t* b, float* d);
 
 #ifdef __cplusplus
 }

@@ -15,8 +15,8 @@ ROOT = os.path.dirname(PKG)
 CSRC = os.path.join(PKG, "csrc")
 LIB = os.path.join(PKG, "libmpcd_gpu.so")
 
-HIP_SOURCES = [os.path.join(CSRC, "mpcd_engine.hip")]
-CXX_SOURCES = [os.path.join(CSRC, "mpcd_api.cpp"), os.path.join(CSRC, "host_io.cpp")]
+HIP_SOURCES = [os.path.join(CSRC, "mpcd_engine.hip"), os.path.join(CSRC, "mpcd_net.hip")]
+CXX_SOURCES = [os.path.join(CSRC, "mpcd_api.cpp"), os.path.join(CSRC, "host_io.cpp"), os.path.join(CSRC, "host_diag.cpp")]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 COMMON = ["-std=c++17", "-O3", "-fPIC", "-ffp-contract=off", "-I" + os.path.join(ROOT, "include"),

@@ -326,6 +326,19 @@ ZonePassResult run_zone_pass(const State& stateIn, const Params& params, const s
     ok(mpcd_gpu_zone_pass_gauss_seidel(&p, out.stateOut.x.data(), out.stateOut.v.data(),
                                        layout_mode(params, layoutMode), rngSeedBase, &m, out.zoneOwnerMap.data()));
     out.afterFields = fields_of(out.stateOut.x, out.stateOut.v, params, rhoT);
+    mpcd_occupancy_stats sb{}, sa{};
+    ok(mpcd_gpu_occupancy_stats(&p, out.beforeFields.N.data(), &sb));
+    ok(mpcd_gpu_occupancy_stats(&p, out.afterFields.N.data(), &sa));
+    out.metrics.occStdBefore = sb.occStd;
+    out.metrics.occStdAfter = sa.occStd;
+    out.metrics.outBandBefore = sb.outBand;
+    out.metrics.outBandAfter = sa.outBand;
+    out.metrics.nDenseBefore = sb.nDense;
+    out.metrics.nDenseAfter = sa.nDense;
+    out.metrics.nSparseBefore = sb.nSparse;
+    out.metrics.nSparseAfter = sa.nSparse;
+    out.metrics.nEmptyActiveBefore = sb.nEmptyActive;
+    out.metrics.nEmptyActiveAfter = sa.nEmptyActive;
     out.metrics.layoutMode = layoutMode;
     out.metrics.nZonesExecuted = m.nZones;
     out.metrics.nParticlesMovedDense = m.nParticlesMovedDense;

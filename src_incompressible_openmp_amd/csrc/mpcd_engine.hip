@@ -20,6 +20,7 @@
 #include <vector>
 
 #include "mpcd_core.h"
+#include "mpcd_hip_util.h"
 #include "mpcd_host.h"
 
 using namespace mpcdk;
@@ -27,17 +28,6 @@ using namespace mpcdk;
 namespace {
 
 thread_local std::string g_last_error;
-
-struct HipError : std::runtime_error {
-    using std::runtime_error::runtime_error;
-};
-
-#define HIP_OK(expr)                                                                                  \
-    do {                                                                                              \
-        hipError_t e_ = (expr);                                                                       \
-        if (e_ != hipSuccess)                                                                         \
-            throw HipError(std::string(#expr) + ": " + hipGetErrorString(e_));                        \
-    } while (0)
 
 constexpr int TPB = 256;
 constexpr int ZONE_TPB = 64;
@@ -720,6 +710,52 @@ __global__ void __launch_bounds__(TPB) k_kick_reorder(Geom g, const int* __restr
 }
 
 /* per-cell closure metrics partials: sum hypot(beta*duR), sum pdriveRaw, sum pdrive */
+/* ---- closure surrogate ---- */
+
+/* the surrogate's inputs from the red state's cell fields: occupancy
+ * deviation (N - gamma)/gamma, cell velocity, and T/kBT - 1 (0 where the
+ * temperature is undefined, N < 2) */
+__global__ void k_features(int Nc, const int* __restrict__ N, const double* __restrict__ Ux,
+                           const double* __restrict__ Uy, const double* __restrict__ T, double gamma, double kBT,
+                           float* __restrict__ feat) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= Nc) return;
+    const double t = T[c];
+    feat[4 * (size_t)c] = (float)(((double)N[c] - gamma) / gamma);
+    feat[4 * (size_t)c + 1] = (float)Ux[c];
+    feat[4 * (size_t)c + 2] = (float)Uy[c];
+    feat[4 * (size_t)c + 3] = (N[c] >= 2 && isfinite(t)) ? (float)(t / kBT - 1.0) : 0.0f;
+}
+
+/* the plain closure's applied correction per cell, beta duRepair + duEOS
+ * (the surrogate's training target, scaled by 1/outScale) */
+__global__ void k_closure_target(int Nc, const double* __restrict__ scal, const double* __restrict__ duRX,
+                                 const double* __restrict__ duRY, const double* __restrict__ duEX,
+                                 const double* __restrict__ duEY, double invScale, float* __restrict__ tgt) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= Nc) return;
+    const double beta = scal[3];
+    tgt[2 * (size_t)c] = (float)((beta * duRX[c] + duEX[c]) * invScale);
+    tgt[2 * (size_t)c + 1] = (float)((beta * duRY[c] + duEY[c]) * invScale);
+}
+
+/* the surrogate's correction as the kick's grids (duEOS = prediction,
+ * duRepair = 0, beta = 0); flag = 1 when a value is not finite or exceeds
+ * the cap (the residual check that sends the step to the plain closure) */
+__global__ void k_learned_du(int Nc, const float* __restrict__ pred, double scale, double cap,
+                             double* __restrict__ duRX, double* __restrict__ duRY, double* __restrict__ duEX,
+                             double* __restrict__ duEY, double* __restrict__ scal, int* __restrict__ flag) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c == 0) scal[3] = 0.0;
+    if (c >= Nc) return;
+    const double ux = (double)pred[2 * (size_t)c] * scale, uy = (double)pred[2 * (size_t)c + 1] * scale;
+    if (!(fabs(ux) <= cap) || !(fabs(uy) <= cap)) atomicOr(flag, 1);
+    duRX[c] = 0.0;
+    duRY[c] = 0.0;
+    duEX[c] = ux;
+    duEY[c] = uy;
+}
+
 __global__ void __launch_bounds__(TPB) k_closure_cell_metrics(int Nc, const double* __restrict__ duRX,
                                                               const double* __restrict__ duRY,
                                                               const double* __restrict__ raw,
@@ -754,14 +790,6 @@ __global__ void __launch_bounds__(TPB) k_closure_cell_metrics(int Nc, const doub
 /* ===================================================================== */
 /*                               engine                                  */
 /* ===================================================================== */
-
-template <class T>
-T* dalloc(size_t n) {
-    void* p = nullptr;
-    if (n == 0) n = 1;
-    HIP_OK(hipMalloc(&p, n * sizeof(T)));
-    return (T*)p;
-}
 
 struct Binning {
     int *cnt = nullptr, *start = nullptr, *cursor = nullptr, *list = nullptr;
@@ -798,6 +826,16 @@ struct mpcd_gpu_sim {
     double* part = nullptr;
     size_t partCap = 0;
     double* scal = nullptr; /* device scalars */
+    /* closure surrogate */
+    mpcd_gpu_net* sur = nullptr;
+    int surK = 0;
+    double surScale = 1.0, surCap = 0.0;
+    bool wantSample = false;
+    int* redN = nullptr;
+    double* redT = nullptr;
+    float *featBuf = nullptr, *tgtBuf = nullptr;
+    int* surFlag = nullptr;
+    int nLearned = 0, nPlain = 0, nFallback = 0;
     int* err = nullptr;
     WallAcc* wallPart = nullptr;
     ZoneRect* rects[2] = {nullptr, nullptr};
@@ -892,6 +930,11 @@ struct mpcd_gpu_sim {
                             &duRX, &duRY, &duEX, &duEY, &pdrive, &cellTmp};
         for (double** c : cells) *c = dalloc<double>(Nc);
         mask = dalloc<int>(Nc);
+        redN = dalloc<int>(Nc);
+        redT = dalloc<double>(Nc);
+        featBuf = dalloc<float>(4 * (size_t)Nc);
+        tgtBuf = dalloc<float>(2 * (size_t)Nc);
+        surFlag = dalloc<int>(1);
         scal = dalloc<double>(64);
         err = dalloc<int>(1);
         ensure_parts((size_t)std::max(nblk(n) * 6, nblk(Nc) * 3) + 64);
@@ -961,7 +1004,8 @@ struct mpcd_gpu_sim {
         void* ptrs[] = {x, y, vx, vy, vx2, vy2, x2, y2, pid, pid2, cid, cidS, bin.cnt, bin.start, bin.cursor, bin.list, bin.key, tileSum, next,
                         tmpg, zsize, zoff, owner[0], owner[1], cover[0], cover[1], overCells[0], overCells[1], stage,
                         refPx, refPy, refUx, refUy, pdriveRaw, redPx, redPy, redUx, redUy, duRX, duRY, duEX, duEY,
-                        pdrive, cellTmp, mask, part, scal, err, wallPart, rects[0], rects[1], zout, zscratch, duG};
+                        pdrive, cellTmp, mask, part, scal, err, wallPart, rects[0], rects[1], zout, zscratch, duG,
+                        redN, redT, featBuf, tgtBuf, surFlag};
         for (void* q : ptrs)
             if (q) (void)hipFree(q);
         for (hipEvent_t e : evPool) (void)hipEventDestroy(e);
@@ -1220,6 +1264,47 @@ struct mpcd_gpu_sim {
         m->corrMaxDU = mx;
     }
 
+    /* The surrogate's closure: red fields -> features -> network -> kick.
+     * A prediction that fails the residual check (not finite, or beyond the
+     * cap) sends the step to the plain closure instead.  ref sums and the
+     * red binning as for closure(). */
+    void closure_learned() {
+        FieldsOut f{};
+        f.N = redN;
+        f.Ux = redUx;
+        f.Uy = redUy;
+        f.T = redT;
+        hipLaunchKernelGGL(k_cell_fields, dim3(nblk(Nc, CELL_BLK)), dim3(CELL_BLK), cellCap * 2 * sizeof(double), stream,
+                           bin.start, bin.list, Nc, vx, vy, vc(), p.Kvirial, rho_target(), f, cellCap);
+        const int nbc = (int)nblk(Nc);
+        hipLaunchKernelGGL(k_features, dim3(nbc), dim3(TPB), 0, stream, Nc, redN, redUx, redUy, redT, p.gamma, p.kBT,
+                           featBuf);
+        const float* pred = net_forward_device(sur, featBuf, stream);
+        hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, stream, surFlag, 1, 0);
+        hipLaunchKernelGGL(k_learned_du, dim3(nbc), dim3(TPB), 0, stream, Nc, pred, surScale, surCap, duRX, duRY, duEX,
+                           duEY, scal, surFlag);
+        check_launch();
+        int bad = 0;
+        HIP_OK(hipMemcpyAsync(&bad, surFlag, sizeof(int), hipMemcpyDeviceToHost, stream));
+        HIP_OK(hipStreamSynchronize(stream));
+        if (bad) {
+            ++nFallback;
+            closure(nullptr);
+            return;
+        }
+        ++nLearned;
+        pbeg(MPCD_PROF_KICK);
+        hipLaunchKernelGGL(k_kick_reorder, dim3(nblk(n)), dim3(TPB), 0, stream, g, bin.list, x, y, vx, vy, pid, x2, y2,
+                           vx2, vy2, pid2, duRX, duRY, duEX, duEY, scal, (double*)nullptr);
+        pend(MPCD_PROF_KICK);
+        std::swap(x, x2);
+        std::swap(y, y2);
+        std::swap(vx, vx2);
+        std::swap(vy, vy2);
+        std::swap(pid, pid2);
+        check_launch();
+    }
+
     /* closure: ref sums already in ref*; red binning current */
     void closure(mpcd_closure_metrics* cm) {
         if (!p.useLiquidClosure) throw ApiError(MPCD_E_INVALID, "run_liquid_closure requires useLiquidClosure=true");
@@ -1228,6 +1313,10 @@ struct mpcd_gpu_sim {
         f.Uy = redUy;
         f.Px = redPx;
         f.Py = redPy;
+        if (wantSample) {
+            f.N = redN;
+            f.T = redT;
+        }
         hipLaunchKernelGGL(k_cell_fields, dim3(nblk(Nc, CELL_BLK)), dim3(CELL_BLK), cellCap * 2 * sizeof(double), stream,
                            bin.start, bin.list, Nc, vx, vy, vc(), p.Kvirial, rho_target(), f, cellCap);
         ClosureArgs ca{};
@@ -1252,6 +1341,12 @@ struct mpcd_gpu_sim {
                            anyMask, pdrive);
         const double dtr = p.dt / rho_target();
         hipLaunchKernelGGL(k_eos, dim3(nbc), dim3(TPB), 0, stream, g, pdrive, dtr, p.betaEOS, duEX, duEY);
+        if (wantSample) {
+            hipLaunchKernelGGL(k_features, dim3(nbc), dim3(TPB), 0, stream, Nc, redN, redUx, redUy, redT, p.gamma, p.kBT,
+                               featBuf);
+            hipLaunchKernelGGL(k_closure_target, dim3(nbc), dim3(TPB), 0, stream, Nc, scal, duRX, duRY, duEX, duEY,
+                               1.0 / surScale, tgtBuf);
+        }
         check_launch();
         const int nbp = (int)nblk(n);
         if (cm) {
@@ -1344,7 +1439,12 @@ struct mpcd_gpu_sim {
         pend(MPCD_PROF_BIN);
         if (stages && stages[4]) get_state(stages[4], stages[5]);
         pbeg(MPCD_PROF_CLOSURE);
-        closure(sm ? &sm->closure : nullptr);
+        if (sur && surK > 0 && s % surK != 0) {
+            closure_learned();
+        } else {
+            ++nPlain;
+            closure(sm ? &sm->closure : nullptr);
+        }
         pend(MPCD_PROF_CLOSURE);
     }
 };
@@ -1353,24 +1453,9 @@ struct mpcd_gpu_sim {
 /*                                C ABI                                  */
 /* ===================================================================== */
 
-namespace {
+void mpcdk::set_last_error(const std::string& m) { g_last_error = m; }
 
-template <class F>
-int guarded(F&& f) {
-    try {
-        f();
-        return MPCD_OK;
-    } catch (const ApiError& e) {
-        g_last_error = e.what();
-        return e.code;
-    } catch (const HipError& e) {
-        g_last_error = e.what();
-        return MPCD_E_HIP;
-    } catch (const std::exception& e) {
-        g_last_error = e.what();
-        return MPCD_E_INVALID;
-    }
-}
+namespace {
 
 /* Scratch simulations reused across stateless calls of the same shape. */
 std::mutex g_cache_mu;
@@ -1648,6 +1733,49 @@ int mpcd_gpu_zone_pass(const mpcd_params* p, double* x, double* v, int layoutMod
         if (dOver) HIP_OK(hipFree(dOver));
         if (m) *m = mm;
         if (ownerMap) std::memcpy(ownerMap, own.data(), own.size() * sizeof(int32_t));
+    });
+}
+
+int mpcd_gpu_sim_set_surrogate(mpcd_gpu_sim* s, mpcd_gpu_net* net, int K, double outScale, double duCap) {
+    return guarded([&] {
+        if (!s) throw ApiError(MPCD_E_INVALID, "null simulation");
+        if (net && K > 0) {
+            int nx, ny, px, py;
+            net_grid(net, &nx, &ny, &px, &py);
+            const int sx = s->p.bc_left == MPCD_BC_PERIODIC && s->p.bc_right == MPCD_BC_PERIODIC;
+            const int sy = s->p.bc_bottom == MPCD_BC_PERIODIC && s->p.bc_top == MPCD_BC_PERIODIC;
+            if (nx != s->p.Nx || ny != s->p.Ny || px != sx || py != sy)
+                throw ApiError(MPCD_E_INVALID, "surrogate grid does not match the simulation");
+            if (!(outScale > 0.0) || !(duCap > 0.0)) throw ApiError(MPCD_E_INVALID, "outScale and duCap must be > 0");
+        }
+        s->sur = (K > 0) ? net : nullptr;
+        s->surK = K;
+        if (outScale > 0.0) s->surScale = outScale;
+        s->surCap = duCap;
+    });
+}
+
+int mpcd_gpu_sim_surrogate_counts(mpcd_gpu_sim* s, int32_t* counts3) {
+    return guarded([&] {
+        if (!s || !counts3) throw ApiError(MPCD_E_INVALID, "null argument");
+        counts3[0] = s->nLearned;
+        counts3[1] = s->nPlain;
+        counts3[2] = s->nFallback;
+    });
+}
+
+int mpcd_gpu_sim_closure_sample(mpcd_gpu_sim* s, int enable, double outScale, float* feat, float* target) {
+    return guarded([&] {
+        if (!s) throw ApiError(MPCD_E_INVALID, "null simulation");
+        if (feat || target) {
+            HIP_OK(hipStreamSynchronize(s->stream));
+            if (feat)
+                HIP_OK(hipMemcpy(feat, s->featBuf, 4 * (size_t)s->Nc * sizeof(float), hipMemcpyDeviceToHost));
+            if (target)
+                HIP_OK(hipMemcpy(target, s->tgtBuf, 2 * (size_t)s->Nc * sizeof(float), hipMemcpyDeviceToHost));
+        }
+        s->wantSample = enable != 0;
+        if (outScale > 0.0) s->surScale = outScale;
     });
 }
 

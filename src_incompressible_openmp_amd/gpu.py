@@ -196,6 +196,22 @@ def zone_pass(p: MpcdParams, x: np.ndarray, v: np.ndarray, layout: int, seed_bas
     return struct_to_dict(m), owner
 
 
+class OccupancyStats(ctypes.Structure):
+    _fields_ = [("occStd", ctypes.c_double), ("outBand", ctypes.c_double), ("nDense", ctypes.c_int32),
+                ("nSparse", ctypes.c_int32), ("nEmptyActive", ctypes.c_int32), ("_pad", ctypes.c_int32)]
+
+
+def occupancy_stats(p: MpcdParams, counts: np.ndarray) -> Dict[str, float]:
+    """occStd, outBand, nDense, nSparse, nEmptyActive of a zone pass's before
+    or after counts (host computation in the library)."""
+    c = np.ascontiguousarray(counts, np.int32)
+    if c.size != p.Nc:
+        raise ValueError("counts must have Nc entries")
+    out = OccupancyStats()
+    _check(load().mpcd_gpu_occupancy_stats(ctypes.byref(p), _ptr(c, ctypes.c_int32), ctypes.byref(out)))
+    return struct_to_dict(out)
+
+
 def liquid_closure(p: MpcdParams, x_ref, v_ref, x_red, v_red):
     n2 = 2 * p.n
     xo, vo = np.zeros(n2), np.zeros(n2)
@@ -268,6 +284,30 @@ class Sim:
         cnt = np.zeros(len(PROF_CLASSES), np.int32)
         _check(load().mpcd_gpu_sim_profile_read(self._h, _ptr(ms), _ptr(cnt, ctypes.c_int32)))
         return {k: (float(ms[i]), int(cnt[i])) for i, k in enumerate(PROF_CLASSES)}
+
+    def set_surrogate(self, net, K: int, out_scale: float, du_cap: float) -> None:
+        """Steps s with s % K != 0 use the closure surrogate `net`
+        (src_incompressible_openmp_amd.surrogate.Net); net None or K <= 0:
+        plain closure every step."""
+        h = net._h if net is not None else ctypes.c_void_p()
+        _check(load().mpcd_gpu_sim_set_surrogate(self._h, h, int(K), ctypes.c_double(out_scale),
+                                                 ctypes.c_double(du_cap)))
+
+    def surrogate_counts(self):
+        c = np.zeros(3, np.int32)
+        _check(load().mpcd_gpu_sim_surrogate_counts(self._h, _ptr(c, ctypes.c_int32)))
+        return {"learned": int(c[0]), "plain": int(c[1]), "fallback": int(c[2])}
+
+    def closure_sample(self, enable: bool = True, out_scale: float = 0.0, fetch: bool = False):
+        """Enables storing each plain closure's features and target
+        (correction / out_scale); with fetch, returns the last pair."""
+        f = np.zeros((self.p.Nc, 4), np.float32) if fetch else None
+        t = np.zeros((self.p.Nc, 2), np.float32) if fetch else None
+        _check(load().mpcd_gpu_sim_closure_sample(
+            self._h, int(enable), ctypes.c_double(out_scale),
+            f.ctypes.data_as(ctypes.POINTER(ctypes.c_float)) if fetch else None,
+            t.ctypes.data_as(ctypes.POINTER(ctypes.c_float)) if fetch else None))
+        return (f, t) if fetch else None
 
     def stream(self) -> int:
         s = ctypes.c_void_p()

@@ -1,0 +1,136 @@
+"""Closure surrogate kernels against the CPU reference (tests/surrogate_ref.py,
+which rounds to bfloat16 at the same points as the GPU and sums in float64):
+the matrix-core lane layout, forward, loss and gradient, the Adam step, and
+that training on the GPU fits a teacher network."""
+import numpy as np
+import pytest
+import torch
+
+from tests import surrogate_ref as R
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def S():
+    from src_incompressible_openmp_amd import gpu, surrogate
+
+    gpu.load()
+    assert gpu.device_count() >= 1
+    return surrogate
+
+
+def to_gpu(t):
+    """[B,C,X,Y] -> [B, X*Y, C] (cell index iy + Ny*ix)"""
+    t = np.asarray(t)
+    B, C, X, Y = t.shape
+    return np.ascontiguousarray(t.transpose(0, 2, 3, 1).reshape(B, X * Y, C))
+
+
+def from_gpu(a, X, Y):
+    B, _, C = a.shape
+    return a.reshape(B, X, Y, C).transpose(0, 3, 1, 2)
+
+
+def bf16_bits(x):
+    t = torch.as_tensor(np.asarray(x, np.float32)).to(torch.bfloat16)
+    return t.view(torch.int16).numpy().astype(np.uint16), t.to(torch.float64).numpy()
+
+
+def test_mfma_lane_layout(S):
+    """A[i][k] from lane 16*(k//8) + i, element k%8; B[k][n] from lane
+    16*(k//8) + n; D[i][n] to lane 16*(i//4) + n, element i%4."""
+    rng = np.random.default_rng(0)
+    A = rng.normal(size=(16, 32))
+    Bm = rng.normal(size=(32, 16))
+    a = np.zeros((64, 8))
+    b = np.zeros((64, 8))
+    for lane in range(64):
+        for j in range(8):
+            a[lane, j] = A[lane % 16, 8 * (lane // 16) + j]
+            b[lane, j] = Bm[8 * (lane // 16) + j, lane % 16]
+    abits, aval = bf16_bits(a)
+    bbits, bval = bf16_bits(b)
+    Aq = np.zeros((16, 32))
+    Bq = np.zeros((32, 16))
+    for lane in range(64):
+        for j in range(8):
+            Aq[lane % 16, 8 * (lane // 16) + j] = aval[lane, j]
+            Bq[8 * (lane // 16) + j, lane % 16] = bval[lane, j]
+    want = Aq @ Bq
+    d = S.mfma_probe(abits, bbits)
+    got = np.zeros((16, 16))
+    for lane in range(64):
+        for r in range(4):
+            got[4 * (lane // 16) + r, lane % 16] = d[lane, r]
+    assert np.max(np.abs(got - want)) <= 1e-4 * np.max(np.abs(want))
+
+
+GRIDS = [(12, 10, True, False), (9, 7, False, True), (8, 16, True, True)]
+
+
+def rand_case(seed, X, Y, B=2):
+    rng = np.random.default_rng(seed)
+    f = rng.normal(size=(B, 4, X, Y)) * np.array([0.3, 0.5, 0.5, 0.2]).reshape(1, 4, 1, 1)
+    t = rng.normal(size=(B, 2, X, Y))
+    return f, t
+
+
+@pytest.mark.parametrize("X,Y,perx,pery", GRIDS)
+def test_forward_matches_reference(S, X, Y, perx, pery):
+    theta = R.init_params(11)
+    f, _ = rand_case(1, X, Y)
+    net = S.Net(X, Y, perx, pery, max_batch=2)
+    net.set_params(theta)
+    got = from_gpu(net.forward(to_gpu(f)), X, Y)
+    want, _ = R.forward(theta, f.astype(np.float32), perx, pery, bf=True)
+    want = want.numpy()
+    assert np.max(np.abs(got - want)) <= 5e-3 * np.max(np.abs(want))
+
+
+@pytest.mark.parametrize("X,Y,perx,pery", GRIDS)
+def test_loss_and_gradient_match_reference(S, X, Y, perx, pery):
+    theta = R.init_params(12)
+    f, t = rand_case(2, X, Y)
+    net = S.Net(X, Y, perx, pery, max_batch=2)
+    net.set_params(theta)
+    loss, g = net.loss_grad(to_gpu(f), to_gpu(t))
+    lref, gref, _ = R.loss_and_grad(theta, f.astype(np.float32), t.astype(np.float32), perx, pery, bf=True)
+    assert abs(loss - lref) <= 1e-3 * lref
+    assert np.linalg.norm(g - gref) <= 2e-2 * np.linalg.norm(gref)
+    # per layer, so that no layer's gradient hides behind another's size
+    o = 0
+    for co, _, ci in R.layer_shapes():
+        k = co * 9 * ci + co
+        assert np.linalg.norm(g[o:o + k] - gref[o:o + k]) <= 3e-2 * np.linalg.norm(gref[o:o + k]), (co, ci)
+        o += k
+
+
+def test_adam_step_matches_reference(S):
+    theta = R.init_params(13)
+    net = S.Net(6, 5, True, False)
+    net.set_params(theta)
+    rng = np.random.default_rng(3)
+    m = np.zeros_like(theta)
+    v = np.zeros_like(theta)
+    th = theta.copy()
+    for step in (1, 2, 3):
+        g = rng.normal(size=theta.size)
+        net.adam_step(1e-3, g)
+        th, m, v = R.adam(th, m, v, g, step, 1e-3)
+    assert np.max(np.abs(net.get_params() - th)) <= 1e-12
+
+
+def test_training_fits_a_teacher(S):
+    """Targets from a teacher network; the student (other initial weights)
+    trained by the GPU's own kernels lowers the loss many times over."""
+    X, Y = 16, 12
+    rng = np.random.default_rng(5)
+    feats = rng.normal(size=(16, 4, X, Y)) * 0.5
+    teacher = R.init_params(21) * 0.5
+    tgt = R.forward(teacher, feats.astype(np.float32), True, False, bf=False)[0].numpy()
+    net = S.Net(X, Y, True, False, max_batch=4)
+    net.set_params(R.init_params(22))
+    losses = net.train(to_gpu(feats), to_gpu(tgt), batch=4, iters=300, lr=2e-3, seed=7)
+    assert np.all(np.isfinite(losses))
+    assert losses[-20:].mean() < 0.2 * losses[:5].mean()
