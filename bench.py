@@ -38,10 +38,31 @@ HBM_PEAK_GBS = 8000.0
 # algorithmic FP64 bytes per particle of each kernel class (launch covers all particles)
 ALG_BYTES = {"stream_bc": 64, "collide": 48, "zone_kernel": 48, "kick": 48}
 STEP_ALG_BYTES = 256  # compulsory particle traffic of one plain step
+# kernel of each timed class, as rocprofv3 names it
+KERNEL_NAME = {"stream_bc": "k_stream_bc", "collide": "k_collide", "zone_kernel": "k_zone<unsigned short, true>",
+               "kick": "k_kick_reorder"}
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
+
+
+def pmc_traffic(kernel: str, workload: str):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 counter
+    summary (FETCH_SIZE, WRITE_SIZE per dispatch, each scaled by its copy
+    calibration factor), or None when there is none for this workload."""
+    path = os.path.join(ROOT, "profiles", f"traffic_{workload}.md")
+    if not os.path.exists(path):
+        return None
+    with open(path) as fh:
+        for line in fh:
+            cells = [c.strip() for c in line.strip().strip("|").split("|")]
+            if len(cells) >= 6 and cells[0] == kernel:
+                try:
+                    return float(cells[5])
+                except ValueError:
+                    return None
+    return None
 
 
 def cpu_baseline(nx: int, ny: int, ppc: int, steps: int):
@@ -86,6 +107,67 @@ def cpu_baseline(nx: int, ny: int, ppc: int, steps: int):
     }
 
 
+def learned_closure(sim, p, s0, args):
+    """Trains the closure surrogate on the GPU from samples of the running
+    plain simulation, then times the same step with the surrogate (plain
+    closure every K-th step and on a failed residual check)."""
+    from src_incompressible_openmp_amd import surrogate
+
+    log(f"bench: closure surrogate: sampling {args.train_samples} plain closures")
+    sim.closure_sample(True, 1.0)
+    feats, tgts = [], []
+    s = s0
+    for _ in range(args.train_samples):
+        sim.step(s, 2)
+        s += 2
+        f, t = sim.closure_sample(True, 1.0, fetch=True)
+        feats.append(f)
+        tgts.append(t)
+    sim.closure_sample(False)
+    feats = np.stack(feats)
+    tgts = np.stack(tgts)
+    scale = float(np.std(tgts)) or 1.0
+    tgts /= scale
+    net = surrogate.Net(p.Nx, p.Ny, True, False, max_batch=2)
+    rng = np.random.default_rng(1)
+    theta = []
+    for co, ci in ((32, 4), (32, 32), (32, 32), (2, 32)):
+        b = np.sqrt(6.0 / (9 * ci))
+        theta += [rng.uniform(-b, b, co * 9 * ci), np.zeros(co)]
+    net.set_params(np.concatenate(theta))
+    ntr = max(1, len(feats) - 4)
+    log(f"bench: training {args.train_iters} Adam steps on {ntr} samples")
+    t0 = time.perf_counter()
+    losses = net.train(feats[:ntr], tgts[:ntr], batch=2, iters=args.train_iters, lr=2e-3, seed=1)
+    gpu.device_synchronize()
+    t_train = time.perf_counter() - t0
+    held = feats[ntr:]
+    r2 = None
+    if len(held):
+        pred = net.forward(held)
+        r2 = float(1.0 - np.mean((pred - tgts[ntr:]) ** 2) / np.mean(tgts[ntr:] ** 2))
+    sim.set_surrogate(net, args.learned_K, scale, 50.0 * scale)
+    sim.step(s, args.warmup)
+    s += args.warmup
+    gpu.device_synchronize()
+    log(f"bench: timing {args.steps} steps with the surrogate")
+    c0 = sim.surrogate_counts()
+    t0 = time.perf_counter()
+    sim.step_async(s, args.steps)
+    sim.synchronize()
+    gpu.device_synchronize()
+    el = time.perf_counter() - t0
+    sim.check()
+    c1 = sim.surrogate_counts()
+    sim.set_surrogate(None, 0, 1.0, 1.0)
+    net.close()
+    return {"value": p.n * args.steps / el, "ms_per_step": 1e3 * el / args.steps, "K": args.learned_K,
+            "steps_learned": c1["learned"] - c0["learned"], "steps_plain": c1["plain"] - c0["plain"],
+            "steps_fallback": c1["fallback"] - c0["fallback"], "train_samples": int(ntr),
+            "train_iters": args.train_iters, "train_s": t_train, "train_loss_first": float(losses[0]),
+            "train_loss_last": float(losses[-1]), "heldout_r2": r2, "out_scale": scale}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
@@ -99,7 +181,25 @@ def main() -> int:
     ap.add_argument("--cpu-steps", type=int, default=5)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-profile", action="store_true", help="time without per-kernel events")
+    ap.add_argument("--no-learned", action="store_true", help="skip the closure-surrogate measurement")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only copy 83,886,080 bytes device to device 5 times (the counter calibration of the "
+                         "traffic figures) and exit")
+    ap.add_argument("--learned-K", type=int, default=50, help="plain closure every K-th step with the surrogate")
+    ap.add_argument("--train-samples", type=int, default=24)
+    ap.add_argument("--train-iters", type=int, default=150)
     args = ap.parse_args()
+
+    if args.copy_calibration:
+        import torch
+
+        a = torch.ones(10485760, dtype=torch.float64, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(5):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        log("bench: copy calibration done (5 x 83,886,080 bytes)")
+        return 0
 
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -129,7 +229,10 @@ def main() -> int:
     elapsed = t1 - t0
     prof = None if args.no_profile else sim.profile_read()
     sim.profile(False)
-    m = sim.step(1 + args.warmup + args.steps, 1, metrics=True)
+    s_next = 1 + args.warmup + args.steps
+    m = sim.step(s_next, 1, metrics=True)
+    s_next += 1
+    learned = None if args.no_learned else learned_closure(sim, p, s_next, args)
     sim.close()
 
     value = n * args.steps / elapsed
@@ -151,9 +254,13 @@ def main() -> int:
         dur = per[dom]["ms_per_launch"] * 1e-3
         alg = ALG_BYTES[dom] * n
         ach = alg / dur / 1e9
+        traffic = pmc_traffic(KERNEL_NAME[dom], out["config"]["workload"])
         out["roofline"] = {"bound": "hbm", "kernel": dom, "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                           "frac": ach / HBM_PEAK_GBS, "traffic": None,
+                           "frac": ach / HBM_PEAK_GBS, "traffic": traffic,
                            "alg_bytes_per_launch": alg, "ms_per_launch": per[dom]["ms_per_launch"]}
+    if learned:
+        out["learned"] = learned
+        out["learned"]["speedup_vs_plain"] = learned["value"] / value
     out["last_step"] = {"moved_dense": m["base"]["nParticlesMovedDense"] + m["shifted"]["nParticlesMovedDense"],
                         "moved_sparse": m["base"]["nParticlesMovedSparse"] + m["shifted"]["nParticlesMovedSparse"],
                         "beta_applied": m["closure"]["betaRepairApplied"]}

@@ -262,7 +262,8 @@ __global__ void __launch_bounds__(TPB) k_scatter(const int* __restrict__ cid, co
     if (c >= 0) key[s0 + base + rank] = ((unsigned long long)p << 32) | (unsigned)i;
 }
 
-/* Order each cell's run by pid and write the slots.  A block takes
+/* Order each cell's run by pid; writes the sorted keys back and the slots
+ * to the list.  A block takes
  * SEG_CELLS consecutive cells, whose runs are one contiguous range of the
  * keys, stages the range in LDS and insertion-sorts each run (runs are short
  * and nearly sorted); a range too large for LDS is sorted in place in
@@ -292,7 +293,11 @@ __global__ void __launch_bounds__(SEG_CELLS) k_segsort(const int* __restrict__ s
             }
         }
         __syncthreads();
-        for (int q = b0 + (int)threadIdx.x; q < b1; q += SEG_CELLS) list[q] = (int)(unsigned)(sk[q - b0] & 0xffffffffull);
+        for (int q = b0 + (int)threadIdx.x; q < b1; q += SEG_CELLS) {
+            const unsigned long long kv = sk[q - b0];
+            key[q] = kv; /* the sorted keys: slot and pid of each position in one load */
+            list[q] = (int)(unsigned)(kv & 0xffffffffull);
+        }
         return;
     }
     if (c >= c1) return;
@@ -1180,7 +1185,7 @@ struct mpcd_gpu_sim {
         a.in.gcnt = bin.cnt;
         a.in.gstart = bin.start;
         a.in.glist = bin.list;
-        a.in.gpid = pid;
+        a.in.gkey = bin.key;
         a.in.owner = single ? nullptr : ownerDev;
         a.in.gcover = single ? nullptr : coverDev;
         a.in.x = x;

@@ -164,7 +164,7 @@ struct ZoneIn {
     const int* gcnt;    /* pass-input binning: counts, run starts, slots */
     const int* gstart;
     const int* glist;
-    const int* gpid;    /* particle id of each slot */
+    const unsigned long long* gkey; /* (pid << 32 | slot) of each binning position */
     const int* owner;   /* owner map of the layout, or nullptr: each zone owns its cells */
     const unsigned char* gcover; /* per cell: several zones cover it, or nullptr */
     double *x, *y, *vx, *vy;
@@ -187,7 +187,8 @@ struct ZoneCtxT {
     int zid, P, nL;
     int allOwned; /* copied from the shared set-up result */
     uint64_t seed;
-    const int *gcnt, *gstart, *glist, *gpid;
+    const int *gcnt, *gstart, *glist;
+    const unsigned long long* gkey;
     double *x, *y, *vx, *vy, *vsx, *vsy;
     IT* nxL;     /* links of k < nL */
     IT* nxG;     /* links of k >= nL (zone's global region, indexed by k) */
@@ -412,6 +413,13 @@ template <class Z>
 ZHD int slot_in(const Z& z, int k, int col) {
     return z.glist[z.colG[col] + (k - z.colL[col])];
 }
+/* slot and particle id of the zone's k-th particle in one load */
+template <class Z>
+ZHD int slot_pid_in(const Z& z, int k, int col, uint32_t& pid) {
+    const unsigned long long key = z.gkey[z.colG[col] + (k - z.colL[col])];
+    pid = (uint32_t)(key >> 32);
+    return (int)(uint32_t)key;
+}
 template <class Z>
 ZHD int slot_of(const Z& z, int k) {
     return slot_in(z, k, col_of(z, k));
@@ -603,13 +611,85 @@ ZHD void dense_targets(Z& z, int src, int high, int pass, int phase, int lane, i
             const int q = t - 25;
             const int k = srcMat ? z.mvk[q] : h0 + q;
             const int col = col_of(z, k);
-            const int slot = slot_in(z, k, col);
+            uint32_t pid;
+            const int slot = slot_pid_in(z, k, col, pid);
             z.mvs[q] = slot;
             z.mvo[q] = own_bits(z, k, col);
-            mpcd_draw_dense_pos(z.seed, pass, (uint32_t)z.gpid[slot], &z.mvx[q], &z.mvy[q]);
+            mpcd_draw_dense_pos(z.seed, pass, pid, &z.mvx[q], &z.mvy[q]);
         }
     }
     ZBAR();
+#if defined(__HIP_DEVICE_COMPILE__)
+    {
+        /* one wavefront: neighbour offset o = lane; each candidate stays in
+         * its lane's registers and the ranking reads the others by readlane */
+        bool keep = false;
+        int gc = -1;
+        if (lane < 24) {
+            gc = z.nbGc[lane];
+            keep = gc >= 0;
+            if (z.zp.dedupe)
+                for (int q = 0; q < lane && keep; ++q)
+                    if (z.nbGc[q] == gc) keep = false;
+        }
+        const unsigned long long msk = wave_ballot(keep);
+        const int m = wave_popc(msk);
+        int lc = -1, bin = 0, cnt = 0, cheb = 0, dist2 = 0;
+        double defn = 0.0, tie = 0.0, th = 0.0;
+        if (keep) {
+            const int jx = gc / z.g.Ny, jy = gc - jx * z.g.Ny;
+            lc = zone_lc(z, jx, jy);
+            int ddx, ddy;
+            min_step(z, sgx, sgy, jx, jy, ddx, ddy);
+            bin = zone_bin(z.zp, (ddy + 2) * 5 + (ddx + 2));
+            th = thr(z, lc, high);
+            cnt = z.cnt[lc];
+            defn = -dmax(0.0, th - (double)cnt);
+            cheb = imax(ddx < 0 ? -ddx : ddx, ddy < 0 ? -ddy : ddy);
+            dist2 = ddx * ddx + ddy * ddy;
+            tie = z.nbTie[lane];
+        }
+        if (lane == 0) z.sh->nOrd = m;
+        if (m == 0) {
+            ZBAR();
+            return;
+        }
+        Cand me;
+        me.deficitNeg = defn;
+        me.cnt = cnt;
+        me.cheb = cheb;
+        me.dist2 = dist2;
+        me.tie = tie;
+        /* rank inside the bin: stable order of the candidate key */
+        int r = 0;
+        for (unsigned long long mm = msk; mm; mm &= mm - 1ull) {
+            const int j = __builtin_ctzll(mm);
+            Cand o;
+            o.deficitNeg = wave_readlane_d(defn, j);
+            o.cnt = wave_readlane(cnt, j);
+            o.cheb = wave_readlane(cheb, j);
+            o.dist2 = wave_readlane(dist2, j);
+            o.tie = wave_readlane_d(tie, j);
+            const int bj = wave_readlane(bin, j);
+            if (keep && j != lane && bj == bin && (cand_less(o, me) || (j < lane && !cand_less(me, o)))) ++r;
+        }
+        /* round robin position: earlier rounds, then earlier bins from startBin */
+        const int sb = z.sh->startBin;
+        const int si = (bin - sb + 8) & 7;
+        int pos = 0;
+        for (unsigned long long mm = msk; mm; mm &= mm - 1ull) {
+            const int j = __builtin_ctzll(mm);
+            const int rj = wave_readlane(r, j);
+            const int sj = (wave_readlane(bin, j) - sb + 8) & 7;
+            if (rj < r || (rj == r && sj < si)) ++pos;
+        }
+        if (keep) {
+            z.ord[pos] = lc;
+            z.ordThr[pos] = th;
+        }
+        ZBAR();
+    }
+#else
     int base = 0;
     for (int b0 = 0; b0 < 24; b0 += nl) {
         const int o = b0 + lane;
@@ -670,6 +750,7 @@ ZHD void dense_targets(Z& z, int src, int high, int pass, int phase, int lane, i
         z.ordThr[pos] = thr(z, lc, high);
     }
     ZBAR();
+#endif
 }
 
 /* spread_particles_from_dense_source_round_robin (zone_pass.cpp:527-589),
@@ -715,9 +796,10 @@ ZHD int spread(Z& z, int src, int high, int needOut, int pass, int nStaged, int*
                     uy = z.mvy[used];
                 } else {
                     const int col = col_of(z, k);
-                    slot = slot_in(z, k, col);
+                    uint32_t pid;
+                    slot = slot_pid_in(z, k, col, pid);
                     own = own_bits(z, k, col);
-                    mpcd_draw_dense_pos(z.seed, pass, (uint32_t)z.gpid[slot], &ux, &uy);
+                    mpcd_draw_dense_pos(z.seed, pass, pid, &ux, &uy);
                 }
                 move_to(z, nb, slot, own, ux, uy);
                 ++used;
@@ -850,9 +932,10 @@ ZHD void donor_transfer(Z& z, int c, int d, int md, int nt, int cgc, T* tmp, int
         const int kq = (int)tmp[q];
         if (q < nt) {
             const int col = col_of(z, kq);
-            const int slot = slot_in(z, kq, col);
+            uint32_t pid;
+            const int slot = slot_pid_in(z, kq, col, pid);
             double ux, uy;
-            mpcd_draw_sparse_pos(z.seed, (uint32_t)z.gpid[slot], &ux, &uy);
+            mpcd_draw_sparse_pos(z.seed, pid, &ux, &uy);
             move_to(z, c, slot, own_bits(z, kq, col), ux, uy);
             nx_set(z, kq, q + 1 < nt ? (int)tmp[q + 1] : -1);
         } else {
@@ -1256,7 +1339,7 @@ ZHD void zone_init(ZoneCtxT<IT>& z, const ZoneIn& in, ZoneRect r, int zid, uint6
     z.gcnt = in.gcnt;
     z.gstart = in.gstart;
     z.glist = in.glist;
-    z.gpid = in.gpid;
+    z.gkey = in.gkey;
     z.x = in.x;
     z.y = in.y;
     z.vx = in.vx;

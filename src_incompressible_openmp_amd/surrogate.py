@@ -48,6 +48,7 @@ class Net:
 
     def __init__(self, Nx: int, Ny: int, perx: bool, pery: bool, max_batch: int = 1):
         self.Nx, self.Ny, self.Nc = Nx, Ny, Nx * Ny
+        self.max_batch = max_batch
         self._h = ctypes.c_void_p()
         gpu._check(_lib().mpcd_gpu_net_create(Nx, Ny, int(perx), int(pery), max_batch, ctypes.byref(self._h)))
 
@@ -74,9 +75,15 @@ class Net:
         return t
 
     def forward(self, feat: np.ndarray) -> np.ndarray:
+        """outputs of any number of samples, max_batch at a time"""
         f = _f32(feat, 4).reshape(-1, self.Nc, 4)
         out = np.zeros((f.shape[0], self.Nc, 2), np.float32)
-        gpu._check(_lib().mpcd_gpu_net_forward(self._h, f.ctypes.data_as(_fp), f.shape[0], out.ctypes.data_as(_fp)))
+        for b0 in range(0, f.shape[0], self.max_batch):
+            fb = np.ascontiguousarray(f[b0:b0 + self.max_batch])
+            ob = np.zeros((fb.shape[0], self.Nc, 2), np.float32)
+            gpu._check(_lib().mpcd_gpu_net_forward(self._h, fb.ctypes.data_as(_fp), fb.shape[0],
+                                                   ob.ctypes.data_as(_fp)))
+            out[b0:b0 + fb.shape[0]] = ob
         return out
 
     def loss_grad(self, feat: np.ndarray, target: np.ndarray):

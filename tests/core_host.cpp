@@ -147,7 +147,10 @@ extern "C" int core_zone_pass(const mpcd_params* p, double* xa, double* va, int 
     in.gcnt = b.cnt.data();
     in.gstart = b.start.data();
     in.glist = b.list.data();
-    in.gpid = pid.data();
+    std::vector<unsigned long long> key(b.list.size());
+    for (size_t j = 0; j < key.size(); ++j)
+        key[j] = ((unsigned long long)(uint32_t)pid[(size_t)b.list[j]] << 32) | (uint32_t)b.list[j];
+    in.gkey = key.data();
     in.owner = owner.data();
     in.gcover = anyOver ? over.data() : nullptr;
     in.x = x.data();

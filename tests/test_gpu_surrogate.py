@@ -1,7 +1,7 @@
 """Closure surrogate kernels against the CPU reference (tests/surrogate_ref.py,
 which rounds to bfloat16 at the same points as the GPU and sums in float64):
-the matrix-core lane layout, forward, loss and gradient, the Adam step, and
-that training on the GPU fits a teacher network."""
+the matrix-core lane layout, forward, loss and gradient, the Adam step,
+training on the GPU, and the surrogate in the resident step."""
 import numpy as np
 import pytest
 import torch
@@ -134,3 +134,64 @@ def test_training_fits_a_teacher(S):
     losses = net.train(to_gpu(feats), to_gpu(tgt), batch=4, iters=300, lr=2e-3, seed=7)
     assert np.all(np.isfinite(losses))
     assert losses[-20:].mean() < 0.2 * losses[:5].mean()
+
+
+def closure_run_params(Nx=48, Ny=24, gamma=10.0):
+    from src_incompressible_openmp_amd.params import poiseuille_params, to_struct
+
+    return to_struct(poiseuille_params(Nx=Nx, Ny=Ny, gamma=gamma, zoneTileNx=12, zoneTileNy=12))
+
+
+def test_surrogate_learns_the_closure_and_steps(S):
+    """Samples from a plain run train the network on the GPU; on held-out
+    samples it explains most of the closure's correction (what the network
+    gains over no correction); a run that uses it keeps the particle count,
+    kinetic energy and flow of the plain run."""
+    from src_incompressible_openmp_amd import gpu
+    from src_incompressible_openmp_amd.fileio import uniform_state
+
+    p = closure_run_params()
+    x, v, _, _ = uniform_state(p.n, p.Lx, p.Ly, seed=3)
+    sim = gpu.Sim(p)
+    sim.set_state(x, v)
+    sim.step(1, 40)
+    sim.closure_sample(True, 1.0)
+    feats, tgts = [], []
+    for s in range(41, 161):
+        sim.step(s)
+        f, t = sim.closure_sample(True, 1.0, fetch=True)
+        feats.append(f)
+        tgts.append(t)
+    sim.closure_sample(False)
+    feats = np.stack(feats)
+    tgts = np.stack(tgts)
+    scale = float(np.std(tgts))
+    tgts = tgts / scale
+    train, test = slice(0, 100), slice(100, 120)
+    net = S.Net(p.Nx, p.Ny, True, False, max_batch=8)
+    net.set_params(R.init_params(1))
+    losses = net.train(feats[train], tgts[train], batch=8, iters=400, lr=2e-3, seed=3)
+    assert np.all(np.isfinite(losses))
+    pred = net.forward(feats[test])
+    resid = np.mean((pred - tgts[test]) ** 2)
+    base = np.mean(tgts[test] ** 2)
+    print(f"surrogate: train loss {losses[:5].mean():.3g} -> {losses[-20:].mean():.3g}, held-out R2 {1 - resid / base:.3f}")
+    assert resid < 0.5 * base
+    # the resident loop with the surrogate (plain closure every 10th step)
+    xs, vs = sim.get_state()
+    ref = gpu.Sim(p)
+    ref.set_state(xs, vs)
+    ref.step(161, 60)
+    xr, vr = ref.get_state()
+    lrn = gpu.Sim(p)
+    lrn.set_state(xs, vs)
+    lrn.set_surrogate(net, 10, scale, 50.0 * scale)
+    lrn.step(161, 60)
+    xl, vl = lrn.get_state()
+    c = lrn.surrogate_counts()
+    assert c["learned"] + c["fallback"] == 54 and c["plain"] == 6
+    assert xl.shape == xr.shape and np.all(np.isfinite(xl)) and np.all(np.isfinite(vl))
+    ke_r, ke_l = 0.5 * np.mean(vr ** 2) * 2, 0.5 * np.mean(vl ** 2) * 2
+    assert abs(ke_l - ke_r) <= 0.05 * ke_r
+    # mean flow: the two runs differ by thermal noise of the wall exchanges
+    assert abs(np.mean(vl[0::2]) - np.mean(vr[0::2])) <= 0.03
