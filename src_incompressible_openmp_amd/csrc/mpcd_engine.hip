@@ -746,19 +746,52 @@ __global__ void k_closure_target(int Nc, const double* __restrict__ scal, const 
 
 /* the surrogate's correction as the kick's grids (duEOS = prediction,
  * duRepair = 0, beta = 0); flag = 1 when a value is not finite or exceeds
- * the cap (the residual check that sends the step to the plain closure) */
-__global__ void k_learned_du(int Nc, const float* __restrict__ pred, double scale, double cap,
-                             double* __restrict__ duRX, double* __restrict__ duRY, double* __restrict__ duEX,
-                             double* __restrict__ duEY, double* __restrict__ scal, int* __restrict__ flag) {
+ * the cap (the residual check that sends the step to the plain closure);
+ * per-block partials of the momentum it would add, sum of N du */
+__global__ void __launch_bounds__(TPB) k_learned_du(int Nc, const float* __restrict__ pred, const int* __restrict__ N,
+                                                    double scale, double cap, double* __restrict__ duRX,
+                                                    double* __restrict__ duRY, double* __restrict__ duEX,
+                                                    double* __restrict__ duEY, double* __restrict__ scal,
+                                                    int* __restrict__ flag, double* __restrict__ part) {
+    __shared__ double sx[TPB], sy[TPB];
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c == 0) scal[3] = 0.0;
+    double mx = 0.0, my = 0.0;
+    if (c < Nc) {
+        const double ux = (double)pred[2 * (size_t)c] * scale, uy = (double)pred[2 * (size_t)c + 1] * scale;
+        if (!(fabs(ux) <= cap) || !(fabs(uy) <= cap)) atomicOr(flag, 1);
+        duRX[c] = 0.0;
+        duRY[c] = 0.0;
+        duEX[c] = ux;
+        duEY[c] = uy;
+        mx = (double)N[c] * ux;
+        my = (double)N[c] * uy;
+    }
+    sx[threadIdx.x] = mx;
+    sy[threadIdx.x] = my;
+    __syncthreads();
+    for (int st = TPB / 2; st > 0; st >>= 1) {
+        if ((int)threadIdx.x < st) {
+            sx[threadIdx.x] += sx[threadIdx.x + st];
+            sy[threadIdx.x] += sy[threadIdx.x + st];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        part[2 * (size_t)blockIdx.x] = sx[0];
+        part[2 * (size_t)blockIdx.x + 1] = sy[0];
+    }
+}
+
+/* removes the predicted correction's net momentum: du -= (sum N du) / n,
+ * so the surrogate's kick, like a closure kick, leaves the total momentum
+ * to the physics (sums in sums[0..1]) */
+__global__ void k_learned_center(int Nc, const double* __restrict__ sums, double invN, double* __restrict__ duEX,
+                                 double* __restrict__ duEY) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= Nc) return;
-    const double ux = (double)pred[2 * (size_t)c] * scale, uy = (double)pred[2 * (size_t)c + 1] * scale;
-    if (!(fabs(ux) <= cap) || !(fabs(uy) <= cap)) atomicOr(flag, 1);
-    duRX[c] = 0.0;
-    duRY[c] = 0.0;
-    duEX[c] = ux;
-    duEY[c] = uy;
+    duEX[c] -= sums[0] * invN;
+    duEY[c] -= sums[1] * invN;
 }
 
 __global__ void __launch_bounds__(TPB) k_closure_cell_metrics(int Nc, const double* __restrict__ duRX,
@@ -1286,8 +1319,11 @@ struct mpcd_gpu_sim {
                            featBuf);
         const float* pred = net_forward_device(sur, featBuf, stream);
         hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, stream, surFlag, 1, 0);
-        hipLaunchKernelGGL(k_learned_du, dim3(nbc), dim3(TPB), 0, stream, Nc, pred, surScale, surCap, duRX, duRY, duEX,
-                           duEY, scal, surFlag);
+        hipLaunchKernelGGL(k_learned_du, dim3(nbc), dim3(TPB), 0, stream, Nc, pred, redN, surScale, surCap, duRX, duRY,
+                           duEX, duEY, scal, surFlag, part);
+        hipLaunchKernelGGL(k_reduce_parts, dim3(1), dim3(TPB), 0, stream, part, nbc, 2, scal + 8);
+        hipLaunchKernelGGL(k_learned_center, dim3(nbc), dim3(TPB), 0, stream, Nc, scal + 8, 1.0 / (double)n, duEX,
+                           duEY);
         check_launch();
         int bad = 0;
         HIP_OK(hipMemcpyAsync(&bad, surFlag, sizeof(int), hipMemcpyDeviceToHost, stream));

@@ -1,5 +1,5 @@
 """Drop-in check: the reference's own drivers, relinked against this
-library's mpcd:: API (tests/dropin/Makefile), run on the GPU and agree with
+library's mpcd:: API (oracle/dropin.mk), run on the GPU and agree with
 the oracle's counter mode on the same inputs.  Same bar as the C-ABI parity
 tests: positions bitwise, velocities through the closure's reduced beta to
 1e-12 (1e-9 after several steps)."""
@@ -17,13 +17,13 @@ from tests.oracle_lib import CTR
 pytestmark = pytest.mark.gpu
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-BIN = os.path.join(ROOT, "tests", "_build", "dropin")
+BIN = os.path.join(ROOT, "oracle", "_ref", "dropin")
 
 
 def exe(name):
     path = os.path.join(BIN, name)
     if not os.path.exists(path):
-        pytest.skip(f"{name} not built (tests/dropin/Makefile needs the reference sources)")
+        pytest.skip(f"{name} not built (oracle/dropin.mk needs the reference sources)")
     return path
 
 
