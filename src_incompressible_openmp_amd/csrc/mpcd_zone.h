@@ -142,6 +142,7 @@ struct ZoneOut {
 #define MPCD_ZONE_TMP_L 256
 #endif
 constexpr int ZONE_MAX_MOVE = MPCD_ZONE_MAX_MOVE; /* dense movers staged per source cell */
+static_assert(ZONE_MAX_MOVE <= 64, "a dense cell's movers are the lanes of one wavefront");
 constexpr int ZONE_TMP_L = MPCD_ZONE_TMP_L;       /* donor lists up to this length are shuffled in LDS */
 
 /* per-cell flag bits */
@@ -822,9 +823,321 @@ ZHD int spread(Z& z, int src, int high, int needOut, int pass, int nStaged, int*
     return moved;
 }
 
+#if defined(__HIP_DEVICE_COMPILE__)
+/* ---- dense cells on the device: candidates and movers held in lanes ----
+ *
+ * Lane p holds candidate p of a ranking and lane q the q-th particle of the
+ * source's list (a mover); everything the spread decides is computed by all
+ * lanes from readlane values, so the wavefront follows one uniform control
+ * path, and the moves and list links are written by the lanes at once.  The
+ * results are those of dense_targets + spread + transfer_masks. */
+
+/* compiler-level ordering of LDS accesses between lanes of the one
+ * wavefront (the LDS executes a wavefront's accesses in issue order) */
+#define ZWSYNC()                                                    \
+    do {                                                            \
+        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");      \
+        __builtin_amdgcn_wave_barrier();                            \
+    } while (0)
+
+/* value held by lane src (per-lane src) */
+ZHD int lane_get(int v, int src) { return __shfl(v, src, 64); }
+ZHD double lane_get_d(double v, int src) { return __shfl(v, src, 64); }
+ZHD unsigned long long lane_get_u64(unsigned long long v, int src) {
+    const unsigned lo = (unsigned)__shfl((int)(unsigned)v, src, 64);
+    const unsigned hi = (unsigned)__shfl((int)(unsigned)(v >> 32), src, 64);
+    return ((unsigned long long)hi << 32) | lo;
+}
+/* every lane's v to lane dst (dst a permutation of the lanes) */
+ZHD int lane_put(int v, int dst) { return __builtin_amdgcn_ds_permute(dst << 2, v); }
+ZHD double lane_put_d(double v, int dst) {
+    const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+    const unsigned lo = (unsigned)lane_put((int)(unsigned)b, dst);
+    const unsigned hi = (unsigned)lane_put((int)(unsigned)(b >> 32), dst);
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+/* The ranking of dense_targets (zone_pass.cpp:413-490).  Candidate o (lane
+ * o < 24) is neighbour offset o; its key (deficit, count, Chebyshev and
+ * squared distance, tie draw) is compared only with the other members of its
+ * angular bin, fetched by lane permutes; the round-robin position follows
+ * from the rank in the bin and the bins' sizes.  On return lane p < nOrd
+ * (the return value, same on all lanes) holds the p-th target: zone cell,
+ * threshold and count. */
+template <class Z>
+ZHD int dense_rank_wave(const Z& z, int sgx, int sgy, int sgc, int high, int pass, int phase, int lane, int& lcP,
+                        double& thP, int& cP) {
+    int gc = -1, lc = -1, cnt = 0, ddx = 0, ddy = 0;
+    double th = 0.0;
+    if (lane < 24) {
+        const int f = lane < 12 ? lane : lane + 1;
+        const int dy = f / 5 - 2, dx = f % 5 - 2;
+        int jx, jy;
+        if (shift_idx(z, sgx, sgy, dx, dy, jx, jy)) {
+            const int l = zone_lc(z, jx, jy);
+            if (l >= 0) {
+                const int c = (int)z.cnt[l];
+                const double t = thr(z, l, high);
+                if ((double)c < t) {
+                    gc = jy + z.g.Ny * jx;
+                    lc = l;
+                    cnt = c;
+                    th = t;
+                    min_step(z, sgx, sgy, jx, jy, ddx, ddy);
+                }
+            }
+        }
+    }
+    bool keep = gc >= 0;
+    if (z.zp.dedupe)
+        for (int q = 0; q < 23; ++q) {
+            const int gq = wave_readlane(gc, q);
+            if (q < lane && gq == gc) keep = false;
+        }
+    const unsigned long long msk = wave_ballot(keep);
+    const int m = wave_popc(msk);
+    lcP = -1;
+    cP = 0;
+    thP = 0.0;
+    if (m == 0) return 0;
+    /* one Philox block per lane: tie break (lanes < 24), start bin (lane 24),
+     * the counters of mpcd_draw_tie and mpcd_draw_start_bin */
+    const mpcd_u32x4 rb = mpcd_philox(z.seed,
+                                      mpcd_ctr0(lane < 24 ? MPCD_TAG_RTIE : MPCD_TAG_RBIN, (uint32_t)((pass << 1) | phase)),
+                                      (uint32_t)sgc, lane < 24 ? (uint32_t)gc : 0u, 0u);
+    const double tie = mpcd_u01(rb.w[0], rb.w[1]);
+    const int sb = wave_readlane((int)mpcd_uint_below(rb.w[0], 8u), 24);
+    const int bin = keep ? zone_bin(z.zp, (ddy + 2) * 5 + (ddx + 2)) : 8;
+    const double defn = -dmax(0.0, th - (double)cnt);
+    const int cheb = imax(ddx < 0 ? -ddx : ddx, ddy < 0 ? -ddy : ddy);
+    /* (count, Chebyshev, squared distance) in one ordered word: cheb <= 2, dist2 <= 8 */
+    const unsigned long long sm =
+        ((unsigned long long)(unsigned)cnt << 8) | (unsigned long long)((cheb << 4) | (ddx * ddx + ddy * ddy));
+    unsigned long long bm[8];
+    unsigned long long mine = 0ull;
+    int smax = 0;
+#pragma unroll
+    for (int b = 0; b < 8; ++b) {
+        bm[b] = wave_ballot(bin == b);
+        smax = imax(smax, wave_popc(bm[b]));
+        if (bin == b) mine = bm[b];
+    }
+    /* rank inside the bin: the stable order of the candidate key */
+    unsigned long long mm = mine & ~(1ull << lane);
+    int r = 0;
+    for (int s = 1; s < smax; ++s) {
+        const bool have = mm != 0ull;
+        const int j = have ? (int)__builtin_ctzll(mm) : lane;
+        if (have) mm &= mm - 1ull;
+        const double od = lane_get_d(defn, j);
+        const unsigned long long osm = lane_get_u64(sm, j);
+        const double ot = lane_get_d(tie, j);
+        if (have) {
+            const bool oLess = od < defn || (od == defn && (osm < sm || (osm == sm && ot < tie)));
+            const bool meLess = defn < od || (defn == od && (sm < osm || (sm == osm && tie < ot)));
+            if (oLess || (j < lane && !meLess)) ++r;
+        }
+    }
+    /* round-robin position: every bin's members of earlier rounds, then this
+     * round's members of the bins before this one from the start bin */
+    const int si = (bin - sb + 8) & 7;
+    int pos = 0;
+#pragma unroll
+    for (int b = 0; b < 8; ++b) {
+        const int nb = wave_popc(bm[b]);
+        pos += imin(nb, r) + ((((b - sb + 8) & 7) < si && nb > r) ? 1 : 0);
+    }
+    const int dst = keep ? pos : m + wave_popc(~msk & lanes_below(lane));
+    lcP = lane_put(lc, dst);
+    cP = lane_put(cnt, dst);
+    thP = lane_put_d(th, dst);
+    return m;
+}
+
+/* One sweep series of spread (zone_pass.cpp:527-589) over the ranking in the
+ * lanes: assigns movers used.. to candidates (myT), chains the movers of one
+ * candidate in order (nextQ), and keeps each candidate's first and last mover
+ * and new count.  All values uniform except the per-lane ones. */
+ZHD int spread_plan_wave(int nOrd, double thP, int& cP, int need, int& avail, int& used, int cap, int lane,
+                         int& myT, int& nextQ, int& firstQ, int& lastQ) {
+    int moved = 0;
+    myT = -1;
+    firstQ = -1;
+    lastQ = -1;
+    if (need <= 0 || nOrd == 0 || avail <= 0) return 0;
+    while (need > 0 && avail > 0) {
+        const int defP = lane < nOrd ? imax(0, (int)MPCD_FLOOR(thP - (double)cP)) : 0;
+        unsigned long long mk = wave_ballot(defP > 0);
+        if (mk == 0ull) break;
+        int R = wave_popc(mk);
+        bool stop = false;
+        for (; mk; mk &= mk - 1ull) {
+            const int kk = (int)__builtin_ctzll(mk);
+            const int d = wave_readlane(defP, kk);
+            const int remaining = R--;
+            const int quota = imax(1, (int)MPCD_CEIL((double)need / (double)imax(1, remaining)));
+            /* cap (the staged movers) never binds: the callers stage
+             * enough for both phases; it only keeps lanes in bounds */
+            const int nt = imin(imax(1, imin(imin(quota, d), imin(need, avail))), cap - used);
+            if (nt <= 0) {
+                stop = true;
+                break;
+            }
+            const int prev = wave_readlane(lastQ, kk);
+            if (lane >= used && lane < used + nt) {
+                myT = kk;
+                nextQ = lane + 1 < used + nt ? lane + 1 : -1;
+            }
+            if (prev >= 0 && lane == prev) nextQ = used;
+            if (lane == kk) {
+                cP += nt;
+                if (firstQ < 0) firstQ = used;
+                lastQ = used + nt - 1;
+            }
+            used += nt;
+            avail -= nt;
+            need -= nt;
+            moved += nt;
+            if (need <= 0 || avail <= 0) {
+                stop = true;
+                break;
+            }
+        }
+        if (stop) break;
+    }
+    return moved;
+}
+
+/* Applies a plan: pops the movers from src's head, writes their new
+ * positions, appends them to their targets in mover order. */
+template <class Z>
+ZHD void spread_apply_wave(Z& z, int src, bool srcMat, int lane, int q0, int moved, int avail, int myK, int slot,
+                           unsigned short own, double ux, double uy, int myT, int nextQ, int lcP, int cP, int firstQ,
+                           int lastQ) {
+    if (moved <= 0) return;
+    /* src first: a materialised list's next head is read before the movers'
+     * links are rewritten */
+    const int kLast = wave_readlane(myK, q0 + moved - 1);
+    const int nh = srcMat ? nx_get(z, kLast) : (kLast < tail_of(z, src) ? kLast + 1 : -1);
+    const int tlc = lane_get(lcP, myT >= 0 ? myT : 0);
+    const int nk = lane_get(myK, nextQ >= 0 ? nextQ : 0);
+    const int fk = lane_get(myK, firstQ >= 0 ? firstQ : 0);
+    const int lk = lane_get(myK, lastQ >= 0 ? lastQ : 0);
+    ZWSYNC();
+    if (lane == 0) {
+        head_set(z, src, nh);
+        if (nh < 0) tail_set(z, src, -1);
+        fset(z, src, ZF_CHG);
+        cnt_set(z, src, avail);
+    }
+    if (myT >= 0) {
+        move_to(z, tlc, slot, own, ux, uy);
+        nx_set(z, myK, nextQ >= 0 ? nk : -1);
+    }
+    if (firstQ >= 0) {
+        ensure_mat(z, lcP);
+        const int t = tail_of(z, lcP);
+        if (t < 0) head_set(z, lcP, fk);
+        else nx_set(z, t, fk);
+        tail_set(z, lcP, lk);
+        fset(z, lcP, ZF_CHG);
+        cnt_set(z, lcP, cP);
+    }
+    ZWSYNC();
+}
+
+/* One dense cell (zone_pass.cpp:738-779: both target phases, the spreads and
+ * the transfer masks) when its movers fit the wavefront:
+ * min(needOut + 1, count) <= 64 (phase 1 moves at most what phase 0 left of
+ * needOut, up to one more by the rounding of its ceil).  Returns the number
+ * of particles moved. */
+template <class Z>
+ZHD int dense_cell_wave(Z& z, int src, int nc, double hl, int needOut, int pass, int lane) {
+    const int sgx = lc_gx(z, src), sgy = lc_gy(z, src), sgc = lc_gc(z, src);
+    const int nStage = imin(needOut + 1, nc);
+    const bool srcMat = fget(z, src, ZF_MAT);
+    /* movers: lane q is the q-th particle of src's list */
+    int myK = -1;
+    if (srcMat) {
+        if (lane == 0) {
+            int k = head_of(z, src);
+            for (int q = 0; q < nStage && k >= 0; ++q) {
+                z.mvk[q] = k;
+                k = nx_get(z, k);
+            }
+        }
+        ZWSYNC();
+        if (lane < nStage) myK = z.mvk[lane];
+    } else if (lane < nStage) {
+        myK = head_of(z, src) + lane;
+    }
+    /* slot, particle id and ownership; an implicit run lies in src's input
+     * run, so in src's column with src's ownership */
+    unsigned long long key = 0ull;
+    unsigned short own = 0;
+    if (lane < nStage) {
+        const int col = srcMat ? col_of(z, myK) : src / z.zh;
+        key = z.gkey[z.colG[col] + (myK - z.colL[col])];
+        own = srcMat ? own_bits(z, myK, col)
+                     : (z.allOwned ? (unsigned short)ZF_OWN : (unsigned short)(z.fl[src] & (ZF_OWN | ZF_COVER)));
+    }
+    const int slot = (int)(uint32_t)key;
+    double ux = 0.0, uy = 0.0;
+    bool drawn = false;
+    int used = 0, avail = nc, moved = 0;
+    int lcP, cP, myT, nextQ = -1, firstQ, lastQ;
+    double thP;
+    /* phase 0: low thresholds */
+    const int n0 = dense_rank_wave(z, sgx, sgy, sgc, 0, pass, 0, lane, lcP, thP, cP);
+    int q0 = used;
+    int m0 = spread_plan_wave(n0, thP, cP, needOut, avail, used, nStage, lane, myT, nextQ, firstQ, lastQ);
+    if (m0 > 0) {
+        if (lane < nStage) mpcd_draw_dense_pos(z.seed, pass, (uint32_t)(key >> 32), &ux, &uy);
+        drawn = true;
+        spread_apply_wave(z, src, srcMat, lane, q0, m0, avail, myK, slot, own, ux, uy, myT, nextQ, lcP, cP, firstQ,
+                          lastQ);
+    }
+    moved += m0;
+    const int rcv0 = firstQ >= 0 ? lcP : -1;
+    /* phase 1: high thresholds, for what is left above the high threshold */
+    int rcv1 = -1;
+    const int remaining = imax(0, (int)MPCD_CEIL((double)avail - hl));
+    if (remaining > 0) {
+        const int n1 = dense_rank_wave(z, sgx, sgy, sgc, 1, pass, 1, lane, lcP, thP, cP);
+        q0 = used;
+        nextQ = -1;
+        const int m1 = spread_plan_wave(n1, thP, cP, remaining, avail, used, nStage, lane, myT, nextQ, firstQ, lastQ);
+        if (m1 > 0) {
+            if (!drawn && lane < nStage) mpcd_draw_dense_pos(z.seed, pass, (uint32_t)(key >> 32), &ux, &uy);
+            spread_apply_wave(z, src, srcMat, lane, q0, m1, avail, myK, slot, own, ux, uy, myT, nextQ, lcP, cP,
+                              firstQ, lastQ);
+        }
+        moved += m1;
+        rcv1 = firstQ >= 0 ? lcP : -1;
+    }
+    /* transfer masks (zone_pass.cpp:492-525) over both phases' receivers */
+    const bool nonBulk = (rcv0 >= 0 && !fget(z, rcv0, ZF_BULK0)) || (rcv1 >= 0 && !fget(z, rcv1, ZF_BULK0));
+    const bool allBulk = fget(z, src, ZF_BULK0) && wave_ballot(nonBulk) == 0ull;
+    const unsigned short setb = (unsigned short)(ZF_RECV | (allBulk ? ZF_BB : ZF_SKIP));
+    const unsigned short keepb = (unsigned short)(allBulk ? 0xffff : ~ZF_BB);
+    if (rcv0 >= 0) z.fl[rcv0] = (unsigned short)((z.fl[rcv0] | setb) & keepb);
+    ZWSYNC();
+    if (rcv1 >= 0) z.fl[rcv1] = (unsigned short)((z.fl[rcv1] | setb) & keepb);
+    if (lane == 0) {
+        if (allBulk) fset(z, src, ZF_BB);
+        else z.fl[src] = (unsigned short)((z.fl[src] | ZF_SKIP) & ~ZF_BB);
+    }
+    ZWSYNC();
+    return moved;
+}
+#endif
+
 /* Dense passes (zone_pass.cpp:708-781), all lanes. */
 template <class Z>
 ZHD void zone_dense_passes(Z& z, int lane, int nl) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    int nHighW = 0, movedW = 0;
+#endif
     for (int pass = 1; pass <= z.zp.maxPasses; ++pass) {
         for (int lc = lane; lc < z.C; lc += nl) zone_maps(z, lc, 0);
         ZBAR();
@@ -838,6 +1151,13 @@ ZHD void zone_dense_passes(Z& z, int lane, int nl) {
             const double hl = high_thr(z, c);
             if ((double)nc <= hl) continue;
             const int needOut = imax(0, (int)MPCD_CEIL((double)nc - hl));
+#if defined(__HIP_DEVICE_COMPILE__)
+            if (needOut > 0 && imin(needOut + 1, nc) <= ZONE_MAX_MOVE) {
+                ++nHighW;
+                movedW += dense_cell_wave(z, c, nc, hl, needOut, pass, lane);
+                continue;
+            }
+#endif
             int moved = 0;
             if (lane == 0) {
                 ++z.sh->nHigh;
@@ -875,6 +1195,13 @@ ZHD void zone_dense_passes(Z& z, int lane, int nl) {
             ZBAR();
         }
     }
+#if defined(__HIP_DEVICE_COMPILE__)
+    if (lane == 0) {
+        z.sh->nHigh += nHighW;
+        z.sh->movedDense += movedW;
+    }
+    ZBAR();
+#endif
 }
 
 /* ---- sparse phase ---- */
