@@ -463,6 +463,7 @@ struct ZoneArgs {
     void* nextG;    /* links beyond LDS, IT per particle */
     void* tmpG;     /* long donor lists, IT per particle */
     double* duG;    /* 2 per cell per zone */
+    unsigned char* rec; /* last move of each particle, 1 per particle slot of the zone regions */
     int duStride;   /* doubles per zone */
     int pMin, pMax; /* zones with pMin <= P <= pMax run in this launch */
     size_t ldsBytes;
@@ -510,6 +511,7 @@ __global__ void __launch_bounds__(ZONE_TPB) k_zone(ZoneArgs a) {
     z.nxG = (IT*)a.nextG + off;
     z.tmpG = (IT*)a.tmpG + off;
     z.duG = a.duG + (size_t)zi * (size_t)a.duStride;
+    z.rec = a.rec + off;
     if (kLds) zone_carve(z, lds, a.ldsBytes);
     else zone_carve(z, a.gscratch + (size_t)zi * a.gstride, a.gstride);
     zone_setup(z, a.in, lane, ZONE_TPB);
@@ -886,6 +888,7 @@ struct mpcd_gpu_sim {
     ZoneOut* zout = nullptr;
     int zoutCap = 0;
     double* duG = nullptr;
+    unsigned char* recG = nullptr; /* move records, 1 byte per particle slot of the zone regions */
     size_t duCap = 0;
     unsigned char* zscratch = nullptr;
     size_t zscratchCap = 0;
@@ -1027,6 +1030,7 @@ struct mpcd_gpu_sim {
         scratchCap = (long long)coverMax * (long long)n;
         next = dalloc<int>((size_t)scratchCap + 1);
         tmpg = dalloc<int>((size_t)scratchCap + 1);
+        recG = dalloc<unsigned char>((size_t)scratchCap + 1);
         zsize = dalloc<int>((size_t)std::max(nz[0], nz[1]) + 2);
         zoff = dalloc<int>((size_t)std::max(nz[0], nz[1]) + 2);
         ensure_zout(std::max(nz[0], nz[1]));
@@ -1042,7 +1046,7 @@ struct mpcd_gpu_sim {
         void* ptrs[] = {x, y, vx, vy, vx2, vy2, x2, y2, pid, pid2, cid, cidS, bin.cnt, bin.start, bin.cursor, bin.list, bin.key, tileSum, next,
                         tmpg, zsize, zoff, owner[0], owner[1], cover[0], cover[1], overCells[0], overCells[1], stage,
                         refPx, refPy, refUx, refUy, pdriveRaw, redPx, redPy, redUx, redUy, duRX, duRY, duEX, duEY,
-                        pdrive, cellTmp, mask, part, scal, err, wallPart, rects[0], rects[1], zout, zscratch, duG,
+                        pdrive, cellTmp, mask, part, scal, err, wallPart, rects[0], rects[1], zout, zscratch, duG, recG,
                         redN, redT, featBuf, tgtBuf, surFlag};
         for (void* q : ptrs)
             if (q) (void)hipFree(q);
@@ -1237,6 +1241,7 @@ struct mpcd_gpu_sim {
         a.nextG = next;
         a.tmpG = tmpg;
         a.duG = duG;
+        a.rec = recG;
         a.duStride = 2 * maxC;
         a.out = zout;
         a.err = err;

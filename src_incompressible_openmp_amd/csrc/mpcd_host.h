@@ -145,6 +145,8 @@ inline ZoneParams make_zone_params(const mpcd_params& p) {
     zp.highThrFloor = p.highThrFloor;
     zp.useLocal = p.useLocalFluidFractionThresholds;
     zp.maxPasses = p.maxRedistribPasses;
+    /* a particle's move record holds the dense pass that last moved it in a byte */
+    if (p.maxRedistribPasses > 254) throw ApiError(MPCD_E_INVALID, "maxRedistribPasses above 254 is not supported");
     zp.corrEnabled = p.enableMomentumCorrectionPostRedistribution;
     zp.dedupe = ((p.bc_left == MPCD_BC_PERIODIC && p.bc_right == MPCD_BC_PERIODIC && p.Nx < 5) ||
                  (p.bc_bottom == MPCD_BC_PERIODIC && p.bc_top == MPCD_BC_PERIODIC && p.Ny < 5))

@@ -195,6 +195,7 @@ struct ZoneCtxT {
     IT* nxG;     /* links of k >= nL (zone's global region, indexed by k) */
     IT* tmpG;    /* donor lists longer than ZONE_TMP_L (zone's global region) */
     double* duG; /* velocity change per cell (zone's global region, 2 per cell) */
+    unsigned char* rec; /* last move of each particle by local index (zone's global region, 1 per particle) */
     /* cell arrays */
     IT *cnt, *head, *tail, *dl, *cntS, *k0;
     unsigned short* fl;
@@ -212,10 +213,7 @@ struct ZoneCtxT {
     int* dord;      /* 8 */
     int* dord2;     /* 8 */
     double* nsm4;   /* 4 */
-    double *mvx, *mvy; /* ZONE_MAX_MOVE: staged dense movers' new-position draws */
-    int* mvk;          /* local index */
-    int* mvs;          /* slot */
-    unsigned short* mvo; /* ZF_OWN / ZF_COVER of the input cell */
+    int* mvk;          /* ZONE_MAX_MOVE: local indices of a materialised list's head run */
     IT* tmpL;          /* ZONE_TMP_L */
     ZoneShared* sh;
 };
@@ -468,16 +466,25 @@ ZHD void list_append(Z& z, int lc, int k) {
     T[lc] = (IT)k;
 }
 
-/* zone_pass.cpp:280-294: new position uniform in the target cell, kept
- * only where this zone owns the particle */
+/* A moved particle's new position (zone_pass.cpp:280-294: uniform in its
+ * target cell, drawn per dense pass or for the sparse phase) depends only on
+ * its last move, so a move records what it was -- ZM_NONE, the dense pass
+ * (1..254) or ZM_SPARSE -- and the correction sweep, which visits every
+ * particle in its final cell, writes the positions (owned particles only).
+ * The serial phases then touch no global memory for moves. */
+enum { ZM_NONE = 0, ZM_SPARSE = 255 };
 template <class Z>
-ZHD void move_to(Z& z, int lc, int slot, unsigned short own, double ux, double uy) {
-    if (!(own & ZF_OWN)) return;
+ZHD void mark_moved(Z& z, int k, int tag) {
+    z.rec[k] = (unsigned char)tag;
+}
+template <class Z>
+ZHD void place_moved(Z& z, int lc, int slot, uint32_t pid, int tag) {
+    double ux, uy;
+    if (tag == ZM_SPARSE) mpcd_draw_sparse_pos(z.seed, pid, &ux, &uy);
+    else mpcd_draw_dense_pos(z.seed, tag, pid, &ux, &uy);
     const int ix = lc / z.zh;
-    const double nx = (double)(z.xs + ix) * z.g.dxc + z.g.dxc * ux;
-    const double ny = (double)(z.ys + (lc - ix * z.zh)) * z.g.dyc + z.g.dyc * uy;
-    z.x[slot] = nx;
-    z.y[slot] = ny;
+    z.x[slot] = (double)(z.xs + ix) * z.g.dxc + z.g.dxc * ux;
+    z.y[slot] = (double)(z.ys + (lc - ix * z.zh)) * z.g.dyc + z.g.dyc * uy;
 }
 
 /* the bulk/skip masks of a transfer (zone_pass.cpp:492-525) */
@@ -576,25 +583,11 @@ ZHD void zone_shuffle(T* a, int m, int lane, int nl, ShufDraw draw) {
  * sorted by the candidate key, then one per bin per round.  All lanes; the
  * order lands in z.ord[0..nOrd) with each target's threshold in z.ordThr.
  * Every draw is taken in one round: the tie breaks (t = 0..23, one per
- * neighbour offset), the start bin (t = 24) and, when nStage > 0, the first
- * nStage particles of src's list, the only ones its spreads can move
- * (t = 25..): their slot, ownership and new-position draw. */
+ * neighbour offset) and the start bin (t = 24). */
 template <class Z>
-ZHD void dense_targets(Z& z, int src, int high, int pass, int phase, int lane, int nl, int nStage) {
+ZHD void dense_targets(Z& z, int src, int high, int pass, int phase, int lane, int nl) {
     const int sgx = lc_gx(z, src), sgy = lc_gy(z, src), sgc = lc_gc(z, src);
-    const bool srcMat = fget(z, src, ZF_MAT);
-    if (nStage > 0 && srcMat) {
-        if (lane == 0) {
-            int k = head_of(z, src);
-            for (int q = 0; q < nStage && k >= 0; ++q) {
-                z.mvk[q] = k;
-                k = nx_get(z, k);
-            }
-        }
-        ZBAR();
-    }
-    const int h0 = head_of(z, src);
-    for (int t = lane; t < 25 + nStage; t += nl) {
+    for (int t = lane; t < 25; t += nl) {
         if (t < 24) {
             /* neighbour offsets in (dy outer, dx inner) order, centre skipped */
             const int f = t < 12 ? t : t + 1;
@@ -606,17 +599,8 @@ ZHD void dense_targets(Z& z, int src, int high, int pass, int phase, int lane, i
             }
             z.nbGc[t] = gc;
             if (gc >= 0) z.nbTie[t] = mpcd_draw_tie(z.seed, pass, phase, (uint32_t)sgc, (uint32_t)gc);
-        } else if (t == 24) {
-            z.sh->startBin = mpcd_draw_start_bin(z.seed, pass, phase, (uint32_t)sgc);
         } else {
-            const int q = t - 25;
-            const int k = srcMat ? z.mvk[q] : h0 + q;
-            const int col = col_of(z, k);
-            uint32_t pid;
-            const int slot = slot_pid_in(z, k, col, pid);
-            z.mvs[q] = slot;
-            z.mvo[q] = own_bits(z, k, col);
-            mpcd_draw_dense_pos(z.seed, pass, pid, &z.mvx[q], &z.mvy[q]);
+            z.sh->startBin = mpcd_draw_start_bin(z.seed, pass, phase, (uint32_t)sgc);
         }
     }
     ZBAR();
@@ -755,12 +739,11 @@ ZHD void dense_targets(Z& z, int src, int high, int pass, int phase, int lane, i
 }
 
 /* spread_particles_from_dense_source_round_robin (zone_pass.cpp:527-589),
- * lane 0.  `used` counts the movers taken so far; the first nStaged were
- * staged by dense_targets.  The number of candidates at or after position kk
+ * lane 0.  The number of candidates at or after position kk
  * that still have a deficit is kept incrementally: within a sweep only the
  * candidate being served changes, and each target appears once. */
 template <class Z>
-ZHD int spread(Z& z, int src, int high, int needOut, int pass, int nStaged, int* rv, int& nr, int& used) {
+ZHD int spread(Z& z, int src, int needOut, int pass, int* rv, int& nr) {
     int moved = 0;
     const int nOrd = z.sh->nOrd;
     if (needOut <= 0 || nOrd == 0) return 0;
@@ -787,23 +770,7 @@ ZHD int spread(Z& z, int src, int high, int needOut, int pass, int nStaged, int*
                 const int nk = next_k(z, src, k);
                 head_set(z, src, nk);
                 if (nk < 0) tail_set(z, src, -1);
-                int slot;
-                unsigned short own;
-                double ux, uy;
-                if (used < nStaged) {
-                    slot = z.mvs[used];
-                    own = z.mvo[used];
-                    ux = z.mvx[used];
-                    uy = z.mvy[used];
-                } else {
-                    const int col = col_of(z, k);
-                    uint32_t pid;
-                    slot = slot_pid_in(z, k, col, pid);
-                    own = own_bits(z, k, col);
-                    mpcd_draw_dense_pos(z.seed, pass, pid, &ux, &uy);
-                }
-                move_to(z, nb, slot, own, ux, uy);
-                ++used;
+                mark_moved(z, k, pass);
                 list_append(z, nb, k);
             }
             avail -= nt;
@@ -1008,18 +975,16 @@ ZHD int spread_plan_wave(int nOrd, double thP, int& cP, int need, int& avail, in
     return moved;
 }
 
-/* Applies a plan: pops the movers from src's head, writes their new
- * positions, appends them to their targets in mover order. */
+/* Applies a plan: pops the movers from src's head, records their moves,
+ * appends them to their targets in mover order. */
 template <class Z>
-ZHD void spread_apply_wave(Z& z, int src, bool srcMat, int lane, int q0, int moved, int avail, int myK, int slot,
-                           unsigned short own, double ux, double uy, int myT, int nextQ, int lcP, int cP, int firstQ,
-                           int lastQ) {
+ZHD void spread_apply_wave(Z& z, int src, bool srcMat, int lane, int q0, int moved, int avail, int pass, int myK,
+                           int myT, int nextQ, int lcP, int cP, int firstQ, int lastQ) {
     if (moved <= 0) return;
     /* src first: a materialised list's next head is read before the movers'
      * links are rewritten */
     const int kLast = wave_readlane(myK, q0 + moved - 1);
     const int nh = srcMat ? nx_get(z, kLast) : (kLast < tail_of(z, src) ? kLast + 1 : -1);
-    const int tlc = lane_get(lcP, myT >= 0 ? myT : 0);
     const int nk = lane_get(myK, nextQ >= 0 ? nextQ : 0);
     const int fk = lane_get(myK, firstQ >= 0 ? firstQ : 0);
     const int lk = lane_get(myK, lastQ >= 0 ? lastQ : 0);
@@ -1031,7 +996,7 @@ ZHD void spread_apply_wave(Z& z, int src, bool srcMat, int lane, int q0, int mov
         cnt_set(z, src, avail);
     }
     if (myT >= 0) {
-        move_to(z, tlc, slot, own, ux, uy);
+        mark_moved(z, myK, pass);
         nx_set(z, myK, nextQ >= 0 ? nk : -1);
     }
     if (firstQ >= 0) {
@@ -1071,19 +1036,6 @@ ZHD int dense_cell_wave(Z& z, int src, int nc, double hl, int needOut, int pass,
     } else if (lane < nStage) {
         myK = head_of(z, src) + lane;
     }
-    /* slot, particle id and ownership; an implicit run lies in src's input
-     * run, so in src's column with src's ownership */
-    unsigned long long key = 0ull;
-    unsigned short own = 0;
-    if (lane < nStage) {
-        const int col = srcMat ? col_of(z, myK) : src / z.zh;
-        key = z.gkey[z.colG[col] + (myK - z.colL[col])];
-        own = srcMat ? own_bits(z, myK, col)
-                     : (z.allOwned ? (unsigned short)ZF_OWN : (unsigned short)(z.fl[src] & (ZF_OWN | ZF_COVER)));
-    }
-    const int slot = (int)(uint32_t)key;
-    double ux = 0.0, uy = 0.0;
-    bool drawn = false;
     int used = 0, avail = nc, moved = 0;
     int lcP, cP, myT, nextQ = -1, firstQ, lastQ;
     double thP;
@@ -1091,12 +1043,7 @@ ZHD int dense_cell_wave(Z& z, int src, int nc, double hl, int needOut, int pass,
     const int n0 = dense_rank_wave(z, sgx, sgy, sgc, 0, pass, 0, lane, lcP, thP, cP);
     int q0 = used;
     int m0 = spread_plan_wave(n0, thP, cP, needOut, avail, used, nStage, lane, myT, nextQ, firstQ, lastQ);
-    if (m0 > 0) {
-        if (lane < nStage) mpcd_draw_dense_pos(z.seed, pass, (uint32_t)(key >> 32), &ux, &uy);
-        drawn = true;
-        spread_apply_wave(z, src, srcMat, lane, q0, m0, avail, myK, slot, own, ux, uy, myT, nextQ, lcP, cP, firstQ,
-                          lastQ);
-    }
+    spread_apply_wave(z, src, srcMat, lane, q0, m0, avail, pass, myK, myT, nextQ, lcP, cP, firstQ, lastQ);
     moved += m0;
     const int rcv0 = firstQ >= 0 ? lcP : -1;
     /* phase 1: high thresholds, for what is left above the high threshold */
@@ -1107,11 +1054,7 @@ ZHD int dense_cell_wave(Z& z, int src, int nc, double hl, int needOut, int pass,
         q0 = used;
         nextQ = -1;
         const int m1 = spread_plan_wave(n1, thP, cP, remaining, avail, used, nStage, lane, myT, nextQ, firstQ, lastQ);
-        if (m1 > 0) {
-            if (!drawn && lane < nStage) mpcd_draw_dense_pos(z.seed, pass, (uint32_t)(key >> 32), &ux, &uy);
-            spread_apply_wave(z, src, srcMat, lane, q0, m1, avail, myK, slot, own, ux, uy, myT, nextQ, lcP, cP,
-                              firstQ, lastQ);
-        }
+        spread_apply_wave(z, src, srcMat, lane, q0, m1, avail, pass, myK, myT, nextQ, lcP, cP, firstQ, lastQ);
         moved += m1;
         rcv1 = firstQ >= 0 ? lcP : -1;
     }
@@ -1164,24 +1107,19 @@ ZHD void zone_dense_passes(Z& z, int lane, int nl) {
                 z.sh->nr = 0;
             }
             if (needOut > 0) {
-                /* the spreads move at most needOut particles, all from the
-                 * head of the list: they are staged with the first target
-                 * ranking's draws */
-                const int nStage = imin(imin(needOut, nc), ZONE_MAX_MOVE);
-                dense_targets(z, c, 0, pass, 0, lane, nl, nStage);
+                dense_targets(z, c, 0, pass, 0, lane, nl);
                 if (lane == 0) {
-                    int used = 0, nr = 0;
-                    moved += spread(z, c, 0, needOut, pass, nStage, z.recvT, nr, used);
-                    z.sh->stage = used;
+                    int nr = 0;
+                    moved += spread(z, c, needOut, pass, z.recvT, nr);
                     z.sh->nr = nr;
                 }
                 ZBAR();
                 const int remaining = imax(0, (int)MPCD_CEIL((double)cnt_of(z, c) - hl));
                 if (remaining > 0) {
-                    dense_targets(z, c, 1, pass, 1, lane, nl, 0);
+                    dense_targets(z, c, 1, pass, 1, lane, nl);
                     if (lane == 0) {
-                        int used = z.sh->stage, nr = z.sh->nr;
-                        moved += spread(z, c, 1, remaining, pass, nStage, z.recvT, nr, used);
+                        int nr = z.sh->nr;
+                        moved += spread(z, c, remaining, pass, z.recvT, nr);
                         z.sh->nr = nr;
                     }
                 }
@@ -1253,17 +1191,12 @@ ZHD void donor_transfer(Z& z, int c, int d, int md, int nt, int cgc, T* tmp, int
     const int dgc = lc_gc(z, d);
     zone_shuffle(tmp, md, lane, nl,
                  ShufDraw{z.seed, ZS_PARTICLE, (uint32_t)cgc, (uint32_t)dgc});
-    /* the movers: new positions, links of the moved run and of the
-     * donor's remaining run, all lanes */
+    /* the movers' records, links of the moved run and of the donor's
+     * remaining run, all lanes */
     for (int q = lane; q < md; q += nl) {
         const int kq = (int)tmp[q];
         if (q < nt) {
-            const int col = col_of(z, kq);
-            uint32_t pid;
-            const int slot = slot_pid_in(z, kq, col, pid);
-            double ux, uy;
-            mpcd_draw_sparse_pos(z.seed, pid, &ux, &uy);
-            move_to(z, c, slot, own_bits(z, kq, col), ux, uy);
+            mark_moved(z, kq, ZM_SPARSE);
             nx_set(z, kq, q + 1 < nt ? (int)tmp[q + 1] : -1);
         } else {
             nx_set(z, kq, q + 1 < md ? (int)tmp[q + 1] : -1);
@@ -1501,16 +1434,19 @@ ZHD double zone_correct_write(Z& z, int lc) {
         const double dux = z.duG[2 * lc], duy = z.duG[2 * lc + 1];
         int k = head_of(z, lc);
         while (k >= 0) {
-            int sl[8];
+            int sl[8], tg[8];
+            uint32_t pd[8];
             unsigned short ow[8];
             double ax[8], ay[8];
             for (int u = 0; u < 8; ++u) {
                 sl[u] = -1;
                 ow[u] = 0;
+                tg[u] = ZM_NONE;
                 if (k >= 0) {
                     const int col = col_of(z, k);
-                    sl[u] = slot_in(z, k, col);
+                    sl[u] = slot_pid_in(z, k, col, pd[u]);
                     ow[u] = own_bits(z, k, col);
+                    tg[u] = z.rec[k];
                     k = next_k(z, lc, k);
                 }
             }
@@ -1521,6 +1457,7 @@ ZHD double zone_correct_write(Z& z, int lc) {
             }
             for (int u = 0; u < 8; ++u) {
                 if (sl[u] < 0 || !(ow[u] & ZF_OWN)) continue;
+                if (tg[u] != ZM_NONE) place_moved(z, lc, sl[u], pd[u], tg[u]);
                 double vx = ax[u], vy = ay[u];
                 if (corr) {
                     vx += dux;
@@ -1593,11 +1530,11 @@ ZHD size_t zone_fixed_bytes(int C, int W, int useLocal) {
     size_t b = 0;
     if (useLocal) b += (size_t)C * 2 * sizeof(double);         /* lowA highA */
     b += 24 * sizeof(Cand) + 2 * 24 * sizeof(double) + 8 * sizeof(double) + 4 * sizeof(double) +
-         (size_t)ZONE_MAX_MOVE * (2 * sizeof(double) + 2 * sizeof(int)); /* 8-byte items first */
+         (size_t)ZONE_MAX_MOVE * sizeof(int); /* 8-byte items first */
     b += (size_t)W * 2 * sizeof(int);                           /* colL colG */
     b += (5 * 24 + 3 * 8) * sizeof(int);                        /* ord nbGc rnk recvT (+pad) dord dord2 (+pad) */
     b += (size_t)C * 6 * sizeof(IT);                            /* cnt head tail dl cntS k0 */
-    b += (size_t)C * sizeof(unsigned short) + ZONE_MAX_MOVE * sizeof(unsigned short); /* fl mvo */
+    b += (size_t)C * sizeof(unsigned short); /* fl */
     b += (size_t)ZONE_TMP_L * sizeof(IT);                       /* tmpL */
     return (b + 15) & ~(size_t)15;
 }
@@ -1620,10 +1557,7 @@ ZHD void zone_carve(ZoneCtxT<IT>& z, unsigned char* base, size_t bytes) {
     z.nbTie = (double*)p; p += 24 * sizeof(double);
     z.dsc = (double*)p; p += 8 * sizeof(double);
     z.nsm4 = (double*)p; p += 4 * sizeof(double);
-    z.mvx = (double*)p; p += ZONE_MAX_MOVE * sizeof(double);
-    z.mvy = (double*)p; p += ZONE_MAX_MOVE * sizeof(double);
     z.mvk = (int*)p; p += ZONE_MAX_MOVE * sizeof(int);
-    z.mvs = (int*)p; p += ZONE_MAX_MOVE * sizeof(int);
     z.colL = (int*)p; p += (size_t)W * sizeof(int);
     z.colG = (int*)p; p += (size_t)W * sizeof(int);
     z.ord = (int*)p; p += 24 * sizeof(int);
@@ -1640,7 +1574,6 @@ ZHD void zone_carve(ZoneCtxT<IT>& z, unsigned char* base, size_t bytes) {
     z.k0 = (IT*)p; p += (size_t)C * sizeof(IT);
     z.tmpL = (IT*)p; p += (size_t)ZONE_TMP_L * sizeof(IT);
     z.fl = (unsigned short*)p; p += (size_t)C * sizeof(unsigned short);
-    z.mvo = (unsigned short*)p; p += ZONE_MAX_MOVE * sizeof(unsigned short);
     const size_t fixed = zone_fixed_bytes<IT>(C, W, z.zp.useLocal);
     p = base + fixed;
     z.nxL = (IT*)p;
@@ -1682,6 +1615,7 @@ template <class Z>
 ZHD void zone_run(Z& z, int lane, int nl) {
     z.P = z.sh->P;
     z.allOwned = z.sh->allOwned;
+    for (int k = lane; k < z.P; k += nl) z.rec[k] = (unsigned char)ZM_NONE;
     zone_dense_passes(z, lane, nl);
     zone_sparse_pass(z, lane, nl);
     ZBAR();

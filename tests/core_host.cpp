@@ -100,6 +100,8 @@ void run_zone(const ZoneIn& in, ZoneRect r, int zid, uint64_t seed, size_t bytes
               std::vector<IT>& tmpG, std::vector<double>& duG, ZoneShared& sh) {
     ZoneCtxT<IT> z{};
     zone_init(z, in, r, zid, seed, &sh);
+    std::vector<unsigned char> rec(nxG.size(), 0xEE);
+    z.rec = rec.data();
     z.nxG = nxG.data();
     z.tmpG = tmpG.data();
     z.duG = duG.data();
