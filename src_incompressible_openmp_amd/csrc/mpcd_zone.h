@@ -201,6 +201,7 @@ struct ZoneCtxT {
     unsigned short* fl;
     int *colL, *colG; /* per column: first local index, start in the binning list */
     double *lowA, *highA; /* local fluid-fraction thresholds (only with useLocal) */
+    IT* halo;             /* pass-input counts of the ring of width 2 around the zone */
     /* small scratch */
     Cand* cand;     /* 24 */
     int* ord;       /* 24 */
@@ -310,12 +311,52 @@ ZHD int lc_gc(const Z& z, int lc) {
     return lc_gy(z, lc) + z.g.Ny * lc_gx(z, lc);
 }
 
+/* Slot in z.halo of the cell at zone-relative offset (dx, dy) of the ring of
+ * width 2 around the zone: two rows below and above (all zw+4 columns), then
+ * two columns left and right (the zh rows between); -1 inside the zone or
+ * beyond the ring. */
+template <class Z>
+ZHD int halo_slot(const Z& z, int dx, int dy) {
+    if (dx < -2 || dx >= z.zw + 2 || dy < -2 || dy >= z.zh + 2) return -1;
+    const int hx = dx + 2, hy = dy + 2, W4 = z.zw + 4;
+    if (hy < 2) return hx * 2 + hy;
+    if (hy >= z.zh + 2) return 2 * W4 + hx * 2 + (hy - z.zh - 2);
+    if (hx < 2) return 4 * W4 + hx * z.zh + (hy - 2);
+    if (hx >= z.zw + 2) return 4 * W4 + 2 * z.zh + (hx - z.zw - 2) * z.zh + (hy - 2);
+    return -1;
+}
+template <class Z>
+ZHD int halo_size(const Z& z) {
+    return 4 * (z.zw + 4) + 4 * z.zh;
+}
+/* the ring slot of global cell (gx, gy) outside the zone, periodic axes
+ * taken to the image next to the zone */
+template <class Z>
+ZHD int halo_of(const Z& z, int gx, int gy) {
+    int dx = gx - z.xs, dy = gy - z.ys;
+    if (z.g.perx) {
+        if (dx < -2) dx += z.g.Nx;
+        else if (dx >= z.zw + 2) dx -= z.g.Nx;
+    }
+    if (z.g.pery) {
+        if (dy < -2) dy += z.g.Ny;
+        else if (dy >= z.zh + 2) dy -= z.g.Ny;
+    }
+    return halo_slot(z, dx, dy);
+}
+
 /* count at a global cell: live zone count (or the sparse-phase copy), else
- * the pass-input snapshot. */
+ * the pass-input snapshot, from the ring in LDS (a count the index type
+ * cannot hold is read from global memory). */
 template <class Z>
 ZHD int count_at(const Z& z, int gx, int gy, int use_copy) {
     const int lc = zone_lc(z, gx, gy);
     if (lc >= 0) return use_copy ? (int)z.cntS[lc] : (int)z.cnt[lc];
+    const int h = halo_of(z, gx, gy);
+    if (h >= 0) {
+        const int v = iget(z.halo[h]);
+        if (v >= 0) return v;
+    }
     return z.gcnt[gy + z.g.Ny * gx];
 }
 
@@ -1346,6 +1387,33 @@ ZHD void zone_setup(Z& z, const ZoneIn& in, int lane, int nl) {
         base += wave_bcast(incl, nl - 1);
     }
     const bool anyMixed = wave_ballot(mixed) != 0ull;
+    /* the ring of pass-input counts around the zone */
+    const int nh = halo_size(z);
+    for (int h = lane; h < nh; h += nl) {
+        const int W4 = z.zw + 4;
+        int hx, hy;
+        if (h < 2 * W4) {
+            hx = h / 2;
+            hy = h % 2;
+        } else if (h < 4 * W4) {
+            hx = (h - 2 * W4) / 2;
+            hy = z.zh + 2 + (h - 2 * W4) % 2;
+        } else if (h < 4 * W4 + 2 * z.zh) {
+            hx = (h - 4 * W4) / z.zh;
+            hy = 2 + (h - 4 * W4) % z.zh;
+        } else {
+            hx = z.zw + 2 + (h - 4 * W4 - 2 * z.zh) / z.zh;
+            hy = 2 + (h - 4 * W4 - 2 * z.zh) % z.zh;
+        }
+        int gx = z.xs + hx - 2, gy = z.ys + hy - 2;
+        if (z.g.perx) gx = wrapi(gx, z.g.Nx);
+        if (z.g.pery) gy = wrapi(gy, z.g.Ny);
+        int v = 0;
+        if (gx >= 0 && gx < z.g.Nx && gy >= 0 && gy < z.g.Ny) v = z.gcnt[gy + z.g.Ny * gx];
+        /* the unsigned 16-bit type keeps its top value for "read the global count" */
+        if (sizeof(typename Z::idx_t) == 2 && v >= 0xFFFF) v = -1;
+        z.halo[h] = (typename Z::idx_t)v;
+    }
     if (lane == 0) {
         z.sh->P = base;
         z.sh->allOwned = anyMixed ? 0 : 1;
@@ -1534,6 +1602,7 @@ ZHD size_t zone_fixed_bytes(int C, int W, int useLocal) {
     b += (size_t)W * 2 * sizeof(int);                           /* colL colG */
     b += (5 * 24 + 3 * 8) * sizeof(int);                        /* ord nbGc rnk recvT (+pad) dord dord2 (+pad) */
     b += (size_t)C * 6 * sizeof(IT);                            /* cnt head tail dl cntS k0 */
+    b += (size_t)(4 * (W + 4) + 4 * (C / (W > 0 ? W : 1))) * sizeof(IT); /* halo */
     b += (size_t)C * sizeof(unsigned short); /* fl */
     b += (size_t)ZONE_TMP_L * sizeof(IT);                       /* tmpL */
     return (b + 15) & ~(size_t)15;
@@ -1572,6 +1641,7 @@ ZHD void zone_carve(ZoneCtxT<IT>& z, unsigned char* base, size_t bytes) {
     z.dl = (IT*)p; p += (size_t)C * sizeof(IT);
     z.cntS = (IT*)p; p += (size_t)C * sizeof(IT);
     z.k0 = (IT*)p; p += (size_t)C * sizeof(IT);
+    z.halo = (IT*)p; p += (size_t)halo_size(z) * sizeof(IT);
     z.tmpL = (IT*)p; p += (size_t)ZONE_TMP_L * sizeof(IT);
     z.fl = (unsigned short*)p; p += (size_t)C * sizeof(unsigned short);
     const size_t fixed = zone_fixed_bytes<IT>(C, W, z.zp.useLocal);
