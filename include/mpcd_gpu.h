@@ -141,7 +141,8 @@ enum {
     MPCD_PROF_ZONE_KERNEL = 5, /* the zone kernel launch alone (2 per step) */
     MPCD_PROF_CLOSURE = 6,     /* closure: fields, beta, stencil, kick */
     MPCD_PROF_KICK = 7,        /* the closure's per-particle kick alone */
-    MPCD_PROF_CLASSES = 8
+    MPCD_PROF_ZONE_FINISH = 8, /* the zone passes' momentum correction kernel (2 per step) */
+    MPCD_PROF_CLASSES = 9
 };
 
 /* Opaque device-resident simulation (state, binnings, layouts). */

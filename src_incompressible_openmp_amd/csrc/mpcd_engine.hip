@@ -464,6 +464,8 @@ struct ZoneArgs {
     void* tmpG;     /* long donor lists, IT per particle */
     double* duG;    /* 2 per cell per zone */
     unsigned char* rec; /* last move of each particle, 1 per particle slot of the zone regions */
+    unsigned char* zstate; /* per-zone state between the passes and the correction */
+    size_t zsStride;
     int duStride;   /* doubles per zone */
     int pMin, pMax; /* zones with pMin <= P <= pMax run in this launch */
     size_t ldsBytes;
@@ -516,17 +518,119 @@ __global__ void __launch_bounds__(ZONE_TPB) k_zone(ZoneArgs a) {
     else zone_carve(z, a.gscratch + (size_t)zi * a.gstride, a.gstride);
     zone_setup(z, a.in, lane, ZONE_TPB);
     __syncthreads();
-    zone_run(z, lane, ZONE_TPB);
-    __syncthreads();
+    zone_passes(z, lane, ZONE_TPB);
+    zone_dump(z, zone_state_at<IT>(a.zstate + (size_t)zi * a.zsStride, z.C, z.zw), lane, ZONE_TPB);
     if (lane == 0) {
         ZoneOut o;
         o.nHigh = sh.nHigh;
         o.nLow = sh.nLow;
         o.movedDense = sh.movedDense;
         o.movedSparse = sh.movedSparse;
-        o.stats4 = sh.stats4;
-        o.stats5 = sh.stats5;
+        o.stats4 = 0.0;
+        o.stats5 = 0.0;
         a.out[zi] = o;
+    }
+}
+
+/* The momentum correction of the zones k_zone left (zone_pass.cpp:870-922),
+ * FINISH_TPB threads per zone: one thread per cell for the sums and the
+ * writes, then one wavefront for the ordered stats.  The same per-cell code
+ * as the single-wavefront form, so the same results. */
+constexpr int FINISH_TPB = 256;
+constexpr int FINISH_LDS = 40 * 1024; /* cell arrays, column map and as many links as fit */
+template <class IT, bool kLds>
+__global__ void __launch_bounds__(FINISH_TPB) k_zone_finish(ZoneArgs a) {
+    extern __shared__ __align__(16) unsigned char lds[];
+    const int zi = blockIdx.x;
+    if (zi >= a.nZones) return;
+    const int P = a.zsize[zi];
+    if (P < a.pMin || P > a.pMax) return;
+    const long long off = a.zoff[zi];
+    if (off + (long long)P > a.scratchCap) return;
+    const int tid = threadIdx.x;
+    ZoneCtxT<IT> z;
+    const uint64_t seed = a.singleSeed ? a.seedBase : a.seedBase + 104729ULL * (uint64_t)(zi + 1);
+    zone_init(z, a.in, a.rects[zi], zi + 1, seed, nullptr);
+    z.nxG = (IT*)a.nextG + off;
+    z.tmpG = (IT*)a.tmpG + off;
+    z.duG = a.duG + (size_t)zi * (size_t)a.duStride;
+    z.rec = a.rec + off;
+    const ZoneState<IT> st = zone_state_at<IT>(a.zstate + (size_t)zi * a.zsStride, z.C, z.zw);
+    zone_attach_state(z, st);
+    if (kLds) {
+        /* the state and the first links to LDS */
+        unsigned char* p = lds;
+        IT* cnt = (IT*)p; p += (size_t)z.C * sizeof(IT);
+        IT* head = (IT*)p; p += (size_t)z.C * sizeof(IT);
+        IT* tail = (IT*)p; p += (size_t)z.C * sizeof(IT);
+        IT* k0 = (IT*)p; p += (size_t)z.C * sizeof(IT);
+        unsigned short* fl = (unsigned short*)p; p += (size_t)z.C * sizeof(unsigned short);
+        p = lds + (((size_t)(p - lds) + 3) & ~(size_t)3);
+        int* colL = (int*)p; p += (size_t)z.zw * sizeof(int);
+        int* colG = (int*)p; p += (size_t)z.zw * sizeof(int);
+        IT* links = (IT*)p;
+        const long long room = ((long long)FINISH_LDS - (long long)(p - lds)) / (long long)sizeof(IT);
+        const int nL = (int)(room < (long long)z.P ? (room > 0 ? room : 0) : z.P);
+        for (int lc = tid; lc < z.C; lc += FINISH_TPB) {
+            cnt[lc] = st.cnt[lc];
+            head[lc] = st.head[lc];
+            tail[lc] = st.tail[lc];
+            k0[lc] = st.k0[lc];
+            fl[lc] = st.fl[lc];
+        }
+        for (int c = tid; c < z.zw; c += FINISH_TPB) {
+            colL[c] = st.colL[c];
+            colG[c] = st.colG[c];
+        }
+        for (int k = tid; k < nL; k += FINISH_TPB) links[k] = z.nxG[k];
+        z.cnt = cnt;
+        z.head = head;
+        z.tail = tail;
+        z.k0 = k0;
+        z.fl = fl;
+        z.colL = colL;
+        z.colG = colG;
+        z.nxL = links;
+        z.nL = nL;
+        __syncthreads();
+    }
+    /* phase 1: input-run sums, kept by the thread of each cell */
+    constexpr int CPT = 4; /* cells per thread held in registers */
+    double pbx[CPT], pby[CPT];
+    const bool regs = z.C <= CPT * FINISH_TPB;
+    if (regs)
+        for (int j = 0; j < CPT; ++j) {
+            const int lc = tid + j * FINISH_TPB;
+            pbx[j] = pby[j] = 0.0;
+            if (lc < z.C) fin_input_sum(z, lc, pbx[j], pby[j]);
+        }
+    __syncthreads();
+    /* phase 2: final lists */
+    if (regs) {
+        for (int j = 0; j < CPT; ++j) {
+            const int lc = tid + j * FINISH_TPB;
+            if (lc < z.C) st.dsq[lc] = fin_cell(z, lc, pbx[j], pby[j]);
+        }
+    } else {
+        /* more cells than threads x CPT: the two-sweep form, positions included */
+        for (int lc = tid; lc < z.C; lc += FINISH_TPB) zone_correct_sums(z, lc);
+        __syncthreads();
+        for (int lc = tid; lc < z.C; lc += FINISH_TPB) st.dsq[lc] = zone_correct_write(z, lc);
+    }
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+    /* phase 3: moved particles' positions (the two-sweep form placed them) */
+    if (regs) {
+        __shared__ int queue[FINISH_TPB];
+        fin_place(z, tid, FINISH_TPB, queue + (tid & ~63));
+    }
+    if (tid < 64) {
+        double rms, mx;
+        zone_correct_stats(z, st.dsq, tid, 64, rms, mx);
+        if (tid == 0) {
+            a.out[zi].stats4 = rms;
+            a.out[zi].stats5 = mx;
+        }
     }
 }
 
@@ -889,6 +993,8 @@ struct mpcd_gpu_sim {
     int zoutCap = 0;
     double* duG = nullptr;
     unsigned char* recG = nullptr; /* move records, 1 byte per particle slot of the zone regions */
+    unsigned char* zstate = nullptr; /* per-zone pass state for the correction kernel */
+    size_t zstateCap = 0;
     size_t duCap = 0;
     unsigned char* zscratch = nullptr;
     size_t zscratchCap = 0;
@@ -1046,7 +1152,7 @@ struct mpcd_gpu_sim {
         void* ptrs[] = {x, y, vx, vy, vx2, vy2, x2, y2, pid, pid2, cid, cidS, bin.cnt, bin.start, bin.cursor, bin.list, bin.key, tileSum, next,
                         tmpg, zsize, zoff, owner[0], owner[1], cover[0], cover[1], overCells[0], overCells[1], stage,
                         refPx, refPy, refUx, refUy, pdriveRaw, redPx, redPy, redUx, redUy, duRX, duRY, duEX, duEY,
-                        pdrive, cellTmp, mask, part, scal, err, wallPart, rects[0], rects[1], zout, zscratch, duG, recG,
+                        pdrive, cellTmp, mask, part, scal, err, wallPart, rects[0], rects[1], zout, zscratch, duG, recG, zstate,
                         redN, redT, featBuf, tgtBuf, surFlag};
         for (void* q : ptrs)
             if (q) (void)hipFree(q);
@@ -1065,6 +1171,12 @@ struct mpcd_gpu_sim {
         if (duG) HIP_OK(hipFree(duG));
         duG = dalloc<double>(m);
         duCap = m;
+    }
+    void ensure_zstate(size_t m) {
+        if (m <= zstateCap) return;
+        if (zstate) HIP_OK(hipFree(zstate));
+        zstate = dalloc<unsigned char>(m);
+        zstateCap = m;
     }
     void ensure_zout(int m) {
         if (m <= zoutCap) return;
@@ -1242,6 +1354,9 @@ struct mpcd_gpu_sim {
         a.tmpG = tmpg;
         a.duG = duG;
         a.rec = recG;
+        a.zsStride = zone_state_bytes<int>(maxC, maxW);
+        ensure_zstate(a.zsStride * (size_t)nZ);
+        a.zstate = zstate;
         a.duStride = 2 * maxC;
         a.out = zout;
         a.err = err;
@@ -1274,6 +1389,27 @@ struct mpcd_gpu_sim {
             hipLaunchKernelGGL((k_zone<int, false>), dim3(nZ), dim3(ZONE_TPB), 0, stream, a);
         }
         pend(MPCD_PROF_ZONE_KERNEL);
+        check_launch();
+        /* the correction, with the same split of the zones by index type */
+        pbeg(MPCD_PROF_ZONE_FINISH);
+        /* LDS staging when the cell arrays and the column map leave room for links */
+        const size_t cellB16 = (size_t)maxC * (4 * sizeof(uint16_t) + 2) + 8 * (size_t)maxW + 16;
+        const size_t cellB32 = (size_t)maxC * (4 * sizeof(int) + 2) + 8 * (size_t)maxW + 16;
+        if (fast) {
+            a.pMin = 0;
+            a.pMax = 65534;
+            if (cellB16 + 4096 <= (size_t)FINISH_LDS)
+                hipLaunchKernelGGL((k_zone_finish<uint16_t, true>), dim3(nZ), dim3(FINISH_TPB), FINISH_LDS, stream, a);
+            else
+                hipLaunchKernelGGL((k_zone_finish<uint16_t, false>), dim3(nZ), dim3(FINISH_TPB), 0, stream, a);
+        }
+        a.pMin = fast ? 65535 : 0;
+        a.pMax = 0x7fffffff;
+        if (cellB32 + 4096 <= (size_t)FINISH_LDS)
+            hipLaunchKernelGGL((k_zone_finish<int, true>), dim3(nZ), dim3(FINISH_TPB), FINISH_LDS, stream, a);
+        else
+            hipLaunchKernelGGL((k_zone_finish<int, false>), dim3(nZ), dim3(FINISH_TPB), 0, stream, a);
+        pend(MPCD_PROF_ZONE_FINISH);
         check_launch();
         if (!single && coverDev && nOverCells > 0) {
             hipLaunchKernelGGL(k_overlap_copyback, dim3(nblk(nOverCells)), dim3(TPB), 0, stream, overDev, nOverCells,

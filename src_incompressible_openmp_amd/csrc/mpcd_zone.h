@@ -58,6 +58,13 @@ namespace mpcdk {
 
 #if defined(__HIP_DEVICE_COMPILE__)
 #define ZBAR() __syncthreads()
+/* compiler-level ordering of LDS accesses between lanes of the one
+ * wavefront (the LDS executes a wavefront's accesses in issue order) */
+#define ZWSYNC()                                               \
+    do {                                                       \
+        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront"); \
+        __builtin_amdgcn_wave_barrier();                       \
+    } while (0)
 ZHD unsigned long long wave_ballot(bool f) { return __ballot(f); }
 ZHD int wave_popc(unsigned long long m) { return __popcll(m); }
 ZHD int wave_shfl_up_add(int v, int lane) {
@@ -89,6 +96,7 @@ ZHD double wave_max(double v) {
 }
 #else
 #define ZBAR() ((void)0)
+#define ZWSYNC() ((void)0)
 ZHD unsigned long long wave_ballot(bool f) { return f ? 1ull : 0ull; }
 ZHD int wave_popc(unsigned long long m) { return __builtin_popcountll(m); }
 ZHD int wave_shfl_up_add(int v, int) { return v; }
@@ -840,13 +848,6 @@ ZHD int spread(Z& z, int src, int needOut, int pass, int* rv, int& nr) {
  * path, and the moves and list links are written by the lanes at once.  The
  * results are those of dense_targets + spread + transfer_masks. */
 
-/* compiler-level ordering of LDS accesses between lanes of the one
- * wavefront (the LDS executes a wavefront's accesses in issue order) */
-#define ZWSYNC()                                                    \
-    do {                                                            \
-        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");      \
-        __builtin_amdgcn_wave_barrier();                            \
-    } while (0)
 
 /* value held by lane src (per-lane src) */
 ZHD int lane_get(int v, int src) { return __shfl(v, src, 64); }
@@ -1679,17 +1680,311 @@ ZHD void zone_init(ZoneCtxT<IT>& z, const ZoneIn& in, ZoneRect r, int zid, uint6
     z.sh = sh;
 }
 
-/* The zone after zone_setup and a barrier: both phases, the correction and
- * the stats. */
+/* The zone after zone_setup and a barrier: both phases. */
 template <class Z>
-ZHD void zone_run(Z& z, int lane, int nl) {
+ZHD void zone_passes(Z& z, int lane, int nl) {
     z.P = z.sh->P;
     z.allOwned = z.sh->allOwned;
     for (int k = lane; k < z.P; k += nl) z.rec[k] = (unsigned char)ZM_NONE;
     zone_dense_passes(z, lane, nl);
     zone_sparse_pass(z, lane, nl);
     ZBAR();
+}
+
+/* Both phases, the correction and the stats, in one go. */
+template <class Z>
+ZHD void zone_run(Z& z, int lane, int nl) {
+    zone_passes(z, lane, nl);
     zone_correct(z, lane, nl);
+}
+
+/* ---- the zone's state between the passes and the correction ----
+ *
+ * On the device the correction runs in a kernel of its own, with many
+ * threads per zone: the pass kernel leaves the cell arrays, the column map
+ * and the flags in a per-zone state region of global memory, and the links
+ * all in the zone's global link region.  Layout (16-byte aligned):
+ * |dsq C doubles|hdr 4 ints|colL W ints|colG W ints|cnt|head|tail|k0 C IT each|fl C u16| */
+template <class IT>
+struct ZoneState {
+    double* dsq; /* |du|^2 of each cell, for the ordered stats */
+    int* hdr;    /* P, allOwned */
+    int *colL, *colG;
+    IT *cnt, *head, *tail, *k0;
+    unsigned short* fl;
+};
+
+template <class IT>
+ZHD size_t zone_state_bytes(int C, int W) {
+    const size_t b = (size_t)C * sizeof(double) + 4 * sizeof(int) + (size_t)W * 2 * sizeof(int) +
+                     (size_t)C * 4 * sizeof(IT) + (size_t)C * sizeof(unsigned short);
+    return (b + 15) & ~(size_t)15;
+}
+
+template <class IT>
+ZHD ZoneState<IT> zone_state_at(unsigned char* base, int C, int W) {
+    ZoneState<IT> st;
+    unsigned char* p = base;
+    st.dsq = (double*)p; p += (size_t)C * sizeof(double);
+    st.hdr = (int*)p; p += 4 * sizeof(int);
+    st.colL = (int*)p; p += (size_t)W * sizeof(int);
+    st.colG = (int*)p; p += (size_t)W * sizeof(int);
+    st.cnt = (IT*)p; p += (size_t)C * sizeof(IT);
+    st.head = (IT*)p; p += (size_t)C * sizeof(IT);
+    st.tail = (IT*)p; p += (size_t)C * sizeof(IT);
+    st.k0 = (IT*)p; p += (size_t)C * sizeof(IT);
+    st.fl = (unsigned short*)p;
+    return st;
+}
+
+/* all lanes: the state after zone_passes, and the links held in LDS moved
+ * to the zone's global link region */
+template <class IT>
+ZHD void zone_dump(ZoneCtxT<IT>& z, const ZoneState<IT>& st, int lane, int nl) {
+    for (int lc = lane; lc < z.C; lc += nl) {
+        st.cnt[lc] = z.cnt[lc];
+        st.head[lc] = z.head[lc];
+        st.tail[lc] = z.tail[lc];
+        st.k0[lc] = z.k0[lc];
+        st.fl[lc] = z.fl[lc];
+    }
+    for (int c = lane; c < z.zw; c += nl) {
+        st.colL[c] = z.colL[c];
+        st.colG[c] = z.colG[c];
+    }
+    if (lane == 0) {
+        st.hdr[0] = z.P;
+        st.hdr[1] = z.allOwned;
+    }
+    const int nk = z.P < z.nL ? z.P : z.nL;
+    const IT* const L = z.nxL;
+    IT* const G = z.nxG;
+    for (int k = lane; k < nk; k += nl) ZG_STORE(G + k, L[k]);
+}
+
+/* the correction's view of a dumped zone: cell arrays and column map in the
+ * state region, every link in the global link region */
+template <class IT>
+ZHD void zone_attach_state(ZoneCtxT<IT>& z, const ZoneState<IT>& st) {
+    z.cnt = st.cnt;
+    z.head = st.head;
+    z.tail = st.tail;
+    z.k0 = st.k0;
+    z.fl = st.fl;
+    z.colL = st.colL;
+    z.colG = st.colG;
+    z.P = st.hdr[0];
+    z.allOwned = st.hdr[1];
+    z.nxL = nullptr;
+    z.nL = 0;
+}
+
+/* length of cell lc's input run (its local indices k0[lc] ..) */
+template <class Z>
+ZHD int in_cnt(const Z& z, int lc) {
+    const int col = lc / z.zh;
+    const int end = (lc - col * z.zh + 1 < z.zh) ? (int)z.k0[lc + 1] : (col + 1 < z.zw ? (int)z.colL[col + 1] : z.P);
+    return end - (int)z.k0[lc];
+}
+
+/* ---- the correction kernel's per-cell form (one thread per cell) ----
+ * Phase 1 sums every changed cell's input run; after a barrier, phase 2
+ * walks each cell's final list once: a list of up to FIN_L entries stays in
+ * registers from the sums to the writes (a particle is in exactly one final
+ * list, and every input-run read happened in phase 1), a longer one is
+ * walked twice.  The moved particles' links are set to their final cell for
+ * phase 3, which places them.  Same sums in the same order as
+ * zone_correct_sums / zone_correct_write. */
+constexpr int FIN_L = 24;
+
+/* column of particle k of cell lc's final list: its own input run lies in
+ * lc's column */
+template <class Z>
+ZHD int fin_col(const Z& z, int lc, int kb, int ke, int k) {
+    return (k >= kb && k < ke) ? lc / z.zh : col_of(z, k);
+}
+
+template <class Z>
+ZHD void fin_input_sum(const Z& z, int lc, double& pbx, double& pby) {
+    pbx = 0.0;
+    pby = 0.0;
+    if (!fget(z, lc, ZF_CHG) || cnt_of(z, lc) == 0) return;
+    const int gc = lc_gc(z, lc);
+    const int s0 = z.gstart[gc], n = z.gcnt[gc];
+    for (int j0 = 0; j0 < n; j0 += FIN_L) {
+        int sl[FIN_L];
+        double ax[FIN_L], ay[FIN_L];
+        for (int u = 0; u < FIN_L; ++u) sl[u] = (j0 + u < n) ? z.glist[s0 + j0 + u] : -1;
+        for (int u = 0; u < FIN_L; ++u) {
+            ax[u] = sl[u] >= 0 ? z.vx[sl[u]] : 0.0;
+            ay[u] = sl[u] >= 0 ? z.vy[sl[u]] : 0.0;
+        }
+        for (int u = 0; u < FIN_L; ++u)
+            if (sl[u] >= 0) {
+                pbx += ax[u];
+                pby += ay[u];
+            }
+    }
+}
+
+template <class Z>
+ZHD void fin_write(Z& z, int sl, unsigned short ow, bool corr, double vx, double vy, double dux, double duy) {
+    if (!(ow & ZF_OWN)) return;
+    if (corr) {
+        vx += dux;
+        vy += duy;
+    }
+    if (ow & ZF_COVER) {
+        z.vsx[sl] = vx;
+        z.vsy[sl] = vy;
+    } else {
+        z.vx[sl] = vx;
+        z.vy[sl] = vy;
+    }
+}
+
+/* phase 2 for cell lc; returns |du|^2 (0 for a cell nothing changed) */
+template <class Z>
+ZHD double fin_cell(Z& z, int lc, double pbx, double pby) {
+    const int m = cnt_of(z, lc);
+    if (!fget(z, lc, ZF_CHG)) {
+        zone_correct_write(z, lc); /* the side-buffer copy of a covered cell */
+        return 0.0;
+    }
+    if (m == 0) return 0.0;
+    const int kb = (int)z.k0[lc], ke = kb + in_cnt(z, lc);
+    const bool mat = fget(z, lc, ZF_MAT);
+    const int t = tail_of(z, lc);
+    const unsigned short ownLc = (unsigned short)(z.fl[lc] & (ZF_OWN | ZF_COVER));
+    double px = 0.0, py = 0.0;
+    int kk[FIN_L], sl[FIN_L];
+    unsigned short ow[FIN_L];
+    double ax[FIN_L], ay[FIN_L];
+    int k = head_of(z, lc);
+    bool single = m <= FIN_L;
+    /* sums in list order; a short list keeps its entries */
+    while (k >= 0) {
+        for (int u = 0; u < FIN_L; ++u) {
+            kk[u] = k;
+            if (k >= 0) k = mat ? nx_get(z, k) : (k < t ? k + 1 : -1);
+        }
+        for (int u = 0; u < FIN_L; ++u) {
+            sl[u] = -1;
+            ow[u] = 0;
+            if (kk[u] >= 0) {
+                const int col = fin_col(z, lc, kb, ke, kk[u]);
+                sl[u] = slot_in(z, kk[u], col);
+                ow[u] = (kk[u] >= kb && kk[u] < ke) ? ownLc : own_bits(z, kk[u], col);
+            }
+        }
+        for (int u = 0; u < FIN_L; ++u) {
+            ax[u] = sl[u] >= 0 ? z.vx[sl[u]] : 0.0;
+            ay[u] = sl[u] >= 0 ? z.vy[sl[u]] : 0.0;
+        }
+        for (int u = 0; u < FIN_L; ++u)
+            if (sl[u] >= 0) {
+                px += ax[u];
+                py += ay[u];
+            }
+    }
+    const double dux = (pbx - px) / (double)m, duy = (pby - py) / (double)m;
+    bool corr = z.zp.corrEnabled != 0;
+    if (corr && fget(z, lc, ZF_SKIP)) corr = false;
+    if (corr && fget(z, lc, ZF_RECV) && !fget(z, lc, ZF_BB)) corr = false;
+    if (corr && !(dux != 0.0 || duy != 0.0)) corr = false;
+    if (single) {
+        for (int u = 0; u < FIN_L; ++u)
+            if (sl[u] >= 0) {
+                fin_write(z, sl[u], ow[u], corr, ax[u], ay[u], dux, duy);
+                if (z.rec[kk[u]] != ZM_NONE) nx_set(z, kk[u], lc);
+            }
+    } else {
+        k = head_of(z, lc);
+        while (k >= 0) {
+            for (int u = 0; u < FIN_L; ++u) {
+                kk[u] = k;
+                if (k >= 0) k = mat ? nx_get(z, k) : (k < t ? k + 1 : -1);
+            }
+            for (int u = 0; u < FIN_L; ++u) {
+                sl[u] = -1;
+                ow[u] = 0;
+                if (kk[u] >= 0) {
+                    const int col = fin_col(z, lc, kb, ke, kk[u]);
+                    sl[u] = slot_in(z, kk[u], col);
+                    ow[u] = (kk[u] >= kb && kk[u] < ke) ? ownLc : own_bits(z, kk[u], col);
+                }
+            }
+            for (int u = 0; u < FIN_L; ++u) {
+                ax[u] = sl[u] >= 0 ? z.vx[sl[u]] : 0.0;
+                ay[u] = sl[u] >= 0 ? z.vy[sl[u]] : 0.0;
+            }
+            for (int u = 0; u < FIN_L; ++u)
+                if (sl[u] >= 0) {
+                    fin_write(z, sl[u], ow[u], corr, ax[u], ay[u], dux, duy);
+                    if (z.rec[kk[u]] != ZM_NONE) nx_set(z, kk[u], lc);
+                }
+        }
+    }
+    return dux * dux + duy * duy;
+}
+
+/* phase 3, all threads of the zone: the new positions of the moved
+ * particles (links hold their final cells).  Each wavefront scans its share
+ * of the local indices and gathers the moved ones into a queue of one entry
+ * per lane, so each batch of draws runs once for up to 64 particles. */
+template <class Z>
+ZHD void fin_place_batch(Z& z, const int* queue, int n, int lane) {
+    if (lane < n) {
+        const int kq = queue[lane];
+        const int col = col_of(z, kq);
+        uint32_t pid;
+        const int slot = slot_pid_in(z, kq, col, pid);
+        if (own_bits(z, kq, col) & ZF_OWN) place_moved(z, nx_get(z, kq), slot, pid, z.rec[kq]);
+    }
+}
+template <class Z>
+ZHD void fin_place(Z& z, int tid, int nthr, int* queue) {
+    const int lane = tid & 63;
+    int nq = 0;
+    for (int b = tid - lane; b < z.P; b += nthr) {
+        const int k = b + lane;
+        const bool mv = k < z.P && z.rec[k] != ZM_NONE;
+        const unsigned long long msk = wave_ballot(mv);
+        const int add = wave_popc(msk);
+        if (add == 0) continue;
+        if (nq + add > 64) {
+            ZWSYNC();
+            fin_place_batch(z, queue, nq, lane);
+            ZWSYNC();
+            nq = 0;
+        }
+        if (mv) queue[nq + wave_popc(msk & lanes_below(lane))] = k;
+        nq += add;
+    }
+    ZWSYNC();
+    fin_place_batch(z, queue, nq, lane);
+}
+
+/* The stats of zone_correct from the per-cell |du|^2 in cell order (one
+ * wavefront: each chunk of nl cells added lane by lane). */
+template <class Z>
+ZHD void zone_correct_stats(const Z& z, const double* dsq, int lane, int nl, double& rms, double& mxo) {
+    double acc = 0.0, mx = 0.0;
+    int nDU = 0;
+    for (int b0 = 0; b0 < z.C; b0 += nl) {
+        const int lc = b0 + lane;
+        const double d = lc < z.C ? dsq[lc] : 0.0;
+        mx = dmax(mx, MPCD_SQRT(d));
+        unsigned long long pos = wave_ballot(d > 0.0);
+        nDU += wave_popc(pos);
+        while (pos) {
+            const int l = __builtin_ctzll(pos);
+            pos &= pos - 1ull;
+            acc += wave_readlane_d(d, l);
+        }
+    }
+    mxo = wave_max(mx);
+    rms = nDU > 0 ? MPCD_SQRT(acc / (double)nDU) : 0.0;
 }
 
 }  // namespace mpcdk

@@ -113,7 +113,8 @@ def _check(rc: int) -> None:
         raise MpcdError(rc, load().mpcd_gpu_last_error().decode())
 
 
-PROF_CLASSES = ("stream_bc", "bin", "collide", "cell_fields", "zone_pass", "zone_kernel", "closure", "kick")
+PROF_CLASSES = ("stream_bc", "bin", "collide", "cell_fields", "zone_pass", "zone_kernel", "closure", "kick",
+                "zone_finish")
 
 
 def device_synchronize() -> None:
