@@ -268,6 +268,7 @@ __global__ void __launch_bounds__(TPB) k_scatter(const int* __restrict__ cid, co
  * keys, stages the range in LDS and insertion-sorts each run (runs are short
  * and nearly sorted); a range too large for LDS is sorted in place in
  * global memory. */
+constexpr int STAGE_U = 16; /* entries per thread in flight in the staging loops */
 constexpr int SEG_CELLS = 256;
 constexpr int SEG_CAP = 6144;
 __global__ void __launch_bounds__(SEG_CELLS) k_segsort(const int* __restrict__ start, int Nc,
@@ -278,7 +279,17 @@ __global__ void __launch_bounds__(SEG_CELLS) k_segsort(const int* __restrict__ s
     const int b0 = start[c0], b1 = start[c1];
     const int c = c0 + (int)threadIdx.x;
     if (b1 - b0 <= SEG_CAP) {
-        for (int q = b0 + (int)threadIdx.x; q < b1; q += SEG_CELLS) sk[q - b0] = key[q];
+        for (int q0 = b0; q0 < b1; q0 += SEG_CELLS * STAGE_U) {
+            unsigned long long kv[STAGE_U];
+            for (int u = 0; u < STAGE_U; ++u) {
+                const int q = q0 + u * SEG_CELLS + (int)threadIdx.x;
+                kv[u] = q < b1 ? key[q] : 0ull;
+            }
+            for (int u = 0; u < STAGE_U; ++u) {
+                const int q = q0 + u * SEG_CELLS + (int)threadIdx.x;
+                if (q < b1) sk[q - b0] = kv[u];
+            }
+        }
         __syncthreads();
         if (c < c1) {
             const int b = start[c] - b0, e = start[c + 1] - b0;
@@ -323,6 +334,50 @@ __global__ void __launch_bounds__(SEG_CELLS) k_segsort(const int* __restrict__ s
  * larger than `cap` entries falls back to the global-memory loop. */
 constexpr int CELL_BLK = 128;
 
+/* Staging of a block's particle range list[b0 .. b0+n) to and from LDS.
+ * Each thread issues STAGE_U index loads, then STAGE_U gathers (or
+ * scatters), so a thread's accesses are in flight together rather than one
+ * dependent pair at a time. */
+__device__ inline void stage_in(const int* __restrict__ list, int b0, int n, const double* __restrict__ vx,
+                                const double* __restrict__ vy, double* svx, double* svy) {
+    const int tid = (int)threadIdx.x, nthr = (int)blockDim.x;
+    for (int q0 = 0; q0 < n; q0 += nthr * STAGE_U) {
+        int sl[STAGE_U];
+        for (int u = 0; u < STAGE_U; ++u) {
+            const int q = q0 + u * nthr + tid;
+            sl[u] = q < n ? list[b0 + q] : -1;
+        }
+        double ax[STAGE_U], ay[STAGE_U];
+        for (int u = 0; u < STAGE_U; ++u) {
+            ax[u] = sl[u] >= 0 ? vx[sl[u]] : 0.0;
+            ay[u] = sl[u] >= 0 ? vy[sl[u]] : 0.0;
+        }
+        for (int u = 0; u < STAGE_U; ++u)
+            if (sl[u] >= 0) {
+                const int q = q0 + u * nthr + tid;
+                svx[q] = ax[u];
+                svy[q] = ay[u];
+            }
+    }
+}
+__device__ inline void stage_out(const int* __restrict__ list, int b0, int n, double* __restrict__ vx,
+                                 double* __restrict__ vy, const double* svx, const double* svy) {
+    const int tid = (int)threadIdx.x, nthr = (int)blockDim.x;
+    for (int q0 = 0; q0 < n; q0 += nthr * STAGE_U) {
+        int sl[STAGE_U];
+        for (int u = 0; u < STAGE_U; ++u) {
+            const int q = q0 + u * nthr + tid;
+            sl[u] = q < n ? list[b0 + q] : -1;
+        }
+        for (int u = 0; u < STAGE_U; ++u)
+            if (sl[u] >= 0) {
+                const int q = q0 + u * nthr + tid;
+                vx[sl[u]] = svx[q];
+                vy[sl[u]] = svy[q];
+            }
+    }
+}
+
 __global__ void __launch_bounds__(CELL_BLK) k_collide(CollideParams cp, uint64_t seed, const int* __restrict__ start,
                                                       const int* __restrict__ list, int Nc, double* __restrict__ vx,
                                                       double* __restrict__ vy, int cap) {
@@ -334,22 +389,14 @@ __global__ void __launch_bounds__(CELL_BLK) k_collide(CollideParams cp, uint64_t
     if (b1 - b0 <= cap) {
         double* svx = sv;
         double* svy = sv + cap;
-        for (int q = (int)threadIdx.x; q < b1 - b0; q += CELL_BLK) {
-            const int slot = list[b0 + q];
-            svx[q] = vx[slot];
-            svy[q] = vy[slot];
-        }
+        stage_in(list, b0, b1 - b0, vx, vy, svx, svy);
         __syncthreads();
         if (c < c1) {
             const int o = start[c] - b0;
             collide_cell(cp, seed, c, nullptr, start[c + 1] - start[c], svx + o, svy + o);
         }
         __syncthreads();
-        for (int q = (int)threadIdx.x; q < b1 - b0; q += CELL_BLK) {
-            const int slot = list[b0 + q];
-            vx[slot] = svx[q];
-            vy[slot] = svy[q];
-        }
+        stage_out(list, b0, b1 - b0, vx, vy, svx, svy);
         return;
     }
     if (c >= c1) return;
@@ -424,11 +471,7 @@ __global__ void __launch_bounds__(CELL_BLK) k_cell_fields(const int* __restrict_
     if (b1 - b0 <= cap) {
         double* svx = sv;
         double* svy = sv + cap;
-        for (int q = (int)threadIdx.x; q < b1 - b0; q += CELL_BLK) {
-            const int slot = list[b0 + q];
-            svx[q] = vx[slot];
-            svy[q] = vy[slot];
-        }
+        stage_in(list, b0, b1 - b0, vx, vy, svx, svy);
         __syncthreads();
         if (c >= c1) return;
         const int o = start[c] - b0;
