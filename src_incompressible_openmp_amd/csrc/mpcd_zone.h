@@ -368,19 +368,23 @@ ZHD int count_at(const Z& z, int gx, int gy, int use_copy) {
     return z.gcnt[gy + z.g.Ny * gx];
 }
 
-/* 3x3 binomial smoothing without wrap (zone_pass.cpp:296-319). */
+/* 3x3 binomial smoothing without wrap (zone_pass.cpp:296-319).  The
+ * weights are multiples of 1/16 and the counts integers, so the weighted
+ * sums are exact: they are taken in integers (scaled by 16) and divided once,
+ * which is the same correctly rounded quotient. */
 template <class Z>
 ZHD double nsm_at(const Z& z, int gx, int gy, int use_copy) {
-    double acc = 0.0, ws = 0.0;
+    long long num = 0;
+    int den = 0;
     for (int dy = -1; dy <= 1; ++dy)
         for (int dx = -1; dx <= 1; ++dx) {
             const int jy = gy + dy, jx = gx + dx;
             if (jy < 0 || jy >= z.g.Ny || jx < 0 || jx >= z.g.Nx) continue;
-            const double K = (double)((dx == 0 ? 2 : 1) * (dy == 0 ? 2 : 1)) / 16.0; /* 1 2 1 / 2 4 2 / 1 2 1 */
-            acc += K * (double)count_at(z, jx, jy, use_copy);
-            ws += K;
+            const int w = (dx == 0 ? 2 : 1) * (dy == 0 ? 2 : 1); /* 1 2 1 / 2 4 2 / 1 2 1 */
+            num += (long long)w * count_at(z, jx, jy, use_copy);
+            den += w;
         }
-    return acc / dmax(ws, 1e-30);
+    return (double)num / (double)den;
 }
 
 /* Active thresholds of a zone cell (zone_pass.cpp:321-345, :715-725): the
@@ -608,6 +612,22 @@ struct ShufDraw {
  * the chunk's swaps in order, reading each partner from its lane. */
 template <class T>
 ZHD void zone_shuffle(T* a, int m, int lane, int nl, ShufDraw draw) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    if (m <= 64) {
+        /* element q in lane q's register; each swap reads the two values
+         * and writes them back by lane select */
+        int v = lane < m ? (int)a[lane] : 0;
+        const int j = (lane >= 1 && lane < m) ? (int)draw((uint32_t)lane) : 0;
+        for (int ii = m - 1; ii >= 1; --ii) {
+            const int jj = wave_readlane(j, ii);
+            const int vi = wave_readlane(v, ii), vj = wave_readlane(v, jj);
+            v = lane == ii ? vj : (lane == jj ? vi : v);
+        }
+        if (lane < m) a[lane] = (T)v;
+        ZBAR();
+        return;
+    }
+#endif
     for (int c0 = ((m - 1) / nl) * nl; c0 >= 0 && m > 1; c0 -= nl) {
         const int i = c0 + lane;
         const int j = (i >= 1 && i < m) ? (int)draw((uint32_t)i) : 0;
