@@ -205,7 +205,8 @@ struct ZoneCtxT {
     double* duG; /* velocity change per cell (zone's global region, 2 per cell) */
     unsigned char* rec; /* last move of each particle by local index (zone's global region, 1 per particle) */
     /* cell arrays */
-    IT *cnt, *head, *tail, *dl, *cntS, *k0;
+    IT *cnt, *head, *tail, *dl, *k0;
+    IT* ext; /* sparse phase: the counts it starts from, zone and ring, (zw+4) x (zh+4) column-major */
     unsigned short* fl;
     int *colL, *colG; /* per column: first local index, start in the binning list */
     double *lowA, *highA; /* local fluid-fraction thresholds (only with useLocal) */
@@ -359,7 +360,11 @@ ZHD int halo_of(const Z& z, int gx, int gy) {
 template <class Z>
 ZHD int count_at(const Z& z, int gx, int gy, int use_copy) {
     const int lc = zone_lc(z, gx, gy);
-    if (lc >= 0) return use_copy ? (int)z.cntS[lc] : (int)z.cnt[lc];
+    if (lc >= 0) {
+        if (!use_copy) return (int)z.cnt[lc];
+        const int ix = lc / z.zh;
+        return (int)z.ext[(ix + 2) * (z.zh + 4) + (lc - ix * z.zh) + 2];
+    }
     const int h = halo_of(z, gx, gy);
     if (h >= 0) {
         const int v = iget(z.halo[h]);
@@ -384,6 +389,46 @@ ZHD double nsm_at(const Z& z, int gx, int gy, int use_copy) {
             num += (long long)w * count_at(z, jx, jy, use_copy);
             den += w;
         }
+    return (double)num / (double)den;
+}
+
+/* nsm_at(.., use_copy = 1) for a point within one cell of the zone: the
+ * nine counts are direct reads of the extended array (no branch per
+ * neighbour); the same integer sums and quotient. */
+template <class Z>
+ZHD double nsm_ext(const Z& z, int gx, int gy) {
+    int rx = gx - z.xs, ry = gy - z.ys;
+    if (z.g.perx) {
+        if (rx < -2) rx += z.g.Nx;
+        else if (rx >= z.zw + 2) rx -= z.g.Nx;
+    }
+    if (z.g.pery) {
+        if (ry < -2) ry += z.g.Ny;
+        else if (ry >= z.zh + 2) ry -= z.g.Ny;
+    }
+    if (rx < -1 || rx > z.zw || ry < -1 || ry > z.zh) return nsm_at(z, gx, gy, 1);
+    const int H = z.zh + 4;
+    const int base = (rx + 2) * H + (ry + 2);
+    int v[9];
+    bool ok[9];
+    for (int dx = -1; dx <= 1; ++dx)
+        for (int dy = -1; dy <= 1; ++dy) {
+            const int t = (dx + 1) * 3 + (dy + 1);
+            const int jx = gx + dx, jy = gy + dy;
+            ok[t] = jx >= 0 && jx < z.g.Nx && jy >= 0 && jy < z.g.Ny;
+            v[t] = ok[t] ? iget(z.ext[base + dx * H + dy]) : 0;
+        }
+    long long num = 0;
+    int den = 0;
+    bool far = false;
+    for (int t = 0; t < 9; ++t) {
+        const int w = (t / 3 == 1 ? 2 : 1) * (t % 3 == 1 ? 2 : 1);
+        if (!ok[t]) continue;
+        if (v[t] < 0) far = true;
+        num += (long long)w * v[t];
+        den += w;
+    }
+    if (far) return nsm_at(z, gx, gy, 1); /* a ring count the index type could not hold */
     return (double)num / (double)den;
 }
 
@@ -1225,7 +1270,7 @@ ZHD void inward_dir(Z& z, int gx, int gy, int lane, int nl, double& dxo, double&
     for (int t = lane; t < 4; t += nl) {
         const int qx = (t == 0) ? xa : (t == 1) ? xb : gx;
         const int qy = (t == 2) ? ya : (t == 3) ? yb : gy;
-        z.nsm4[t] = nsm_at(z, qx, qy, 1);
+        z.nsm4[t] = nsm_ext(z, qx, qy);
     }
     ZBAR();
     dxo = (fx == 0.5) ? 0.5 * (z.nsm4[0] - z.nsm4[1]) : (z.nsm4[0] - z.nsm4[1]);
@@ -1288,9 +1333,14 @@ ZHD void donor_transfer(Z& z, int c, int d, int md, int nt, int cgc, T* tmp, int
 template <class Z>
 ZHD void zone_sparse_pass(Z& z, int lane, int nl) {
     typedef typename Z::idx_t IT;
-    for (int lc = lane; lc < z.C; lc += nl) {
-        zone_maps(z, lc, 0);
-        z.cntS[lc] = z.cnt[lc];
+    for (int lc = lane; lc < z.C; lc += nl) zone_maps(z, lc, 0);
+    {
+        const int H = z.zh + 4, ne = (z.zw + 4) * H;
+        for (int e = lane; e < ne; e += nl) {
+            const int rx = e / H - 2, ry = e % H - 2;
+            const int h = halo_slot(z, rx, ry);
+            z.ext[e] = h >= 0 ? z.halo[h] : z.cnt[rx * z.zh + ry];
+        }
     }
     ZBAR();
     const int m = zone_compact(z, lane, nl, ZC_LOW);
@@ -1348,7 +1398,7 @@ ZHD void zone_sparse_pass(Z& z, int lane, int nl) {
             double align = 0.0;
             if (ndir >= 1e-14 && nstep >= 1e-14) align = ((double)ddx * gxd + (double)ddy * gyd) / (ndir * nstep);
             const double noise = 1e-9 * mpcd_draw_donor_noise(z.seed, (uint32_t)cgc, (uint32_t)lc_gc(z, q));
-            z.dsc[t] = 100.0 * align + nsm_at(z, lc_gx(z, q), lc_gy(z, q), 1) + noise;
+            z.dsc[t] = 100.0 * align + nsm_ext(z, lc_gx(z, q), lc_gy(z, q)) + noise;
         }
         ZBAR();
         for (int t = lane; t < nd; t += nl) { /* stable, descending score */
@@ -1622,7 +1672,8 @@ ZHD size_t zone_fixed_bytes(int C, int W, int useLocal) {
          (size_t)ZONE_MAX_MOVE * sizeof(int); /* 8-byte items first */
     b += (size_t)W * 2 * sizeof(int);                           /* colL colG */
     b += (5 * 24 + 3 * 8) * sizeof(int);                        /* ord nbGc rnk recvT (+pad) dord dord2 (+pad) */
-    b += (size_t)C * 6 * sizeof(IT);                            /* cnt head tail dl cntS k0 */
+    b += (size_t)C * 5 * sizeof(IT);                            /* cnt head tail dl k0 */
+    b += (size_t)(W + 4) * (size_t)(C / (W > 0 ? W : 1) + 4) * sizeof(IT); /* ext */
     b += (size_t)(4 * (W + 4) + 4 * (C / (W > 0 ? W : 1))) * sizeof(IT); /* halo */
     b += (size_t)C * sizeof(unsigned short); /* fl */
     b += (size_t)ZONE_TMP_L * sizeof(IT);                       /* tmpL */
@@ -1660,7 +1711,7 @@ ZHD void zone_carve(ZoneCtxT<IT>& z, unsigned char* base, size_t bytes) {
     z.head = (IT*)p; p += (size_t)C * sizeof(IT);
     z.tail = (IT*)p; p += (size_t)C * sizeof(IT);
     z.dl = (IT*)p; p += (size_t)C * sizeof(IT);
-    z.cntS = (IT*)p; p += (size_t)C * sizeof(IT);
+    z.ext = (IT*)p; p += (size_t)(z.zw + 4) * (size_t)(z.zh + 4) * sizeof(IT);
     z.k0 = (IT*)p; p += (size_t)C * sizeof(IT);
     z.halo = (IT*)p; p += (size_t)halo_size(z) * sizeof(IT);
     z.tmpL = (IT*)p; p += (size_t)ZONE_TMP_L * sizeof(IT);
