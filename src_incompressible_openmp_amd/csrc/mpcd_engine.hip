@@ -175,14 +175,29 @@ __device__ inline void wave_groups(int c, bool active, int lane, int& leader, in
     }
 }
 
+/* The binning kernels give each wavefront BIN_ITEMS consecutive chunks of 64
+ * keys (lanes on consecutive keys, so a chunk holds a few cells) and issue
+ * every chunk's loads before the first use. */
+constexpr int BIN_ITEMS = 4;
+__host__ __device__ inline int bin_blocks(int n) {
+    const int per = TPB * BIN_ITEMS;
+    return (n + per - 1) / per;
+}
+
 /* Cell histogram: one atomic per key group, all groups in one instruction. */
 __global__ void __launch_bounds__(TPB) k_hist(const int* __restrict__ cid, int n, int* __restrict__ cnt) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int lane = __lane_id();
-    const int c = (i < n) ? cid[i] : -1;
-    int leader, rank, size;
-    wave_groups(c, c >= 0, lane, leader, rank, size);
-    if (c >= 0 && lane == leader) atomicAdd(&cnt[c], size);
+    const int w0 = (blockIdx.x * TPB + (int)threadIdx.x - lane) * BIN_ITEMS; /* the wavefront's first key */
+    int c[BIN_ITEMS];
+    for (int r = 0; r < BIN_ITEMS; ++r) {
+        const int i = w0 + r * 64 + lane;
+        c[r] = (i < n) ? cid[i] : -1;
+    }
+    for (int r = 0; r < BIN_ITEMS; ++r) {
+        int leader, rank, size;
+        wave_groups(c[r], c[r] >= 0, lane, leader, rank, size);
+        if (c[r] >= 0 && lane == leader) atomicAdd(&cnt[c[r]], size);
+    }
 }
 
 /* exclusive scan, level 1: each block scans SCAN_TILE entries */
@@ -248,18 +263,27 @@ __global__ void k_scan_add(int* __restrict__ out, int m, const int* __restrict__
 __global__ void __launch_bounds__(TPB) k_scatter(const int* __restrict__ cid, const int* __restrict__ pid, int n,
                                                  const int* __restrict__ start, int* __restrict__ cursor,
                                                  unsigned long long* __restrict__ key) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int lane = __lane_id();
-    const bool in = i < n;
-    const int c = in ? cid[i] : -1;
-    const unsigned p = in ? (unsigned)pid[i] : 0u;
-    const int s0 = (c >= 0) ? start[c] : 0;
-    int leader, rank, size;
-    wave_groups(c, c >= 0, lane, leader, rank, size);
-    int old = 0;
-    if (c >= 0 && lane == leader) old = atomicAdd(&cursor[c], size);
-    const int base = __shfl(old, leader < 0 ? lane : leader, 64);
-    if (c >= 0) key[s0 + base + rank] = ((unsigned long long)p << 32) | (unsigned)i;
+    const int w0 = (blockIdx.x * TPB + (int)threadIdx.x - lane) * BIN_ITEMS;
+    int c[BIN_ITEMS], s0[BIN_ITEMS], leader[BIN_ITEMS], rank[BIN_ITEMS], old[BIN_ITEMS];
+    unsigned p[BIN_ITEMS];
+    for (int r = 0; r < BIN_ITEMS; ++r) {
+        const int i = w0 + r * 64 + lane;
+        c[r] = (i < n) ? cid[i] : -1;
+        p[r] = (i < n) ? (unsigned)pid[i] : 0u;
+    }
+    for (int r = 0; r < BIN_ITEMS; ++r) s0[r] = (c[r] >= 0) ? start[c[r]] : 0;
+    for (int r = 0; r < BIN_ITEMS; ++r) {
+        int size;
+        wave_groups(c[r], c[r] >= 0, lane, leader[r], rank[r], size);
+        old[r] = 0;
+        if (c[r] >= 0 && lane == leader[r]) old[r] = atomicAdd(&cursor[c[r]], size);
+    }
+    for (int r = 0; r < BIN_ITEMS; ++r) {
+        const int base = __shfl(old[r], leader[r] < 0 ? lane : leader[r], 64);
+        const int i = w0 + r * 64 + lane;
+        if (c[r] >= 0) key[s0[r] + base + rank[r]] = ((unsigned long long)p[r] << 32) | (unsigned)i;
+    }
 }
 
 /* Order each cell's run by pid; writes the sorted keys back and the slots
@@ -1259,11 +1283,11 @@ struct mpcd_gpu_sim {
         /* fills by kernel, not memset, so the sequence can be graph-captured */
         hipLaunchKernelGGL(k_fill_int, dim3(nblk(Nc + 1)), dim3(TPB), 0, stream, bin.cnt, Nc + 1, 0);
         hipLaunchKernelGGL(k_fill_int, dim3(nblk(Nc + 1)), dim3(TPB), 0, stream, bin.cursor, Nc + 1, 0);
-        hipLaunchKernelGGL(k_hist, dim3(nblk(n)), dim3(TPB), 0, stream, key, n, bin.cnt);
+        hipLaunchKernelGGL(k_hist, dim3(bin_blocks(n)), dim3(TPB), 0, stream, key, n, bin.cnt);
         hipLaunchKernelGGL(k_scan_tiles, dim3(nTiles), dim3(TPB), 0, stream, bin.cnt, Nc + 1, bin.start, tileSum);
         hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(TPB), 0, stream, tileSum, nTiles);
         hipLaunchKernelGGL(k_scan_add, dim3(nblk(Nc + 1)), dim3(TPB), 0, stream, bin.start, Nc + 1, tileSum);
-        hipLaunchKernelGGL(k_scatter, dim3(nblk(n)), dim3(TPB), 0, stream, key, pid, n, bin.start, bin.cursor,
+        hipLaunchKernelGGL(k_scatter, dim3(bin_blocks(n)), dim3(TPB), 0, stream, key, pid, n, bin.start, bin.cursor,
                            bin.key);
         hipLaunchKernelGGL(k_segsort, dim3(nblk(Nc, SEG_CELLS)), dim3(SEG_CELLS), 0, stream, bin.start, Nc, bin.key,
                            bin.list);
