@@ -356,7 +356,7 @@ __global__ void __launch_bounds__(SEG_CELLS) k_segsort(const int* __restrict__ s
  * loads; each thread then runs its cell's sequential sums (particle order,
  * so the same rounding as one pass over global memory) on LDS.  A range
  * larger than `cap` entries falls back to the global-memory loop. */
-constexpr int CELL_BLK = 128;
+constexpr int CELL_BLK = 64;
 
 /* Staging of a block's particle range list[b0 .. b0+n) to and from LDS.
  * Each thread issues STAGE_U index loads, then STAGE_U gathers (or
@@ -1162,9 +1162,11 @@ struct mpcd_gpu_sim {
         cp = make_collide_params(p);
         zp = make_zone_params(p);
         {
-            /* 1.25x the mean particles of a block plus slack, at most 64 KB */
+            /* 1.15x the mean particles of a block plus slack, at most 64 KB: the
+             * LDS per block sets how many blocks a CU holds; a block above
+             * the cap takes the global-memory path */
             const double mean = (double)n / (double)std::max(Nc, 1) * CELL_BLK;
-            cellCap = (int)std::min(4096.0, std::max(256.0, 1.25 * mean + 8.0 * CELL_BLK));
+            cellCap = (int)std::min(4096.0, std::max(256.0, 1.15 * mean + 2.0 * CELL_BLK));
         }
         /* layouts (zones of the shifted layout overlap: a cell is covered by
          * up to four zones, so per-zone particle scratch is sized by the
