@@ -35,12 +35,17 @@ from src_incompressible_openmp_amd.params import poiseuille_params, to_struct, w
 
 METRIC = "particle-steps/s plain vs learned-closure at fixed Poiseuille profile error"
 HBM_PEAK_GBS = 8000.0
-# algorithmic FP64 bytes per particle of each kernel class (launch covers all particles)
-ALG_BYTES = {"stream_bc": 64, "collide": 48, "zone_kernel": 48, "kick": 48}
+# compulsory bytes per particle of each kernel class (a launch covers all particles):
+# stream_bc x,y,vx,vy read+write; collide vx,vy read+write + slot list (x2: in, out)
+# as staged; zone_kernel (the passes) pass-input counts and ring per cell, the
+# move record and the link/state hand-over per particle (about 4 B); zone_finish
+# (the correction) slot list and vx,vy read, vx,vy written, record read (40 B);
+# kick x,y,vx,vy,pid gathered and written in red-cell order
+ALG_BYTES = {"stream_bc": 64, "collide": 48, "zone_kernel": 4, "zone_finish": 40, "kick": 48}
 STEP_ALG_BYTES = 256  # compulsory particle traffic of one plain step
 # kernel of each timed class, as rocprofv3 names it
 KERNEL_NAME = {"stream_bc": "k_stream_bc", "collide": "k_collide", "zone_kernel": "k_zone<unsigned short, true>",
-               "kick": "k_kick_reorder"}
+               "zone_finish": "k_zone_finish<unsigned short, true>", "kick": "k_kick_reorder"}
 
 
 def log(*a):
@@ -258,6 +263,12 @@ def main() -> int:
         out["roofline"] = {"bound": "hbm", "kernel": dom, "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                            "frac": ach / HBM_PEAK_GBS, "traffic": traffic,
                            "alg_bytes_per_launch": alg, "ms_per_launch": per[dom]["ms_per_launch"]}
+        # every particle kernel class against the same bound, for comparison
+        out["kernel_roofline"] = {
+            k: {"ms_per_launch": per[k]["ms_per_launch"],
+                "achieved_GBs": ALG_BYTES[k] * n / (per[k]["ms_per_launch"] * 1e-3) / 1e9,
+                "frac": ALG_BYTES[k] * n / (per[k]["ms_per_launch"] * 1e-3) / 1e9 / HBM_PEAK_GBS}
+            for k in ALG_BYTES if per[k]["launches"]}
     if learned:
         out["learned"] = learned
         out["learned"]["speedup_vs_plain"] = learned["value"] / value

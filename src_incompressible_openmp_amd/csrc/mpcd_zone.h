@@ -7,13 +7,16 @@
  * each of which mutates the counts its successors read.  That order is kept;
  * inside each step the wavefront works together wherever the work is
  * independent by construction: all counter-based draws of a step (shuffle
- * partners, tie breaks, score noise, new positions), candidate evaluation and
- * ranking, order-preserving compactions and prefix sums, the momentum
- * correction.  Only list bookkeeping runs on lane 0, and it touches LDS only:
- * the slots, ownership and new positions of the particles it moves are
- * staged by all lanes beforehand.  The same code runs on the host with one
- * lane (nl = 1), where the wave primitives reduce to scalars and the barrier
- * to nothing.
+ * partners, tie breaks, score noise), candidate evaluation and ranking,
+ * order-preserving compactions and prefix sums.  On the device a dense cell's
+ * candidates and movers are held in lanes and its spread is applied by the
+ * lanes at once (dense_cell_wave); the one-lane form of the same functions
+ * runs on the host (nl = 1), where the wave primitives reduce to scalars and
+ * the barrier to nothing.  A move only records, per particle, the dense pass
+ * or the sparse phase that moved it last; the momentum correction, which
+ * visits every particle in its final cell, writes the new positions from
+ * that record.  On the device the correction runs in a kernel of its own
+ * (the zone's state handed over in global memory, see ZoneState).
  *
  * Memory.  Cell arrays are indexed by the zone-local cell
  * lc = (ix-xs)*zh + (iy-ys) (ascending global cell id); particles by the
