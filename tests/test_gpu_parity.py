@@ -277,3 +277,68 @@ def test_medium_grid_step(gpu, oracle):
     for a in ("ref", "base", "red"):
         assert bits(sg["x_" + a], so["x_" + a]) and bits(sg["v_" + a], so["v_" + a]), a
     assert bits(xg, xo) and vclose(vg, vo)
+
+
+def wide_params(**kw):
+    from src_incompressible_openmp_amd.params import poiseuille_params, to_struct
+
+    return to_struct(poiseuille_params(**kw))
+
+
+def wide_post_collision(oracle, p, seed):
+    from src_incompressible_openmp_amd.fileio import uniform_state
+
+    x, v, _, _ = uniform_state(p.n, p.Lx, p.Ly, p.kBT, seed=seed)
+    # a checkerboard of occupancy: particles of the "even" cells leave with
+    # probability 0.7 for anywhere in the domain, so even cells run low next
+    # to fuller odd ones (sparse transfers as well as dense spreads)
+    rng = np.random.default_rng(seed)
+    px, py = x[0::2], x[1::2]
+    even = ((np.floor(px / (p.Lx / p.Nx)) + np.floor(py / (p.Ly / p.Ny))) % 2) == 0
+    go = even & (rng.random(p.n) < 0.7)
+    px[go] = rng.random(int(go.sum())) * p.Lx
+    py[go] = rng.random(int(go.sum())) * p.Ly
+    x[0::2], x[1::2] = px, py
+    oracle.force_stream(p, x, v)
+    oracle.apply_bc(CTR, p, x, v, 104729)
+    cid, _ = oracle.shift_cell_ids(CTR, p, x, 130363)
+    oracle.collide(CTR, p, v, cid, 433494437)
+    return x, v
+
+
+@pytest.mark.parametrize("passes", [0, 5])
+def test_wide_single_zone(gpu, oracle, passes):
+    """A zone of 3072 cells: its working set does not fit the 16-bit LDS
+    form, so the pass kernel runs with 32-bit indices in global scratch and
+    the correction kernel takes its two-sweep form.  Without dense passes
+    the sparse phase does all the work."""
+    p = wide_params(Nx=64, Ny=48, gamma=8.0, zoneSingleFullDomainMode=True, maxRedistribPasses=passes)
+    x, v = wide_post_collision(oracle, p, 11)
+    _, rects = gpu.build_zone_layout(p, LAYOUT_BASE)
+    assert len(rects) == 1
+    xo, vo = x.copy(), v.copy()
+    st, _, mv = oracle.zone_kernel(CTR, p, xo, vo, LAYOUT_BASE, 0, 9011)
+    xg, vg = x.copy(), v.copy()
+    sg, mg = gpu.run_zone_kernel(p, xg, vg, rects[0], 9011)
+    print("wide single zone moved", mv, flush=True)
+    assert np.array_equal(mg, mv)
+    assert bits(xg, xo) and bits(vg, vo)
+    assert bits(sg, st)
+
+
+@pytest.mark.parametrize("layout", [LAYOUT_BASE, LAYOUT_SHIFTED])
+def test_wide_zone_pass(gpu, oracle, layout):
+    """Zones of 60x48 cells in both layouts (the shifted one overlapping):
+    the 32-bit working-set paths of both zone kernels in a snapshot pass."""
+    p = wide_params(Nx=120, Ny=48, gamma=6.0, zoneTileNx=60, zoneTileNy=48, maxRedistribPasses=1)
+    x, v = wide_post_collision(oracle, p, 12)
+    xo, vo = x.copy(), v.copy()
+    mo, owo = oracle.zone_pass(CTR, p, xo, vo, layout, 88172645463325252)
+    xg, vg = x.copy(), v.copy()
+    mg, owg = gpu.zone_pass(p, xg, vg, layout, 88172645463325252)
+    print("wide zone pass moved", mo["nParticlesMovedDense"], mo["nParticlesMovedSparse"], flush=True)
+    assert np.array_equal(owg, owo)
+    assert mg["nParticlesMovedDense"] == mo["nParticlesMovedDense"]
+    assert mg["nParticlesMovedSparse"] == mo["nParticlesMovedSparse"]
+    assert bits(xg, xo) and bits(vg, vo)
+    assert mg["corrRmsDU"] == mo["corrRmsDU"] and mg["corrMaxDU"] == mo["corrMaxDU"]
