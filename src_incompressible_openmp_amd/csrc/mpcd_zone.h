@@ -1109,6 +1109,17 @@ ZHD void spread_apply_wave(Z& z, int src, bool srcMat, int lane, int q0, int mov
         mark_moved(z, myK, pass);
         nx_set(z, myK, nextQ >= 0 ? nk : -1);
     }
+    /* targets whose lists are still implicit runs: their links written by
+     * all lanes at once, one target at a time */
+    for (unsigned long long nm = wave_ballot(firstQ >= 0 && !fget(z, lcP, ZF_MAT)); nm; nm &= nm - 1ull) {
+        const int lt = wave_readlane(lcP, (int)__builtin_ctzll(nm));
+        const int h = head_of(z, lt), t = tail_of(z, lt);
+        if (h >= 0)
+            for (int k = h + lane; k <= t; k += 64) nx_set(z, k, k < t ? k + 1 : -1);
+        ZWSYNC();
+        if (lane == 0) fset(z, lt, ZF_MAT);
+        ZWSYNC();
+    }
     if (firstQ >= 0) {
         ensure_mat(z, lcP);
         const int t = tail_of(z, lcP);
