@@ -553,6 +553,18 @@ ZHD void ensure_mat(Z& z, int lc) {
     fset(z, lc, ZF_MAT);
 }
 
+/* ensure_mat by all lanes (lc the same on every lane) */
+template <class Z>
+ZHD void ensure_mat_lanes(Z& z, int lc, int lane, int nl) {
+    if (fget(z, lc, ZF_MAT)) return;
+    const int h = head_of(z, lc), t = tail_of(z, lc);
+    if (h >= 0)
+        for (int k = h + lane; k <= t; k += nl) nx_set(z, k, k < t ? k + 1 : -1);
+    ZBAR();
+    if (lane == 0) fset(z, lc, ZF_MAT);
+    ZBAR();
+}
+
 /* requires a materialised list */
 template <class Z>
 ZHD void list_append(Z& z, int lc, int k) {
@@ -1323,8 +1335,8 @@ ZHD void donor_transfer(Z& z, int c, int d, int md, int nt, int cgc, T* tmp, int
             nx_set(z, kq, q + 1 < md ? (int)tmp[q + 1] : -1);
         }
     }
+    ensure_mat_lanes(z, c, lane, nl);
     if (lane == 0) {
-        ensure_mat(z, c);
         const int tc = tail_of(z, c);
         if (tc < 0) head_set(z, c, (int)tmp[0]);
         else nx_set(z, tc, (int)tmp[0]);
