@@ -294,6 +294,7 @@ __global__ void __launch_bounds__(TPB) k_scatter(const int* __restrict__ cid, co
  * global memory. */
 constexpr int STAGE_U = 16; /* entries per thread in flight in the staging loops */
 constexpr int SEG_CELLS = 256;
+constexpr int SORT_NET = 32; /* runs up to this length are sorted by a network on registers */
 constexpr int SEG_CAP = 6144;
 __global__ void __launch_bounds__(SEG_CELLS) k_segsort(const int* __restrict__ start, int Nc,
                                                        unsigned long long* __restrict__ key, int* __restrict__ list) {
@@ -317,14 +318,40 @@ __global__ void __launch_bounds__(SEG_CELLS) k_segsort(const int* __restrict__ s
         __syncthreads();
         if (c < c1) {
             const int b = start[c] - b0, e = start[c + 1] - b0;
-            for (int i = b + 1; i < e; ++i) {
-                const unsigned long long kv = sk[i];
-                int j = i - 1;
-                while (j >= b && sk[j] > kv) {
-                    sk[j + 1] = sk[j];
-                    --j;
+            if (e - b <= SORT_NET) {
+                /* a bitonic network on registers (static indices), the
+                 * run padded with the largest key */
+                unsigned long long v[SORT_NET];
+#pragma unroll
+                for (int u = 0; u < SORT_NET; ++u) v[u] = b + u < e ? sk[b + u] : ~0ull;
+#pragma unroll
+                for (int k = 2; k <= SORT_NET; k <<= 1)
+#pragma unroll
+                    for (int j = k >> 1; j > 0; j >>= 1)
+#pragma unroll
+                        for (int i = 0; i < SORT_NET; ++i) {
+                            const int l = i ^ j;
+                            if (l > i) {
+                                const unsigned long long a0 = v[i], a1 = v[l];
+                                const bool up = (i & k) == 0;
+                                const bool sw = up ? (a0 > a1) : (a0 < a1);
+                                v[i] = sw ? a1 : a0;
+                                v[l] = sw ? a0 : a1;
+                            }
+                        }
+#pragma unroll
+                for (int u = 0; u < SORT_NET; ++u)
+                    if (b + u < e) sk[b + u] = v[u];
+            } else {
+                for (int i = b + 1; i < e; ++i) {
+                    const unsigned long long kv = sk[i];
+                    int j = i - 1;
+                    while (j >= b && sk[j] > kv) {
+                        sk[j + 1] = sk[j];
+                        --j;
+                    }
+                    sk[j + 1] = kv;
                 }
-                sk[j + 1] = kv;
             }
         }
         __syncthreads();
