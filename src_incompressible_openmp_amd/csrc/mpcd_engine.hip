@@ -287,17 +287,19 @@ __global__ void __launch_bounds__(TPB) k_scatter(const int* __restrict__ cid, co
 }
 
 /* Order each cell's run by pid; writes the sorted keys back and the slots
- * to the list.  A block takes
- * SEG_CELLS consecutive cells, whose runs are one contiguous range of the
- * keys, stages the range in LDS and insertion-sorts each run (runs are short
- * and nearly sorted); a range too large for LDS is sorted in place in
+ * to the list.  A block takes SEG_CELLS consecutive cells, whose runs are one
+ * contiguous range of the keys, and stages the range in LDS; a run of up to
+ * SORT_NET keys is sorted by a bitonic network on registers (on 32-bit
+ * values pid << 5 | index when every pid fits 27 bits, else on the keys), a
+ * longer one by insertion; a range too large for LDS is sorted in place in
  * global memory. */
 constexpr int STAGE_U = 16; /* entries per thread in flight in the staging loops */
 constexpr int SEG_CELLS = 256;
 constexpr int SORT_NET = 32; /* runs up to this length are sorted by a network on registers */
 constexpr int SEG_CAP = 6144;
 __global__ void __launch_bounds__(SEG_CELLS) k_segsort(const int* __restrict__ start, int Nc,
-                                                       unsigned long long* __restrict__ key, int* __restrict__ list) {
+                                                       unsigned long long* __restrict__ key, int* __restrict__ list,
+                                                       int narrow) {
     __shared__ unsigned long long sk[SEG_CAP];
     const int c0 = blockIdx.x * SEG_CELLS;
     const int c1 = min(c0 + SEG_CELLS, Nc);
@@ -318,7 +320,35 @@ __global__ void __launch_bounds__(SEG_CELLS) k_segsort(const int* __restrict__ s
         __syncthreads();
         if (c < c1) {
             const int b = start[c] - b0, e = start[c + 1] - b0;
-            if (e - b <= SORT_NET) {
+            if (e - b <= SORT_NET && narrow) {
+                /* 32-bit network values (pid << 5 | index in the run), the
+                 * run padded with the largest value (larger than any real
+                 * one, as pid < 2^27 - 1); the keys then gathered by index */
+                const unsigned* hiw = reinterpret_cast<const unsigned*>(sk) + 1;
+                unsigned v[SORT_NET];
+#pragma unroll
+                for (int u = 0; u < SORT_NET; ++u) v[u] = b + u < e ? (hiw[2 * (b + u)] << 5) | (unsigned)u : ~0u;
+#pragma unroll
+                for (int k = 2; k <= SORT_NET; k <<= 1)
+#pragma unroll
+                    for (int j = k >> 1; j > 0; j >>= 1)
+#pragma unroll
+                        for (int i = 0; i < SORT_NET; ++i) {
+                            const int l = i ^ j;
+                            if (l > i) {
+                                const unsigned lo = min(v[i], v[l]), hi = max(v[i], v[l]);
+                                const bool up = (i & k) == 0;
+                                v[i] = up ? lo : hi;
+                                v[l] = up ? hi : lo;
+                            }
+                        }
+                unsigned long long kv[SORT_NET];
+#pragma unroll
+                for (int u = 0; u < SORT_NET; ++u) kv[u] = b + u < e ? sk[b + (int)(v[u] & 31u)] : 0ull;
+#pragma unroll
+                for (int u = 0; u < SORT_NET; ++u)
+                    if (b + u < e) sk[b + u] = kv[u];
+            } else if (e - b <= SORT_NET) {
                 /* a bitonic network on registers (static indices), the
                  * run padded with the largest key */
                 unsigned long long v[SORT_NET];
@@ -1319,7 +1349,7 @@ struct mpcd_gpu_sim {
         hipLaunchKernelGGL(k_scatter, dim3(bin_blocks(n)), dim3(TPB), 0, stream, key, pid, n, bin.start, bin.cursor,
                            bin.key);
         hipLaunchKernelGGL(k_segsort, dim3(nblk(Nc, SEG_CELLS)), dim3(SEG_CELLS), 0, stream, bin.start, Nc, bin.key,
-                           bin.list);
+                           bin.list, (int)(n < (1 << 27)));
         check_launch();
     }
     void bin_unshifted() {
