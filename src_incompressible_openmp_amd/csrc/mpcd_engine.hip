@@ -202,13 +202,14 @@ __global__ void __launch_bounds__(TPB) k_hist(const int* __restrict__ cid, int n
 
 /* exclusive scan, level 1: each block scans SCAN_TILE entries */
 __global__ void __launch_bounds__(TPB) k_scan_tiles(const int* __restrict__ in, int m, int* __restrict__ out,
-                                                    int* __restrict__ tileSum) {
+                                                    int* __restrict__ tileSum, const int* __restrict__ in2 = nullptr) {
+    /* in2 != nullptr: scans in + in2 */
     __shared__ int s[TPB];
     const int base = blockIdx.x * SCAN_TILE + threadIdx.x * 4;
     int v[4];
     int tot = 0;
     for (int k = 0; k < 4; ++k) {
-        v[k] = (base + k < m) ? in[base + k] : 0;
+        v[k] = (base + k < m) ? in[base + k] + (in2 ? in2[base + k] : 0) : 0;
         tot += v[k];
     }
     s[threadIdx.x] = tot;
@@ -404,6 +405,165 @@ __global__ void __launch_bounds__(SEG_CELLS) k_segsort(const int* __restrict__ s
         key[j + 1] = kv;
     }
     for (int i = b; i < e; ++i) list[i] = (int)(unsigned)(key[i] & 0xffffffffull);
+}
+
+/* ---- incremental unshifted binning ---- */
+
+/* The new cell of each slot against the binned one: a particle that
+ * changed cell leaves its old cell's count and takes a rank among its new
+ * cell's arrivals (rnk = -1 for a stayer). */
+__global__ void k_cells_delta(Geom g, const double* __restrict__ x, const double* __restrict__ y,
+                              int* __restrict__ cid, int* __restrict__ cnt, int* __restrict__ inc,
+                              int* __restrict__ rnk) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= g.n) return;
+    const int cn = cell_of(g, x[i], y[i]);
+    const int co = cid[i];
+    int r = -1;
+    if (cn != co) {
+        cid[i] = cn;
+        atomicSub(&cnt[co], 1);
+        r = atomicAdd(&inc[cn], 1);
+    }
+    rnk[i] = r;
+}
+
+/* Arrivals go to the tail of their new cell's run (after the stayers), in
+ * arrival order; the merge sorts them. */
+__global__ void k_place_arrivals(const int* __restrict__ cid, const int* __restrict__ pid,
+                                 const int* __restrict__ rnk, int n, const int* __restrict__ startB,
+                                 const int* __restrict__ stay, unsigned long long* __restrict__ keyB) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int r = rnk[i];
+    if (r < 0) return;
+    const int c = cid[i];
+    keyB[startB[c] + stay[c] + r] = ((unsigned long long)(unsigned)pid[i] << 32) | (unsigned)i;
+}
+
+constexpr int REB_CELLS = 256; /* threads per block of the merge */
+constexpr int REB_CAP = 6144;  /* keys of a block's new runs staged in LDS */
+constexpr int REB_REG = 32;    /* old-run keys loaded per batch */
+constexpr int REB_IN = 8;      /* arrivals merged on registers (more are inserted) */
+
+/* One cell's new run: its old run's stayers (still in this cell, in pid
+ * order) merged with its arrivals (the tail of the new run in keyB).  The
+ * old run is read in batches of REB_REG keys, their cells gathered at once;
+ * up to REB_IN arrivals are sorted on registers and each stayer's position
+ * counts the arrivals below it.  out[o..o+m) receives the run; the
+ * arrivals are read before out is written (out may be keyB's run). */
+__device__ __forceinline__ void rebin_cell(int c, int b, int mo, int nb, int ms, int ki,
+                                           const unsigned long long* key, const int* cid,
+                                           const unsigned long long* keyB, unsigned long long* out, int o,
+                                           int* err) {
+    const bool regIn = ki <= REB_IN;
+    unsigned long long a[REB_IN];
+#pragma unroll
+    for (int j = 0; j < REB_IN; ++j) a[j] = (regIn && j < ki) ? keyB[nb + ms + j] : ~0ull;
+#pragma unroll
+    for (int k = 2; k <= REB_IN; k <<= 1)
+#pragma unroll
+        for (int j = k >> 1; j > 0; j >>= 1)
+#pragma unroll
+            for (int i = 0; i < REB_IN; ++i) {
+                const int l = i ^ j;
+                if (l > i) {
+                    const unsigned long long a0 = a[i], a1 = a[l];
+                    const bool up = (i & k) == 0;
+                    const bool sw = up ? (a0 > a1) : (a0 < a1);
+                    a[i] = sw ? a1 : a0;
+                    a[l] = sw ? a0 : a1;
+                }
+            }
+    int before = 0;
+    int below[REB_IN]; /* stayers below each arrival */
+#pragma unroll
+    for (int j = 0; j < REB_IN; ++j) below[j] = 0;
+    for (int u0 = 0; u0 < mo; u0 += REB_REG) {
+        unsigned long long v[REB_REG];
+        int sc[REB_REG];
+#pragma unroll
+        for (int u = 0; u < REB_REG; ++u) v[u] = u0 + u < mo ? key[b + u0 + u] : 0ull;
+#pragma unroll
+        for (int u = 0; u < REB_REG; ++u) sc[u] = u0 + u < mo ? cid[(int)(unsigned)v[u]] : -1;
+#pragma unroll
+        for (int u = 0; u < REB_REG; ++u) {
+            if (sc[u] == c) {
+                int pos = before;
+#pragma unroll
+                for (int j = 0; j < REB_IN; ++j) {
+                    const bool lt = a[j] < v[u];
+                    pos += lt ? 1 : 0;
+                    below[j] += lt ? 0 : 1;
+                }
+                if (before < ms) out[o + pos] = v[u];
+                ++before;
+            }
+        }
+    }
+    if (before != ms) {
+        atomicOr(err, 4);
+        return;
+    }
+    if (regIn) {
+#pragma unroll
+        for (int j = 0; j < REB_IN; ++j)
+            if (j < ki) out[o + j + below[j]] = a[j];
+        return;
+    }
+    /* many arrivals: appended after the stayers and inserted */
+    if (out + o != keyB + nb)
+        for (int j = 0; j < ki; ++j) out[o + ms + j] = keyB[nb + ms + j];
+    for (int i = o + ms; i < o + ms + ki; ++i) {
+        const unsigned long long k = out[i];
+        int j = i - 1;
+        while (j >= o && out[j] > k) {
+            out[j + 1] = out[j];
+            --j;
+        }
+        out[j + 1] = k;
+    }
+}
+
+/* A block takes cpb (<= REB_CELLS) consecutive cells; their new runs are
+ * one contiguous range, assembled in LDS and written out coalesced (keys
+ * and slots), or written in place when the range exceeds REB_CAP.  Resets
+ * cnt to the new counts. */
+__global__ void __launch_bounds__(REB_CELLS) k_rebin_merge(const int* __restrict__ start,
+                                                           const unsigned long long* __restrict__ key,
+                                                           const int* __restrict__ cid, int Nc, int cpb,
+                                                           const int* __restrict__ startB, int* __restrict__ cnt,
+                                                           const int* __restrict__ inc, unsigned long long* keyB,
+                                                           int* __restrict__ listB, int* __restrict__ err) {
+    __shared__ unsigned long long so[REB_CAP];
+    const int c0 = blockIdx.x * cpb;
+    const int c1 = min(c0 + cpb, Nc);
+    const int q0 = startB[c0], q1 = startB[c1];
+    const int c = c0 + (int)threadIdx.x;
+    const bool mine = (int)threadIdx.x < cpb && c < c1;
+    const bool lds = q1 - q0 <= REB_CAP; /* uniform over the block */
+    int nb = 0, m = 0;
+    if (mine) {
+        const int b = start[c], mo = start[c + 1] - b;
+        nb = startB[c];
+        const int ms = cnt[c], ki = inc[c];
+        m = ms + ki;
+        if (lds)
+            rebin_cell(c, b, mo, nb, ms, ki, key, cid, keyB, so, nb - q0, err);
+        else
+            rebin_cell(c, b, mo, nb, ms, ki, key, cid, keyB, keyB, nb, err);
+        cnt[c] = m;
+    }
+    if (lds) {
+        __syncthreads();
+        for (int q = q0 + (int)threadIdx.x; q < q1; q += REB_CELLS) {
+            const unsigned long long kv = so[q - q0];
+            keyB[q] = kv;
+            listB[q] = (int)(unsigned)(kv & 0xffffffffull);
+        }
+    } else {
+        for (int q = nb; q < nb + m; ++q) listB[q] = (int)(unsigned)(keyB[q] & 0xffffffffull);
+    }
 }
 
 /* ---- collision ---- */
@@ -1061,7 +1221,11 @@ __global__ void __launch_bounds__(TPB) k_closure_cell_metrics(int Nc, const doub
 
 struct Binning {
     int *cnt = nullptr, *start = nullptr, *cursor = nullptr, *list = nullptr;
-    unsigned long long* key = nullptr; /* (pid << 32 | slot) in cell runs, before the pid sort */
+    unsigned long long* key = nullptr; /* (pid << 32 | slot) in (cell, pid) order */
+    /* the incremental update's second buffers (swapped with the above),
+     * per-cell incoming counts and per-slot rank among a cell's incoming */
+    int *startB = nullptr, *listB = nullptr, *inc = nullptr, *rnk = nullptr;
+    unsigned long long* keyB = nullptr;
 };
 
 }  // namespace
@@ -1194,6 +1358,11 @@ struct mpcd_gpu_sim {
         bin.cursor = dalloc<int>((size_t)Nc + 1);
         bin.list = dalloc<int>(n);
         bin.key = dalloc<unsigned long long>(n);
+        bin.startB = dalloc<int>((size_t)Nc + 1);
+        bin.listB = dalloc<int>(n);
+        bin.inc = dalloc<int>((size_t)Nc + 1);
+        bin.rnk = dalloc<int>(n);
+        bin.keyB = dalloc<unsigned long long>(n);
         nTiles = (int)((Nc + 1 + SCAN_TILE - 1) / SCAN_TILE);
         tileSum = dalloc<int>(nTiles);
         stage = dalloc<double>(2 * (size_t)n);
@@ -1275,7 +1444,7 @@ struct mpcd_gpu_sim {
     }
 
     ~mpcd_gpu_sim() {
-        void* ptrs[] = {x, y, vx, vy, vx2, vy2, x2, y2, pid, pid2, cid, cidS, bin.cnt, bin.start, bin.cursor, bin.list, bin.key, tileSum, next,
+        void* ptrs[] = {x, y, vx, vy, vx2, vy2, x2, y2, pid, pid2, cid, cidS, bin.cnt, bin.start, bin.cursor, bin.list, bin.key, bin.startB, bin.listB, bin.inc, bin.rnk, bin.keyB, tileSum, next,
                         tmpg, zsize, zoff, owner[0], owner[1], cover[0], cover[1], overCells[0], overCells[1], stage,
                         refPx, refPy, refUx, refUy, pdriveRaw, redPx, redPy, redUx, redUy, duRX, duRY, duEX, duEY,
                         pdrive, cellTmp, mask, part, scal, err, wallPart, rects[0], rects[1], zout, zscratch, duG, recG, zstate,
@@ -1357,6 +1526,31 @@ struct mpcd_gpu_sim {
         check_launch();
         bin_cells(cid);
     }
+    /* The unshifted binning of the current positions, from the unshifted
+     * binning of earlier positions of the same slots (cid and bin hold it):
+     * only the particles that changed cell are placed, and each cell's run
+     * is the merge of its stayers and its arrivals, in pid order.  The
+     * result equals bin_unshifted()'s. */
+    void rebin_unshifted() {
+        hipLaunchKernelGGL(k_fill_int, dim3(nblk(Nc + 1)), dim3(TPB), 0, stream, bin.inc, Nc + 1, 0);
+        hipLaunchKernelGGL(k_cells_delta, dim3(nblk(n)), dim3(TPB), 0, stream, g, x, y, cid, bin.cnt, bin.inc,
+                           bin.rnk);
+        hipLaunchKernelGGL(k_scan_tiles, dim3(nTiles), dim3(TPB), 0, stream, bin.cnt, Nc + 1, bin.startB, tileSum,
+                           bin.inc);
+        hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(TPB), 0, stream, tileSum, nTiles);
+        hipLaunchKernelGGL(k_scan_add, dim3(nblk(Nc + 1)), dim3(TPB), 0, stream, bin.startB, Nc + 1, tileSum);
+        hipLaunchKernelGGL(k_place_arrivals, dim3(nblk(n)), dim3(TPB), 0, stream, cid, pid, bin.rnk, n, bin.startB,
+                           bin.cnt, bin.keyB);
+        /* cells per block: the block's runs fit the LDS staging at 1.25x the mean density */
+        int cpb = REB_CELLS;
+        while (cpb > 64 && (double)cpb * (double)n / (double)Nc * 1.25 > (double)REB_CAP) cpb >>= 1;
+        hipLaunchKernelGGL(k_rebin_merge, dim3(nblk(Nc, cpb)), dim3(REB_CELLS), 0, stream, bin.start, bin.key, cid,
+                           Nc, cpb, bin.startB, bin.cnt, bin.inc, bin.keyB, bin.listB, err);
+        check_launch();
+        std::swap(bin.start, bin.startB);
+        std::swap(bin.key, bin.keyB);
+        std::swap(bin.list, bin.listB);
+    }
 
     void clear_err() {
         hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, stream, err, 1, 0);
@@ -1368,6 +1562,7 @@ struct mpcd_gpu_sim {
         HIP_OK(hipStreamSynchronize(stream));
         if (h & 1) throw ApiError(MPCD_E_BC_MODE, "Unsupported boundary mode at a wall a particle crossed");
         if (h & 2) throw ApiError(MPCD_E_CAPACITY, "zone scratch capacity exceeded");
+        if (h & 4) throw ApiError(MPCD_E_INVALID, "binning update: a cell's stayers do not match its count");
     }
 
     /* ---- stages ---- */
@@ -1734,8 +1929,9 @@ struct mpcd_gpu_sim {
         layout_pass(0, zs);
         pend(MPCD_PROF_ZONE_PASS);
         if (sm) pass_metrics(nz[0], &sm->base);
+        /* the passes move a few particles: the binning is updated */
         pbeg(MPCD_PROF_BIN);
-        bin_unshifted();
+        rebin_unshifted();
         pend(MPCD_PROF_BIN);
         if (stages && stages[2]) get_state(stages[2], stages[3]);
         pbeg(MPCD_PROF_ZONE_PASS);
@@ -1743,7 +1939,7 @@ struct mpcd_gpu_sim {
         pend(MPCD_PROF_ZONE_PASS);
         if (sm) pass_metrics(nz[1], &sm->shifted);
         pbeg(MPCD_PROF_BIN);
-        bin_unshifted();
+        rebin_unshifted();
         pend(MPCD_PROF_BIN);
         if (stages && stages[4]) get_state(stages[4], stages[5]);
         pbeg(MPCD_PROF_CLOSURE);

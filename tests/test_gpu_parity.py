@@ -342,3 +342,48 @@ def test_wide_zone_pass(gpu, oracle, layout):
     assert mg["nParticlesMovedSparse"] == mo["nParticlesMovedSparse"]
     assert bits(xg, xo) and bits(vg, vo)
     assert mg["corrRmsDU"] == mo["corrRmsDU"] and mg["corrMaxDU"] == mo["corrMaxDU"]
+
+
+def test_full_step_clustered(gpu, oracle):
+    """A full step from a state with a crowded band of columns: the binning
+    after each zone pass is updated from the one before it (stayers merged
+    with arrivals).  The case has runs longer than a register batch, cells
+    with many arrivals, and blocks of cells whose runs exceed the LDS staging,
+    so every path of the update runs; the step must still equal the oracle's."""
+    from src_incompressible_openmp_amd.fileio import uniform_state
+
+    p = wide_params(Nx=64, Ny=48, gamma=20.0)
+    x, v, _, _ = uniform_state(p.n, p.Lx, p.Ly, p.kBT, seed=21)
+    rng = np.random.default_rng(21)
+    dx = p.Lx / p.Nx
+    go = rng.random(p.n) < 0.15
+    x[0::2][go] = (20.0 + 3.0 * rng.random(int(go.sum()))) * dx
+    xo, vo = x.copy(), v.copy()
+    so = oracle.step(CTR, p, xo, vo, 3, stages=True)
+
+    def cells(xs):
+        ix = np.clip(np.floor(xs[0::2] / dx).astype(np.int64), 0, p.Nx - 1)
+        iy = np.clip(np.floor(xs[1::2] / (p.Ly / p.Ny)).astype(np.int64), 0, p.Ny - 1)
+        return iy + p.Ny * ix
+
+    c_ref, c_base = cells(so["x_ref"]), cells(so["x_base"])
+    moved = c_ref != c_base
+    arrivals = np.bincount(c_base[moved], minlength=p.Nx * p.Ny)
+    runs = np.bincount(c_ref, minlength=p.Nx * p.Ny)
+    block = np.add.reduceat(np.bincount(c_base, minlength=p.Nx * p.Ny), np.arange(0, p.Nx * p.Ny, 128))
+    print("clustered: moved", int(moved.sum()), "max arrivals", arrivals.max(), "max run", runs.max(),
+          "max block", block.max(), flush=True)
+    assert arrivals.max() > 8 and runs.max() > 32 and block.max() > 6144
+    sim = gpu.Sim(p)
+    sim.set_state(x, v)
+    sg = sim.step_stages(3)
+    xg, vg = sim.get_state()
+    for a in ("ref", "base", "red"):
+        assert bits(sg["x_" + a], so["x_" + a]), a
+        assert bits(sg["v_" + a], so["v_" + a]), a
+    assert bits(xg, xo)
+    assert vclose(vg, vo)
+    m = sg["metrics"]
+    for L in ("base", "shifted"):
+        assert m[L]["nParticlesMovedDense"] == so[L]["nParticlesMovedDense"]
+        assert m[L]["nParticlesMovedSparse"] == so[L]["nParticlesMovedSparse"]
