@@ -134,7 +134,7 @@ typedef struct mpcd_step_metrics {
  * simulation's stream around each launch group). */
 enum {
     MPCD_PROF_STREAM_BC = 0,   /* force + stream + wall BC + shifted cell */
-    MPCD_PROF_BIN = 1,         /* counting-sort binnings (4 per step) */
+    MPCD_PROF_BIN = 1,         /* binnings (2 counting sorts, 2 updates per step) */
     MPCD_PROF_COLLIDE = 2,     /* SRD rotation + thermostat */
     MPCD_PROF_CELL_FIELDS = 3, /* ref cell sums for the closure */
     MPCD_PROF_ZONE_PASS = 4,   /* a whole zone pass (2 per step) */
@@ -142,7 +142,8 @@ enum {
     MPCD_PROF_CLOSURE = 6,     /* closure: fields, beta, stencil, kick */
     MPCD_PROF_KICK = 7,        /* the closure's per-particle kick alone */
     MPCD_PROF_ZONE_FINISH = 8, /* the zone passes' momentum correction kernel (2 per step) */
-    MPCD_PROF_CLASSES = 9
+    MPCD_PROF_KICK_STREAM = 9, /* the kick fused with the next step's stream (steps of one call) */
+    MPCD_PROF_CLASSES = 10
 };
 
 /* Opaque device-resident simulation (state, binnings, layouts). */

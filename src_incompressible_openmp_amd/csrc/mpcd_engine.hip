@@ -1036,6 +1036,15 @@ __global__ void k_eos(Geom g, const double* __restrict__ p, double dtOverRho, do
     ey[c] = -dtOverRho * gy * betaEOS;
 }
 
+/* The closure's kick: repair (scaled by beta) then EOS correction. */
+__device__ __forceinline__ void kick_velocity(double v0x, double v0y, double kx, double ky, double ex, double ey,
+                                              double& nx, double& ny) {
+    nx = v0x + kx;
+    ny = v0y + ky;
+    nx += ex;
+    ny += ey;
+}
+
 /* v = (v + beta*duRepair[c]) + duEOS[c] for the particle's (red) cell
  * (apply_cell_kick twice, liquid_closure.cpp:379-436), fused with a reorder
  * of the state into the order of the current (red) binning: position s of
@@ -1060,10 +1069,9 @@ __global__ void __launch_bounds__(TPB) k_kick_reorder(Geom g, const int* __restr
         const int c = cell_of(g, px, py);
         const double kx = beta * duRX[c], ky = beta * duRY[c];
         const double v0x = vx[i], v0y = vy[i];
-        double nx = v0x + kx, ny = v0y + ky;
         const double ex = duEX[c], ey = duEY[c];
-        nx += ex;
-        ny += ey;
+        double nx, ny;
+        kick_velocity(v0x, v0y, kx, ky, ex, ey, nx, ny);
         xo[s] = px;
         yo[s] = py;
         vxo[s] = nx;
@@ -1102,6 +1110,43 @@ __global__ void __launch_bounds__(TPB) k_kick_reorder(Geom g, const int* __restr
     }
     if (threadIdx.x == 0)
         for (int k = 0; k < 6; ++k) part[6 * (size_t)blockIdx.x + k] = sh[k][0];
+}
+
+/* The kick and reorder of one step fused with the next step's force,
+ * stream and wall conditions (k_stream_bc's work on the same values, in
+ * registers): the state is written once, already streamed, with both cell
+ * ids of the next step.  Used between steps of one call that report no
+ * metrics. */
+__global__ void __launch_bounds__(TPB) k_kick_reorder_stream(
+    Geom g, BCParams b, uint64_t seed, const int* __restrict__ list, const double* __restrict__ x,
+    const double* __restrict__ y, const double* __restrict__ vx, const double* __restrict__ vy,
+    const int* __restrict__ pid, double* __restrict__ xo, double* __restrict__ yo, double* __restrict__ vxo,
+    double* __restrict__ vyo, int* __restrict__ pido, const double* __restrict__ duRX,
+    const double* __restrict__ duRY, const double* __restrict__ duEX, const double* __restrict__ duEY,
+    const double* __restrict__ scal, int* __restrict__ cidS, int* __restrict__ cidU, double sx, double sy,
+    double xmax, double ymax, int* __restrict__ err) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= g.n) return;
+    const double beta = scal[3];
+    const int i = list[s];
+    double px = x[i], py = y[i];
+    const int c = cell_of(g, px, py);
+    const double kx = beta * duRX[c], ky = beta * duRY[c];
+    const double ex = duEX[c], ey = duEY[c];
+    double nx, ny;
+    kick_velocity(vx[i], vy[i], kx, ky, ex, ey, nx, ny);
+    const int pi = pid[i];
+    WallAcc acc;
+    wall_acc_zero(acc);
+    const int rc = bc_particle(g, b, seed, (uint32_t)pi, 1, px, py, nx, ny, acc);
+    if (rc != 0) atomicOr(err, 1);
+    xo[s] = px;
+    yo[s] = py;
+    vxo[s] = nx;
+    vyo[s] = ny;
+    pido[s] = pi;
+    cidS[s] = shifted_cell_of(g, px, py, sx, sy, xmax, ymax);
+    cidU[s] = cell_of(g, px, py);
 }
 
 /* per-cell closure metrics partials: sum hypot(beta*duR), sum pdriveRaw, sum pdrive */
@@ -1290,6 +1335,8 @@ struct mpcd_gpu_sim {
     CollideParams cp{};
     BCParams bc{};
     double xmax = 0, ymax = 0;
+    int fuseNext = -1; /* step whose stream the current kick performs (-1: none) */
+    int streamed = -1; /* step whose stream has been performed by a kick (-1: none) */
     int cellCap = 0; /* LDS entries of a per-cell block (k_collide, k_cell_fields) */
 
     /* per-class kernel timing with events on this stream (MPCD_PROF_*) */
@@ -1380,6 +1427,7 @@ struct mpcd_gpu_sim {
         ensure_parts((size_t)std::max(nblk(n) * 6, nblk(Nc) * 3) + 64);
         wallPart = dalloc<WallAcc>(nblk(n));
         hipLaunchKernelGGL(k_iota, dim3(nblk(n)), dim3(TPB), 0, stream, pid, n);
+        hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, stream, err, 1, 0); /* no error flagged yet */
         check_launch();
         /* constants */
         bc = make_bc_params(p);
@@ -1768,6 +1816,35 @@ struct mpcd_gpu_sim {
      * A prediction that fails the residual check (not finite, or beyond the
      * cap) sends the step to the plain closure instead.  ref sums and the
      * red binning as for closure(). */
+    /* The kick and the reorder into red-cell order.  fuseNext >= 0 (set by
+     * step() between steps of one call) and no metrics: the next step's
+     * force, stream and wall conditions run in the same kernel, and that
+     * step skips its own (streamed records which step is done). */
+    void kick_reorder(double* partOut) {
+        const bool fuse = fuseNext >= 0 && !partOut;
+        const int cls = fuse ? MPCD_PROF_KICK_STREAM : MPCD_PROF_KICK;
+        pbeg(cls);
+        if (fuse) {
+            const uint64_t un = (uint64_t)fuseNext;
+            double sh[2];
+            mpcd_draw_shift(130363ULL + 10007ULL * un, p.a0, &sh[0], &sh[1]);
+            hipLaunchKernelGGL(k_kick_reorder_stream, dim3(nblk(n)), dim3(TPB), 0, stream, g, bc,
+                               104729ULL + 10007ULL * un, bin.list, x, y, vx, vy, pid, x2, y2, vx2, vy2, pid2, duRX,
+                               duRY, duEX, duEY, scal, cidS, cid, sh[0], sh[1], xmax, ymax, err);
+            streamed = fuseNext;
+        } else {
+            hipLaunchKernelGGL(k_kick_reorder, dim3(nblk(n)), dim3(TPB), 0, stream, g, bin.list, x, y, vx, vy, pid,
+                               x2, y2, vx2, vy2, pid2, duRX, duRY, duEX, duEY, scal, partOut);
+        }
+        pend(cls);
+        std::swap(x, x2);
+        std::swap(y, y2);
+        std::swap(vx, vx2);
+        std::swap(vy, vy2);
+        std::swap(pid, pid2); /* the binning now describes the old order: stale until rebuilt */
+        check_launch();
+    }
+
     void closure_learned() {
         FieldsOut f{};
         f.N = redN;
@@ -1796,16 +1873,7 @@ struct mpcd_gpu_sim {
             return;
         }
         ++nLearned;
-        pbeg(MPCD_PROF_KICK);
-        hipLaunchKernelGGL(k_kick_reorder, dim3(nblk(n)), dim3(TPB), 0, stream, g, bin.list, x, y, vx, vy, pid, x2, y2,
-                           vx2, vy2, pid2, duRX, duRY, duEX, duEY, scal, (double*)nullptr);
-        pend(MPCD_PROF_KICK);
-        std::swap(x, x2);
-        std::swap(y, y2);
-        std::swap(vx, vx2);
-        std::swap(vy, vy2);
-        std::swap(pid, pid2);
-        check_launch();
+        kick_reorder(nullptr);
     }
 
     /* closure: ref sums already in ref*; red binning current */
@@ -1857,16 +1925,7 @@ struct mpcd_gpu_sim {
                                pdrive, scal, part);
             hipLaunchKernelGGL(k_reduce_parts, dim3(1), dim3(TPB), 0, stream, part, nbc, 3, scal + 16);
         }
-        pbeg(MPCD_PROF_KICK);
-        hipLaunchKernelGGL(k_kick_reorder, dim3(nbp), dim3(TPB), 0, stream, g, bin.list, x, y, vx, vy, pid, x2, y2,
-                           vx2, vy2, pid2, duRX, duRY, duEX, duEY, scal, cm ? part : (double*)nullptr);
-        pend(MPCD_PROF_KICK);
-        std::swap(x, x2);
-        std::swap(y, y2);
-        std::swap(vx, vx2);
-        std::swap(vy, vy2);
-        std::swap(pid, pid2); /* the binning now describes the old order: stale until rebuilt */
-        check_launch();
+        kick_reorder(cm ? part : (double*)nullptr);
         if (cm) {
             std::vector<double> h((size_t)nbp * 6);
             HIP_OK(hipMemcpyAsync(h.data(), part, h.size() * sizeof(double), hipMemcpyDeviceToHost, stream));
@@ -1903,14 +1962,21 @@ struct mpcd_gpu_sim {
 
     /* one full time step with the benchmark driver's seeds
      * (main_benchmark_poiseuille_openmp_profiled_shifted_profiled.cpp:240-279) */
-    void step(int s, mpcd_step_metrics* sm, double* const* stages) {
+    /* next >= 0: the step that follows in the same call (its stream is
+     * fused into this step's kick when no metrics are asked) */
+    void step(int s, mpcd_step_metrics* sm, double* const* stages, int next = -1) {
         check_pass_modes();
         const uint64_t us = (uint64_t)s;
-        double sh[2];
-        mpcd_draw_shift(130363ULL + 10007ULL * us, p.a0, &sh[0], &sh[1]);
-        pbeg(MPCD_PROF_STREAM_BC);
-        stream_bc(104729ULL + 10007ULL * us, 1, sh, sm ? &sm->wall : nullptr);
-        pend(MPCD_PROF_STREAM_BC);
+        if (streamed == s && !sm) {
+            streamed = -1; /* the previous step's kick streamed this one */
+        } else {
+            if (streamed >= 0) throw ApiError(MPCD_E_INVALID, "internal: a fused stream was left unused");
+            double sh[2];
+            mpcd_draw_shift(130363ULL + 10007ULL * us, p.a0, &sh[0], &sh[1]);
+            pbeg(MPCD_PROF_STREAM_BC);
+            stream_bc(104729ULL + 10007ULL * us, 1, sh, sm ? &sm->wall : nullptr);
+            pend(MPCD_PROF_STREAM_BC);
+        }
         pbeg(MPCD_PROF_BIN);
         bin_cells(cidS);
         pend(MPCD_PROF_BIN);
@@ -1943,12 +2009,14 @@ struct mpcd_gpu_sim {
         pend(MPCD_PROF_BIN);
         if (stages && stages[4]) get_state(stages[4], stages[5]);
         pbeg(MPCD_PROF_CLOSURE);
+        fuseNext = (stages || sm) ? -1 : next;
         if (sur && surK > 0 && s % surK != 0) {
             closure_learned();
         } else {
             ++nPlain;
             closure(sm ? &sm->closure : nullptr);
         }
+        fuseNext = -1;
         pend(MPCD_PROF_CLOSURE);
     }
 };
@@ -2009,7 +2077,12 @@ int mpcd_gpu_sim_get_state(mpcd_gpu_sim* s, double* x, double* v) {
 int mpcd_gpu_sim_step(mpcd_gpu_sim* s, int first_step, int n_steps, mpcd_step_metrics* last) {
     return guarded([&] {
         s->clear_err();
-        for (int k = 0; k < n_steps; ++k) s->step(first_step + k, (k == n_steps - 1) ? last : nullptr, nullptr);
+        s->streamed = -1;
+        for (int k = 0; k < n_steps; ++k) {
+            /* the step before the one that reports metrics streams nothing ahead */
+            const bool fuse = k + 1 < n_steps && !(last && k + 2 == n_steps);
+            s->step(first_step + k, (k == n_steps - 1) ? last : nullptr, nullptr, fuse ? first_step + k + 1 : -1);
+        }
         s->check_err();
     });
 }
@@ -2017,7 +2090,8 @@ int mpcd_gpu_sim_step(mpcd_gpu_sim* s, int first_step, int n_steps, mpcd_step_me
 /* Enqueue steps without waiting or checking the wall-mode flag (timed loops). */
 int mpcd_gpu_sim_step_async(mpcd_gpu_sim* s, int first_step, int n_steps) {
     return guarded([&] {
-        for (int k = 0; k < n_steps; ++k) s->step(first_step + k, nullptr, nullptr);
+        s->streamed = -1;
+        for (int k = 0; k < n_steps; ++k) s->step(first_step + k, nullptr, nullptr, k + 1 < n_steps ? first_step + k + 1 : -1);
     });
 }
 
@@ -2030,6 +2104,7 @@ int mpcd_gpu_sim_step_stages(mpcd_gpu_sim* s, int step, double* x_ref, double* v
     return guarded([&] {
         double* st[6] = {x_ref, v_ref, x_base, v_base, x_red, v_red};
         s->clear_err();
+        s->streamed = -1;
         s->step(step, m, st);
         s->check_err();
     });

@@ -114,7 +114,7 @@ def _check(rc: int) -> None:
 
 
 PROF_CLASSES = ("stream_bc", "bin", "collide", "cell_fields", "zone_pass", "zone_kernel", "closure", "kick",
-                "zone_finish")
+                "zone_finish", "kick_stream")
 
 
 def device_synchronize() -> None:

@@ -387,3 +387,39 @@ def test_full_step_clustered(gpu, oracle):
     for L in ("base", "shifted"):
         assert m[L]["nParticlesMovedDense"] == so[L]["nParticlesMovedDense"]
         assert m[L]["nParticlesMovedSparse"] == so[L]["nParticlesMovedSparse"]
+
+
+@pytest.mark.parametrize("case", ["small", "cfg0"])
+def test_steps_in_one_call_equal_single_calls(gpu, case):
+    """Steps run in one call (each step's kick also streams the step after
+    it) equal the same steps run one call each, bit for bit; so do a call
+    that reports the last step's metrics and the asynchronous form."""
+    p = to_struct(case_params(case))
+    x, v, _, _ = case_state(case)
+    a = gpu.Sim(p)
+    a.set_state(x, v)
+    a.step(1, 6)
+    xa, va = a.get_state()
+    b = gpu.Sim(p)
+    b.set_state(x, v)
+    for s in range(1, 7):
+        b.step(s)
+    xb, vb = b.get_state()
+    assert bits(xa, xb) and bits(va, vb)
+    c = gpu.Sim(p)
+    c.set_state(x, v)
+    mc = c.step(1, 6, metrics=True)
+    d = gpu.Sim(p)
+    d.set_state(x, v)
+    d.step(1, 5)
+    md = d.step(6, metrics=True)
+    xc, vc = c.get_state()
+    xd, vd = d.get_state()
+    assert bits(xc, xa) and bits(vc, va) and bits(xd, xa) and bits(vd, va)
+    assert mc == md
+    e = gpu.Sim(p)
+    e.set_state(x, v)
+    e.step_async(1, 6)
+    e.check()
+    xe, ve = e.get_state()
+    assert bits(xe, xa) and bits(ve, va)
