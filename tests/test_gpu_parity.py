@@ -344,20 +344,27 @@ def test_wide_zone_pass(gpu, oracle, layout):
     assert mg["corrRmsDU"] == mo["corrRmsDU"] and mg["corrMaxDU"] == mo["corrMaxDU"]
 
 
+def clustered_state():
+    """64x48 cells, 20 per cell on average, 15% of the particles crowded into
+    three columns (about 84 per cell there)."""
+    from src_incompressible_openmp_amd.fileio import uniform_state
+
+    p = wide_params(Nx=64, Ny=48, gamma=20.0)
+    x, v, _, _ = uniform_state(p.n, p.Lx, p.Ly, p.kBT, seed=21)
+    rng = np.random.default_rng(21)
+    go = rng.random(p.n) < 0.15
+    x[0::2][go] = (20.0 + 3.0 * rng.random(int(go.sum()))) * (p.Lx / p.Nx)
+    return p, x, v
+
+
 def test_full_step_clustered(gpu, oracle):
     """A full step from a state with a crowded band of columns: the binning
     after each zone pass is updated from the one before it (stayers merged
     with arrivals).  The case has runs longer than a register batch, cells
     with many arrivals, and blocks of cells whose runs exceed the LDS staging,
     so every path of the update runs; the step must still equal the oracle's."""
-    from src_incompressible_openmp_amd.fileio import uniform_state
-
-    p = wide_params(Nx=64, Ny=48, gamma=20.0)
-    x, v, _, _ = uniform_state(p.n, p.Lx, p.Ly, p.kBT, seed=21)
-    rng = np.random.default_rng(21)
+    p, x, v = clustered_state()
     dx = p.Lx / p.Nx
-    go = rng.random(p.n) < 0.15
-    x[0::2][go] = (20.0 + 3.0 * rng.random(int(go.sum()))) * dx
     xo, vo = x.copy(), v.copy()
     so = oracle.step(CTR, p, xo, vo, 3, stages=True)
 
@@ -423,3 +430,20 @@ def test_steps_in_one_call_equal_single_calls(gpu, case):
     e.check()
     xe, ve = e.get_state()
     assert bits(xe, xa) and bits(ve, va)
+
+
+def test_clustered_steps_in_one_call(gpu):
+    """The clustered state over four steps in one call (each binning after
+    streaming updated from the previous step's red binning) equals four
+    single-step calls (each binning rebuilt)."""
+    p, x, v = clustered_state()
+    a = gpu.Sim(p)
+    a.set_state(x, v)
+    a.step(1, 4)
+    b = gpu.Sim(p)
+    b.set_state(x, v)
+    for s in range(1, 5):
+        b.step(s)
+    xa, va = a.get_state()
+    xb, vb = b.get_state()
+    assert bits(xa, xb) and bits(va, vb)
