@@ -430,15 +430,28 @@ __global__ void k_cells_delta(Geom g, const double* __restrict__ x, const double
 
 /* Arrivals go to the tail of their new cell's run (after the stayers), in
  * arrival order; the merge sorts them. */
+constexpr int PLACE_PER = 4; /* slots per thread (one 16-byte load of ranks) */
 __global__ void k_place_arrivals(const int* __restrict__ cid, const int* __restrict__ pid,
                                  const int* __restrict__ rnk, int n, const int* __restrict__ startB,
                                  const int* __restrict__ stay, unsigned long long* __restrict__ keyB) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const int r = rnk[i];
-    if (r < 0) return;
-    const int c = cid[i];
-    keyB[startB[c] + stay[c] + r] = ((unsigned long long)(unsigned)pid[i] << 32) | (unsigned)i;
+    const int i0 = (blockIdx.x * blockDim.x + threadIdx.x) * PLACE_PER;
+    if (i0 >= n) return;
+    int r[PLACE_PER];
+    if (i0 + PLACE_PER <= n) {
+        const int4 q = *reinterpret_cast<const int4*>(rnk + i0);
+        r[0] = q.x;
+        r[1] = q.y;
+        r[2] = q.z;
+        r[3] = q.w;
+    } else {
+        for (int u = 0; u < PLACE_PER; ++u) r[u] = i0 + u < n ? rnk[i0 + u] : -1;
+    }
+    for (int u = 0; u < PLACE_PER; ++u) {
+        if (r[u] < 0) continue;
+        const int i = i0 + u;
+        const int c = cid[i];
+        keyB[startB[c] + stay[c] + r[u]] = ((unsigned long long)(unsigned)pid[i] << 32) | (unsigned)i;
+    }
 }
 
 constexpr int REB_CELLS = 256; /* threads per block of the merge */
@@ -1587,8 +1600,8 @@ struct mpcd_gpu_sim {
                            bin.inc);
         hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(TPB), 0, stream, tileSum, nTiles);
         hipLaunchKernelGGL(k_scan_add, dim3(nblk(Nc + 1)), dim3(TPB), 0, stream, bin.startB, Nc + 1, tileSum);
-        hipLaunchKernelGGL(k_place_arrivals, dim3(nblk(n)), dim3(TPB), 0, stream, cid, pid, bin.rnk, n, bin.startB,
-                           bin.cnt, bin.keyB);
+        hipLaunchKernelGGL(k_place_arrivals, dim3(nblk((n + PLACE_PER - 1) / PLACE_PER)), dim3(TPB), 0, stream, cid, pid,
+                           bin.rnk, n, bin.startB, bin.cnt, bin.keyB);
         /* cells per block: the block's runs fit the LDS staging at 1.25x the mean density */
         int cpb = REB_CELLS;
         while (cpb > 64 && (double)cpb * (double)n / (double)Nc * 1.25 > (double)REB_CAP) cpb >>= 1;
@@ -1688,7 +1701,8 @@ struct mpcd_gpu_sim {
      * (nullptr when none do).  single = 1: one zone that owns all its
      * particles, seeded with seedBase itself (run_zone_kernel_inplace). */
     void zone_pass(uint64_t seedBase, const ZoneRect* rectsDev, int nZ, int maxC, int maxW, const int* ownerDev,
-                   const unsigned char* coverDev, const int* overDev, int nOverCells, int single) {
+                   const unsigned char* coverDev, const int* overDev, int nOverCells, int single,
+                   const double* inPx = nullptr, const double* inPy = nullptr) {
         hipLaunchKernelGGL(k_zone_sizes, dim3(nblk(nZ + 1)), dim3(TPB), 0, stream, rectsDev, nZ, bin.start, p.Ny,
                            zsize);
         const int nt = (nZ + 1 + SCAN_TILE - 1) / SCAN_TILE;
@@ -1712,6 +1726,8 @@ struct mpcd_gpu_sim {
         a.in.vy = vy;
         a.in.vsx = vx2; /* the side buffer: free during a pass */
         a.in.vsy = vy2;
+        a.in.inPx = inPx;
+        a.in.inPy = inPy;
         a.rects = rectsDev;
         a.nZones = nZ;
         a.seedBase = seedBase;
@@ -1787,8 +1803,9 @@ struct mpcd_gpu_sim {
         }
     }
 
-    void layout_pass(int L, uint64_t seedBase) {
-        zone_pass(seedBase, rects[L], nz[L], maxCells[L], maxCols[L], owner[L], cover[L], overCells[L], nOver[L], 0);
+    void layout_pass(int L, uint64_t seedBase, const double* inPx = nullptr, const double* inPy = nullptr) {
+        zone_pass(seedBase, rects[L], nz[L], maxCells[L], maxCols[L], owner[L], cover[L], overCells[L], nOver[L], 0,
+                  inPx, inPy);
     }
 
     void pass_metrics(int nZ, mpcd_zone_pass_metrics* m) {
@@ -1992,7 +2009,10 @@ struct mpcd_gpu_sim {
         pend(MPCD_PROF_CELL_FIELDS);
         const uint64_t zs = 88172645463325252ULL + 10007ULL * us;
         pbeg(MPCD_PROF_ZONE_PASS);
-        layout_pass(0, zs);
+        /* the base pass's input is the state the reference fields were
+         * summed over (same runs, same order): the correction takes its
+         * input sums from them */
+        layout_pass(0, zs, refPx, refPy);
         pend(MPCD_PROF_ZONE_PASS);
         if (sm) pass_metrics(nz[0], &sm->base);
         /* the passes move a few particles: the binning is updated */

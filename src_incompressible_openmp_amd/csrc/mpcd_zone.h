@@ -181,6 +181,9 @@ struct ZoneIn {
     const unsigned char* gcover; /* per cell: several zones cover it, or nullptr */
     double *x, *y, *vx, *vy;
     double *vsx, *vsy;  /* side buffer for covered cells */
+    /* per-cell velocity sums of the pass input (each cell's run summed in
+     * order from zero), or nullptr: the correction sums the runs itself */
+    const double *inPx = nullptr, *inPy = nullptr;
 };
 
 /* Scalars the lanes share (LDS on the device). */
@@ -202,6 +205,7 @@ struct ZoneCtxT {
     const int *gcnt, *gstart, *glist;
     const unsigned long long* gkey;
     double *x, *y, *vx, *vy, *vsx, *vsy;
+    const double *inPx, *inPy; /* input run sums per global cell, or nullptr */
     IT* nxL;     /* links of k < nL */
     IT* nxG;     /* links of k >= nL (zone's global region, indexed by k) */
     IT* tmpG;    /* donor lists longer than ZONE_TMP_L (zone's global region) */
@@ -1768,6 +1772,8 @@ ZHD void zone_init(ZoneCtxT<IT>& z, const ZoneIn& in, ZoneRect r, int zid, uint6
     z.gstart = in.gstart;
     z.glist = in.glist;
     z.gkey = in.gkey;
+    z.inPx = in.inPx;
+    z.inPy = in.inPy;
     z.x = in.x;
     z.y = in.y;
     z.vx = in.vx;
@@ -1907,6 +1913,11 @@ ZHD void fin_input_sum(const Z& z, int lc, double& pbx, double& pby) {
     pby = 0.0;
     if (!fget(z, lc, ZF_CHG) || cnt_of(z, lc) == 0) return;
     const int gc = lc_gc(z, lc);
+    if (z.inPx) {
+        pbx = z.inPx[gc];
+        pby = z.inPy[gc];
+        return;
+    }
     const int s0 = z.gstart[gc], n = z.gcnt[gc];
     for (int j0 = 0; j0 < n; j0 += FIN_L) {
         int sl[FIN_L];
