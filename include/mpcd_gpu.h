@@ -138,10 +138,10 @@ enum {
     MPCD_PROF_COLLIDE = 2,     /* SRD rotation + thermostat */
     MPCD_PROF_CELL_FIELDS = 3, /* ref cell sums for the closure */
     MPCD_PROF_ZONE_PASS = 4,   /* a whole zone pass (2 per step) */
-    MPCD_PROF_ZONE_KERNEL = 5, /* the zone kernel launch alone (2 per step) */
+    MPCD_PROF_ZONE_KERNEL = 5, /* the zone kernel launches (passes, and the correction when fused; 2 per step) */
     MPCD_PROF_CLOSURE = 6,     /* closure: fields, beta, stencil, kick */
     MPCD_PROF_KICK = 7,        /* the closure's per-particle kick alone */
-    MPCD_PROF_ZONE_FINISH = 8, /* the zone passes' momentum correction kernel (2 per step) */
+    MPCD_PROF_ZONE_FINISH = 8, /* the separate momentum correction kernel (zones the fused kernel does not take) */
     MPCD_PROF_KICK_STREAM = 9, /* the kick fused with the next step's stream (steps of one call) */
     MPCD_PROF_CLASSES = 10
 };

@@ -790,6 +790,37 @@ __global__ void k_zone_sizes(const ZoneRect* __restrict__ rects, int nZ, const i
 /* One zone per workgroup of one wavefront.  IT = uint16_t keeps the working
  * set in LDS at four zones per CU; IT = int serves zones of 65535 particles
  * or more (and the single-zone layout), in LDS or in global scratch. */
+/* The passes of zone zi by one wavefront (lanes 0..63): set-up, the dense
+ * and sparse phases, the state hand-over to the correction.  The zone code
+ * synchronises its lanes with wavefront-level barriers only. */
+template <class IT, bool kLds>
+__device__ __forceinline__ void zone_pass_body(const ZoneArgs& a, int zi, long long off, int lane,
+                                               unsigned char* lds, ZoneShared* sh) {
+    ZoneCtxT<IT> z;
+    const uint64_t seed = a.singleSeed ? a.seedBase : a.seedBase + 104729ULL * (uint64_t)(zi + 1);
+    zone_init(z, a.in, a.rects[zi], zi + 1, seed, sh);
+    z.nxG = (IT*)a.nextG + off;
+    z.tmpG = (IT*)a.tmpG + off;
+    z.duG = a.duG + (size_t)zi * (size_t)a.duStride;
+    z.rec = a.rec + off;
+    if (kLds) zone_carve(z, lds, a.ldsBytes);
+    else zone_carve(z, a.gscratch + (size_t)zi * a.gstride, a.gstride);
+    zone_setup(z, a.in, lane, ZONE_TPB);
+    ZBAR();
+    zone_passes(z, lane, ZONE_TPB);
+    zone_dump(z, zone_state_at<IT>(a.zstate + (size_t)zi * a.zsStride, z.C, z.zw), lane, ZONE_TPB);
+    if (lane == 0) {
+        ZoneOut o;
+        o.nHigh = sh->nHigh;
+        o.nLow = sh->nLow;
+        o.movedDense = sh->movedDense;
+        o.movedSparse = sh->movedSparse;
+        o.stats4 = 0.0;
+        o.stats5 = 0.0;
+        a.out[zi] = o;
+    }
+}
+
 template <class IT, bool kLds>
 __global__ void __launch_bounds__(ZONE_TPB) k_zone(ZoneArgs a) {
     extern __shared__ __align__(16) unsigned char lds[];
@@ -804,29 +835,7 @@ __global__ void __launch_bounds__(ZONE_TPB) k_zone(ZoneArgs a) {
         if (lane == 0) atomicOr(a.err, 2);
         return;
     }
-    ZoneCtxT<IT> z;
-    const uint64_t seed = a.singleSeed ? a.seedBase : a.seedBase + 104729ULL * (uint64_t)(zi + 1);
-    zone_init(z, a.in, a.rects[zi], zi + 1, seed, &sh);
-    z.nxG = (IT*)a.nextG + off;
-    z.tmpG = (IT*)a.tmpG + off;
-    z.duG = a.duG + (size_t)zi * (size_t)a.duStride;
-    z.rec = a.rec + off;
-    if (kLds) zone_carve(z, lds, a.ldsBytes);
-    else zone_carve(z, a.gscratch + (size_t)zi * a.gstride, a.gstride);
-    zone_setup(z, a.in, lane, ZONE_TPB);
-    __syncthreads();
-    zone_passes(z, lane, ZONE_TPB);
-    zone_dump(z, zone_state_at<IT>(a.zstate + (size_t)zi * a.zsStride, z.C, z.zw), lane, ZONE_TPB);
-    if (lane == 0) {
-        ZoneOut o;
-        o.nHigh = sh.nHigh;
-        o.nLow = sh.nLow;
-        o.movedDense = sh.movedDense;
-        o.movedSparse = sh.movedSparse;
-        o.stats4 = 0.0;
-        o.stats5 = 0.0;
-        a.out[zi] = o;
-    }
+    zone_pass_body<IT, kLds>(a, zi, off, lane, lds, &sh);
 }
 
 /* The momentum correction of the zones k_zone left (zone_pass.cpp:870-922),
@@ -835,16 +844,11 @@ __global__ void __launch_bounds__(ZONE_TPB) k_zone(ZoneArgs a) {
  * as the single-wavefront form, so the same results. */
 constexpr int FINISH_TPB = 256;
 constexpr int FINISH_LDS = 40 * 1024; /* cell arrays, column map and as many links as fit */
-template <class IT, bool kLds>
-__global__ void __launch_bounds__(FINISH_TPB) k_zone_finish(ZoneArgs a) {
-    extern __shared__ __align__(16) unsigned char lds[];
-    const int zi = blockIdx.x;
-    if (zi >= a.nZones) return;
-    const int P = a.zsize[zi];
-    if (P < a.pMin || P > a.pMax) return;
-    const long long off = a.zoff[zi];
-    if (off + (long long)P > a.scratchCap) return;
-    const int tid = threadIdx.x;
+/* The correction of zone zi by TPB threads (CPT cells per thread held in
+ * registers); lds/ldsBytes: the staging space (kLds). */
+template <class IT, bool kLds, int TPB, int CPT>
+__device__ __forceinline__ void zone_finish_body(const ZoneArgs& a, int zi, long long off, int tid,
+                                                 unsigned char* lds, long long ldsBytes, int* queue) {
     ZoneCtxT<IT> z;
     const uint64_t seed = a.singleSeed ? a.seedBase : a.seedBase + 104729ULL * (uint64_t)(zi + 1);
     zone_init(z, a.in, a.rects[zi], zi + 1, seed, nullptr);
@@ -866,20 +870,20 @@ __global__ void __launch_bounds__(FINISH_TPB) k_zone_finish(ZoneArgs a) {
         int* colL = (int*)p; p += (size_t)z.zw * sizeof(int);
         int* colG = (int*)p; p += (size_t)z.zw * sizeof(int);
         IT* links = (IT*)p;
-        const long long room = ((long long)FINISH_LDS - (long long)(p - lds)) / (long long)sizeof(IT);
+        const long long room = (ldsBytes - (long long)(p - lds)) / (long long)sizeof(IT);
         const int nL = (int)(room < (long long)z.P ? (room > 0 ? room : 0) : z.P);
-        for (int lc = tid; lc < z.C; lc += FINISH_TPB) {
+        for (int lc = tid; lc < z.C; lc += TPB) {
             cnt[lc] = st.cnt[lc];
             head[lc] = st.head[lc];
             tail[lc] = st.tail[lc];
             k0[lc] = st.k0[lc];
             fl[lc] = st.fl[lc];
         }
-        for (int c = tid; c < z.zw; c += FINISH_TPB) {
+        for (int c = tid; c < z.zw; c += TPB) {
             colL[c] = st.colL[c];
             colG[c] = st.colG[c];
         }
-        for (int k = tid; k < nL; k += FINISH_TPB) links[k] = z.nxG[k];
+        for (int k = tid; k < nL; k += TPB) links[k] = z.nxG[k];
         z.cnt = cnt;
         z.head = head;
         z.tail = tail;
@@ -892,12 +896,11 @@ __global__ void __launch_bounds__(FINISH_TPB) k_zone_finish(ZoneArgs a) {
         __syncthreads();
     }
     /* phase 1: input-run sums, kept by the thread of each cell */
-    constexpr int CPT = 4; /* cells per thread held in registers */
     double pbx[CPT], pby[CPT];
-    const bool regs = z.C <= CPT * FINISH_TPB;
+    const bool regs = z.C <= CPT * TPB;
     if (regs)
         for (int j = 0; j < CPT; ++j) {
-            const int lc = tid + j * FINISH_TPB;
+            const int lc = tid + j * TPB;
             pbx[j] = pby[j] = 0.0;
             if (lc < z.C) fin_input_sum(z, lc, pbx[j], pby[j]);
         }
@@ -905,22 +908,19 @@ __global__ void __launch_bounds__(FINISH_TPB) k_zone_finish(ZoneArgs a) {
     /* phase 2: final lists */
     if (regs) {
         for (int j = 0; j < CPT; ++j) {
-            const int lc = tid + j * FINISH_TPB;
+            const int lc = tid + j * TPB;
             if (lc < z.C) st.dsq[lc] = fin_cell(z, lc, pbx[j], pby[j]);
         }
     } else {
         /* more cells than threads x CPT: the two-sweep form, positions included */
-        for (int lc = tid; lc < z.C; lc += FINISH_TPB) zone_correct_sums(z, lc);
+        for (int lc = tid; lc < z.C; lc += TPB) zone_correct_sums(z, lc);
         __syncthreads();
-        for (int lc = tid; lc < z.C; lc += FINISH_TPB) st.dsq[lc] = zone_correct_write(z, lc);
+        for (int lc = tid; lc < z.C; lc += TPB) st.dsq[lc] = zone_correct_write(z, lc);
     }
     __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
     /* phase 3: moved particles' positions (the two-sweep form placed them) */
-    if (regs) {
-        __shared__ int queue[FINISH_TPB];
-        fin_place(z, tid, FINISH_TPB, queue + (tid & ~63));
-    }
+    if (regs) fin_place(z, tid, TPB, queue + (tid & ~63));
     if (tid < 64) {
         double rms, mx;
         zone_correct_stats(z, st.dsq, tid, 64, rms, mx);
@@ -929,6 +929,49 @@ __global__ void __launch_bounds__(FINISH_TPB) k_zone_finish(ZoneArgs a) {
             a.out[zi].stats5 = mx;
         }
     }
+}
+
+template <class IT, bool kLds>
+__global__ void __launch_bounds__(FINISH_TPB) k_zone_finish(ZoneArgs a) {
+    extern __shared__ __align__(16) unsigned char lds[];
+    __shared__ int queue[FINISH_TPB];
+    const int zi = blockIdx.x;
+    if (zi >= a.nZones) return;
+    const int P = a.zsize[zi];
+    if (P < a.pMin || P > a.pMax) return;
+    const long long off = a.zoff[zi];
+    if (off + (long long)P > a.scratchCap) return;
+    zone_finish_body<IT, kLds, FINISH_TPB, 4>(a, zi, off, (int)threadIdx.x, lds, FINISH_LDS, queue);
+}
+
+/* The passes and the correction of a zone in one workgroup of FUSED_TPB
+ * threads: wavefront 0 runs the passes while the others wait at the
+ * barrier, then all run the correction.  A zone's correction starts as
+ * soon as its own passes end, beside the passes of slower zones. */
+constexpr int FUSED_TPB = 128;
+template <class IT>
+__global__ void __launch_bounds__(FUSED_TPB, 2) k_zone_fused(ZoneArgs a) { /* 2 waves per SIMD: four zones per CU */
+    extern __shared__ __align__(16) unsigned char lds[];
+    __shared__ ZoneShared sh;
+    __shared__ int queue[FUSED_TPB];
+    const int zi = blockIdx.x;
+    if (zi >= a.nZones) return;
+    const int P = a.zsize[zi];
+    if (P < a.pMin || P > a.pMax) return;
+    const long long off = a.zoff[zi];
+    if (off + (long long)P > a.scratchCap) {
+        if (threadIdx.x == 0) atomicOr(a.err, 2);
+        return;
+    }
+    if (threadIdx.x < ZONE_TPB) {
+        /* the passes are the critical path: their wavefront issues ahead of
+         * the corrections of zones that are done */
+        __builtin_amdgcn_s_setprio(3);
+        zone_pass_body<IT, true>(a, zi, off, (int)threadIdx.x, lds, &sh);
+        __builtin_amdgcn_s_setprio(0);
+    }
+    __syncthreads();
+    zone_finish_body<IT, true, FUSED_TPB, 6>(a, zi, off, (int)threadIdx.x, lds, a.ldsBytes, queue);
 }
 
 /* velocities of the particles of cells several zones cover, from the side
@@ -1748,11 +1791,19 @@ struct mpcd_gpu_sim {
         pbeg(MPCD_PROF_ZONE_KERNEL);
         const size_t fix16 = zone_fixed_bytes<uint16_t>(maxC, maxW, zp.useLocal);
         const bool fast = maxC < 65535 && fix16 + 2048 <= (size_t)ZONE_LDS_FAST;
+        /* LDS staging of the correction: the cell arrays and the column map leave room for links */
+        const size_t cellB16 = (size_t)maxC * (4 * sizeof(uint16_t) + 2) + 8 * (size_t)maxW + 16;
+        const size_t cellB32 = (size_t)maxC * (4 * sizeof(int) + 2) + 8 * (size_t)maxW + 16;
+        /* passes and correction in one kernel when the correction fits the passes' LDS */
+        const bool fused = fast && cellB16 + 4096 <= (size_t)ZONE_LDS_FAST;
         if (fast) {
             a.pMin = 0;
             a.pMax = 65534;
             a.ldsBytes = ZONE_LDS_FAST;
-            hipLaunchKernelGGL((k_zone<uint16_t, true>), dim3(nZ), dim3(ZONE_TPB), ZONE_LDS_FAST, stream, a);
+            if (fused)
+                hipLaunchKernelGGL((k_zone_fused<uint16_t>), dim3(nZ), dim3(FUSED_TPB), ZONE_LDS_FAST, stream, a);
+            else
+                hipLaunchKernelGGL((k_zone<uint16_t, true>), dim3(nZ), dim3(ZONE_TPB), ZONE_LDS_FAST, stream, a);
             check_launch();
         }
         /* zones the uint16_t path does not take (none, as a rule) */
@@ -1777,10 +1828,7 @@ struct mpcd_gpu_sim {
         check_launch();
         /* the correction, with the same split of the zones by index type */
         pbeg(MPCD_PROF_ZONE_FINISH);
-        /* LDS staging when the cell arrays and the column map leave room for links */
-        const size_t cellB16 = (size_t)maxC * (4 * sizeof(uint16_t) + 2) + 8 * (size_t)maxW + 16;
-        const size_t cellB32 = (size_t)maxC * (4 * sizeof(int) + 2) + 8 * (size_t)maxW + 16;
-        if (fast) {
+        if (fast && !fused) {
             a.pMin = 0;
             a.pMax = 65534;
             if (cellB16 + 4096 <= (size_t)FINISH_LDS)

@@ -60,7 +60,15 @@
 namespace mpcdk {
 
 #if defined(__HIP_DEVICE_COMPILE__)
-#define ZBAR() __syncthreads()
+/* The zone code runs on one wavefront: its barrier orders the lanes' memory
+ * operations (workgroup-scope fences) without a workgroup barrier, so a
+ * larger workgroup can run it on one of its wavefronts. */
+#define ZBAR()                                                 \
+    do {                                                       \
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup"); \
+        __builtin_amdgcn_wave_barrier();                       \
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup"); \
+    } while (0)
 /* compiler-level ordering of LDS accesses between lanes of the one
  * wavefront (the LDS executes a wavefront's accesses in issue order) */
 #define ZWSYNC()                                               \
