@@ -300,7 +300,7 @@ constexpr int SORT_NET = 32; /* runs up to this length are sorted by a network o
 constexpr int SEG_CAP = 6144;
 __global__ void __launch_bounds__(SEG_CELLS) k_segsort(const int* __restrict__ start, int Nc,
                                                        unsigned long long* __restrict__ key, int* __restrict__ list,
-                                                       int narrow) {
+                                                       int narrow, int writeKey) {
     __shared__ unsigned long long sk[SEG_CAP];
     const int c0 = blockIdx.x * SEG_CELLS;
     const int c1 = min(c0 + SEG_CELLS, Nc);
@@ -388,7 +388,7 @@ __global__ void __launch_bounds__(SEG_CELLS) k_segsort(const int* __restrict__ s
         __syncthreads();
         for (int q = b0 + (int)threadIdx.x; q < b1; q += SEG_CELLS) {
             const unsigned long long kv = sk[q - b0];
-            key[q] = kv; /* the sorted keys: slot and pid of each position in one load */
+            if (writeKey) key[q] = kv; /* the sorted keys: slot and pid of each position in one load */
             list[q] = (int)(unsigned)(kv & 0xffffffffull);
         }
         return;
@@ -1171,8 +1171,8 @@ __global__ void __launch_bounds__(TPB) k_kick_reorder(Geom g, const int* __restr
 /* The kick and reorder of one step fused with the next step's force,
  * stream and wall conditions (k_stream_bc's work on the same values, in
  * registers): the state is written once, already streamed, with both cell
- * ids of the next step.  Used between steps of one call that report no
- * metrics. */
+ * ids of the next step and their histograms (cntS, cntU, cleared before).
+ * Used between steps of one call that report no metrics. */
 __global__ void __launch_bounds__(TPB) k_kick_reorder_stream(
     Geom g, BCParams b, uint64_t seed, const int* __restrict__ list, const double* __restrict__ x,
     const double* __restrict__ y, const double* __restrict__ vx, const double* __restrict__ vy,
@@ -1180,9 +1180,10 @@ __global__ void __launch_bounds__(TPB) k_kick_reorder_stream(
     double* __restrict__ vyo, int* __restrict__ pido, const double* __restrict__ duRX,
     const double* __restrict__ duRY, const double* __restrict__ duEX, const double* __restrict__ duEY,
     const double* __restrict__ scal, int* __restrict__ cidS, int* __restrict__ cidU, double sx, double sy,
-    double xmax, double ymax, int* __restrict__ err) {
+    double xmax, double ymax, int* __restrict__ cntS, int* __restrict__ cntU, int* __restrict__ err) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= g.n) return;
+    const int lane = __lane_id();
     const double beta = scal[3];
     const int i = list[s];
     double px = x[i], py = y[i];
@@ -1201,8 +1202,16 @@ __global__ void __launch_bounds__(TPB) k_kick_reorder_stream(
     vxo[s] = nx;
     vyo[s] = ny;
     pido[s] = pi;
-    cidS[s] = shifted_cell_of(g, px, py, sx, sy, xmax, ymax);
-    cidU[s] = cell_of(g, px, py);
+    const int cs = shifted_cell_of(g, px, py, sx, sy, xmax, ymax);
+    const int cu = cell_of(g, px, py);
+    cidS[s] = cs;
+    cidU[s] = cu;
+    /* both histograms of the next step's binnings, as k_hist counts them */
+    int leader, rank, size;
+    wave_groups(cs, true, lane, leader, rank, size);
+    if (lane == leader) atomicAdd(&cntS[cs], size);
+    wave_groups(cu, true, lane, leader, rank, size);
+    if (lane == leader) atomicAdd(&cntU[cu], size);
 }
 
 /* per-cell closure metrics partials: sum hypot(beta*duR), sum pdriveRaw, sum pdrive */
@@ -1327,6 +1336,7 @@ struct Binning {
      * per-cell incoming counts and per-slot rank among a cell's incoming */
     int *startB = nullptr, *listB = nullptr, *inc = nullptr, *rnk = nullptr;
     unsigned long long* keyB = nullptr;
+    int* cntS = nullptr; /* counts of the shifted collision cells */
 };
 
 }  // namespace
@@ -1466,6 +1476,7 @@ struct mpcd_gpu_sim {
         bin.inc = dalloc<int>((size_t)Nc + 1);
         bin.rnk = dalloc<int>(n);
         bin.keyB = dalloc<unsigned long long>(n);
+        bin.cntS = dalloc<int>((size_t)Nc + 1);
         nTiles = (int)((Nc + 1 + SCAN_TILE - 1) / SCAN_TILE);
         tileSum = dalloc<int>(nTiles);
         stage = dalloc<double>(2 * (size_t)n);
@@ -1548,7 +1559,7 @@ struct mpcd_gpu_sim {
     }
 
     ~mpcd_gpu_sim() {
-        void* ptrs[] = {x, y, vx, vy, vx2, vy2, x2, y2, pid, pid2, cid, cidS, bin.cnt, bin.start, bin.cursor, bin.list, bin.key, bin.startB, bin.listB, bin.inc, bin.rnk, bin.keyB, tileSum, next,
+        void* ptrs[] = {x, y, vx, vy, vx2, vy2, x2, y2, pid, pid2, cid, cidS, bin.cnt, bin.start, bin.cursor, bin.list, bin.key, bin.startB, bin.listB, bin.inc, bin.rnk, bin.keyB, bin.cntS, tileSum, next,
                         tmpg, zsize, zoff, owner[0], owner[1], cover[0], cover[1], overCells[0], overCells[1], stage,
                         refPx, refPy, refUx, refUy, pdriveRaw, redPx, redPy, redUx, redUy, duRX, duRY, duEX, duEY,
                         pdrive, cellTmp, mask, part, scal, err, wallPart, rects[0], rects[1], zout, zscratch, duG, recG, zstate,
@@ -1611,18 +1622,24 @@ struct mpcd_gpu_sim {
     }
 
     /* ---- binning: cnt/start/list of the cell ids in `key` ---- */
-    void bin_cells(const int* key) {
+    void bin_cells(const int* key) { bin_cells(key, bin.cnt, false, true); }
+    /* cnt: the counts (the unshifted binning keeps them in bin.cnt);
+     * counted: cnt already holds the histogram of key; keepKeys: the sorted
+     * keys are written back (the zone passes and the update read them) */
+    void bin_cells(const int* key, int* cnt, bool counted, bool keepKeys) {
         /* fills by kernel, not memset, so the sequence can be graph-captured */
-        hipLaunchKernelGGL(k_fill_int, dim3(nblk(Nc + 1)), dim3(TPB), 0, stream, bin.cnt, Nc + 1, 0);
+        if (!counted) {
+            hipLaunchKernelGGL(k_fill_int, dim3(nblk(Nc + 1)), dim3(TPB), 0, stream, cnt, Nc + 1, 0);
+            hipLaunchKernelGGL(k_hist, dim3(bin_blocks(n)), dim3(TPB), 0, stream, key, n, cnt);
+        }
         hipLaunchKernelGGL(k_fill_int, dim3(nblk(Nc + 1)), dim3(TPB), 0, stream, bin.cursor, Nc + 1, 0);
-        hipLaunchKernelGGL(k_hist, dim3(bin_blocks(n)), dim3(TPB), 0, stream, key, n, bin.cnt);
-        hipLaunchKernelGGL(k_scan_tiles, dim3(nTiles), dim3(TPB), 0, stream, bin.cnt, Nc + 1, bin.start, tileSum);
+        hipLaunchKernelGGL(k_scan_tiles, dim3(nTiles), dim3(TPB), 0, stream, cnt, Nc + 1, bin.start, tileSum);
         hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(TPB), 0, stream, tileSum, nTiles);
         hipLaunchKernelGGL(k_scan_add, dim3(nblk(Nc + 1)), dim3(TPB), 0, stream, bin.start, Nc + 1, tileSum);
         hipLaunchKernelGGL(k_scatter, dim3(bin_blocks(n)), dim3(TPB), 0, stream, key, pid, n, bin.start, bin.cursor,
                            bin.key);
         hipLaunchKernelGGL(k_segsort, dim3(nblk(Nc, SEG_CELLS)), dim3(SEG_CELLS), 0, stream, bin.start, Nc, bin.key,
-                           bin.list, (int)(n < (1 << 27)));
+                           bin.list, (int)(n < (1 << 27)), (int)keepKeys);
         check_launch();
     }
     void bin_unshifted() {
@@ -1893,9 +1910,11 @@ struct mpcd_gpu_sim {
             const uint64_t un = (uint64_t)fuseNext;
             double sh[2];
             mpcd_draw_shift(130363ULL + 10007ULL * un, p.a0, &sh[0], &sh[1]);
+            hipLaunchKernelGGL(k_fill_int, dim3(nblk(Nc + 1)), dim3(TPB), 0, stream, bin.cntS, Nc + 1, 0);
+            hipLaunchKernelGGL(k_fill_int, dim3(nblk(Nc + 1)), dim3(TPB), 0, stream, bin.cnt, Nc + 1, 0);
             hipLaunchKernelGGL(k_kick_reorder_stream, dim3(nblk(n)), dim3(TPB), 0, stream, g, bc,
                                104729ULL + 10007ULL * un, bin.list, x, y, vx, vy, pid, x2, y2, vx2, vy2, pid2, duRX,
-                               duRY, duEX, duEY, scal, cidS, cid, sh[0], sh[1], xmax, ymax, err);
+                               duRY, duEX, duEY, scal, cidS, cid, sh[0], sh[1], xmax, ymax, bin.cntS, bin.cnt, err);
             streamed = fuseNext;
         } else {
             hipLaunchKernelGGL(k_kick_reorder, dim3(nblk(n)), dim3(TPB), 0, stream, g, bin.list, x, y, vx, vy, pid,
@@ -2032,7 +2051,8 @@ struct mpcd_gpu_sim {
     void step(int s, mpcd_step_metrics* sm, double* const* stages, int next = -1) {
         check_pass_modes();
         const uint64_t us = (uint64_t)s;
-        if (streamed == s && !sm) {
+        const bool pre = streamed == s && !sm;
+        if (pre) {
             streamed = -1; /* the previous step's kick streamed this one */
         } else {
             if (streamed >= 0) throw ApiError(MPCD_E_INVALID, "internal: a fused stream was left unused");
@@ -2042,14 +2062,15 @@ struct mpcd_gpu_sim {
             stream_bc(104729ULL + 10007ULL * us, 1, sh, sm ? &sm->wall : nullptr);
             pend(MPCD_PROF_STREAM_BC);
         }
+        /* after a fused stream both histograms are done (the previous kick) */
         pbeg(MPCD_PROF_BIN);
-        bin_cells(cidS);
+        bin_cells(cidS, bin.cntS, pre, false);
         pend(MPCD_PROF_BIN);
         pbeg(MPCD_PROF_COLLIDE);
         collide(433494437ULL + 10007ULL * us);
         pend(MPCD_PROF_COLLIDE);
         pbeg(MPCD_PROF_BIN);
-        bin_cells(cid); /* collision does not move particles */
+        bin_cells(cid, bin.cnt, pre, true); /* collision does not move particles */
         pend(MPCD_PROF_BIN);
         if (stages && stages[0]) get_state(stages[0], stages[1]);
         pbeg(MPCD_PROF_CELL_FIELDS);
