@@ -945,12 +945,13 @@ __global__ void __launch_bounds__(FINISH_TPB) k_zone_finish(ZoneArgs a) {
 }
 
 /* The passes and the correction of a zone in one workgroup of FUSED_TPB
- * threads: wavefront 0 runs the passes while the others wait at the
- * barrier, then all run the correction.  A zone's correction starts as
+ * threads (four wavefronts of at most 128 registers, so four zones share a
+ * CU): wavefront 0 runs the passes while the others wait at the barrier,
+ * then all run the correction.  A zone's correction starts as
  * soon as its own passes end, beside the passes of slower zones. */
-constexpr int FUSED_TPB = 128;
+constexpr int FUSED_TPB = 256;
 template <class IT>
-__global__ void __launch_bounds__(FUSED_TPB, 2) k_zone_fused(ZoneArgs a) { /* 2 waves per SIMD: four zones per CU */
+__global__ void __launch_bounds__(FUSED_TPB, 4) k_zone_fused(ZoneArgs a) { /* 4 waves per SIMD: four zones per CU */
     extern __shared__ __align__(16) unsigned char lds[];
     __shared__ ZoneShared sh;
     __shared__ int queue[FUSED_TPB];
@@ -971,7 +972,7 @@ __global__ void __launch_bounds__(FUSED_TPB, 2) k_zone_fused(ZoneArgs a) { /* 2 
         __builtin_amdgcn_s_setprio(0);
     }
     __syncthreads();
-    zone_finish_body<IT, true, FUSED_TPB, 6>(a, zi, off, (int)threadIdx.x, lds, a.ldsBytes, queue);
+    zone_finish_body<IT, true, FUSED_TPB, 3>(a, zi, off, (int)threadIdx.x, lds, a.ldsBytes, queue);
 }
 
 /* velocities of the particles of cells several zones cover, from the side

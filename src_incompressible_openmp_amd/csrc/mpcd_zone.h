@@ -1906,7 +1906,7 @@ ZHD int in_cnt(const Z& z, int lc) {
  * walked twice.  The moved particles' links are set to their final cell for
  * phase 3, which places them.  Same sums in the same order as
  * zone_correct_sums / zone_correct_write. */
-constexpr int FIN_L = 24;
+constexpr int FIN_L = 8;
 
 /* column of particle k of cell lc's final list: its own input run lies in
  * lc's column */
