@@ -202,8 +202,9 @@ __global__ void __launch_bounds__(TPB) k_hist(const int* __restrict__ cid, int n
 
 /* exclusive scan, level 1: each block scans SCAN_TILE entries */
 __global__ void __launch_bounds__(TPB) k_scan_tiles(const int* __restrict__ in, int m, int* __restrict__ out,
-                                                    int* __restrict__ tileSum, const int* __restrict__ in2 = nullptr) {
-    /* in2 != nullptr: scans in + in2 */
+                                                    int* __restrict__ tileSum, const int* __restrict__ in2 = nullptr,
+                                                    int* __restrict__ zero = nullptr) {
+    /* in2 != nullptr: scans in + in2; zero != nullptr: cleared over the range */
     __shared__ int s[TPB];
     const int base = blockIdx.x * SCAN_TILE + threadIdx.x * 4;
     int v[4];
@@ -222,7 +223,10 @@ __global__ void __launch_bounds__(TPB) k_scan_tiles(const int* __restrict__ in, 
     }
     int run = s[threadIdx.x] - tot;
     for (int k = 0; k < 4; ++k) {
-        if (base + k < m) out[base + k] = run;
+        if (base + k < m) {
+            out[base + k] = run;
+            if (zero) zero[base + k] = 0;
+        }
         run += v[k];
     }
     if (threadIdx.x == TPB - 1) tileSum[blockIdx.x] = s[TPB - 1];
@@ -546,7 +550,7 @@ __global__ void __launch_bounds__(REB_CELLS) k_rebin_merge(const int* __restrict
                                                            const unsigned long long* __restrict__ key,
                                                            const int* __restrict__ cid, int Nc, int cpb,
                                                            const int* __restrict__ startB, int* __restrict__ cnt,
-                                                           const int* __restrict__ inc, unsigned long long* keyB,
+                                                           int* __restrict__ inc, unsigned long long* keyB,
                                                            int* __restrict__ listB, int* __restrict__ err) {
     __shared__ unsigned long long so[REB_CAP];
     const int c0 = blockIdx.x * cpb;
@@ -560,6 +564,7 @@ __global__ void __launch_bounds__(REB_CELLS) k_rebin_merge(const int* __restrict
         const int b = start[c], mo = start[c + 1] - b;
         nb = startB[c];
         const int ms = cnt[c], ki = inc[c];
+        inc[c] = 0; /* clear for the next update */
         m = ms + ki;
         if (lds)
             rebin_cell(c, b, mo, nb, ms, ki, key, cid, keyB, so, nb - q0, err);
@@ -1496,6 +1501,7 @@ struct mpcd_gpu_sim {
         wallPart = dalloc<WallAcc>(nblk(n));
         hipLaunchKernelGGL(k_iota, dim3(nblk(n)), dim3(TPB), 0, stream, pid, n);
         hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, stream, err, 1, 0); /* no error flagged yet */
+        hipLaunchKernelGGL(k_fill_int, dim3(nblk(Nc + 1)), dim3(TPB), 0, stream, bin.inc, Nc + 1, 0);
         check_launch();
         /* constants */
         bc = make_bc_params(p);
@@ -1633,8 +1639,9 @@ struct mpcd_gpu_sim {
             hipLaunchKernelGGL(k_fill_int, dim3(nblk(Nc + 1)), dim3(TPB), 0, stream, cnt, Nc + 1, 0);
             hipLaunchKernelGGL(k_hist, dim3(bin_blocks(n)), dim3(TPB), 0, stream, key, n, cnt);
         }
-        hipLaunchKernelGGL(k_fill_int, dim3(nblk(Nc + 1)), dim3(TPB), 0, stream, bin.cursor, Nc + 1, 0);
-        hipLaunchKernelGGL(k_scan_tiles, dim3(nTiles), dim3(TPB), 0, stream, cnt, Nc + 1, bin.start, tileSum);
+        /* the scan clears the scatter's cursors */
+        hipLaunchKernelGGL(k_scan_tiles, dim3(nTiles), dim3(TPB), 0, stream, cnt, Nc + 1, bin.start, tileSum,
+                           (const int*)nullptr, bin.cursor);
         hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(TPB), 0, stream, tileSum, nTiles);
         hipLaunchKernelGGL(k_scan_add, dim3(nblk(Nc + 1)), dim3(TPB), 0, stream, bin.start, Nc + 1, tileSum);
         hipLaunchKernelGGL(k_scatter, dim3(bin_blocks(n)), dim3(TPB), 0, stream, key, pid, n, bin.start, bin.cursor,
@@ -1654,7 +1661,7 @@ struct mpcd_gpu_sim {
      * is the merge of its stayers and its arrivals, in pid order.  The
      * result equals bin_unshifted()'s. */
     void rebin_unshifted() {
-        hipLaunchKernelGGL(k_fill_int, dim3(nblk(Nc + 1)), dim3(TPB), 0, stream, bin.inc, Nc + 1, 0);
+        /* bin.inc is zero here: cleared at creation and by every merge */
         hipLaunchKernelGGL(k_cells_delta, dim3(nblk(n)), dim3(TPB), 0, stream, g, x, y, cid, bin.cnt, bin.inc,
                            bin.rnk);
         hipLaunchKernelGGL(k_scan_tiles, dim3(nTiles), dim3(TPB), 0, stream, bin.cnt, Nc + 1, bin.startB, tileSum,
@@ -1767,9 +1774,12 @@ struct mpcd_gpu_sim {
         hipLaunchKernelGGL(k_zone_sizes, dim3(nblk(nZ + 1)), dim3(TPB), 0, stream, rectsDev, nZ, bin.start, p.Ny,
                            zsize);
         const int nt = (nZ + 1 + SCAN_TILE - 1) / SCAN_TILE;
-        hipLaunchKernelGGL(k_scan_tiles, dim3(nt), dim3(TPB), 0, stream, zsize, nZ + 1, zoff, tileSum);
-        hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(TPB), 0, stream, tileSum, nt);
-        hipLaunchKernelGGL(k_scan_add, dim3(nblk(nZ + 1)), dim3(TPB), 0, stream, zoff, nZ + 1, tileSum);
+        hipLaunchKernelGGL(k_scan_tiles, dim3(nt), dim3(TPB), 0, stream, zsize, nZ + 1, zoff, tileSum,
+                           (const int*)nullptr, (int*)nullptr);
+        if (nt > 1) { /* one tile is already the whole scan */
+            hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(TPB), 0, stream, tileSum, nt);
+            hipLaunchKernelGGL(k_scan_add, dim3(nblk(nZ + 1)), dim3(TPB), 0, stream, zoff, nZ + 1, tileSum);
+        }
         check_launch();
         ensure_du((size_t)nZ * 2 * (size_t)maxC);
         ZoneArgs a{};
