@@ -778,18 +778,21 @@ struct ZoneArgs {
 };
 
 /* particles of each zone: its columns' runs of the binning list */
-__global__ void k_zone_sizes(const ZoneRect* __restrict__ rects, int nZ, const int* __restrict__ start, int Ny,
-                             int* __restrict__ sizes) {
-    const int zi = blockIdx.x * blockDim.x + threadIdx.x;
+/* One wavefront per zone (block nZ writes the terminating zero): each lane
+ * takes columns of the zone, whose cells are one run of the binning. */
+__global__ void __launch_bounds__(64) k_zone_sizes(const ZoneRect* __restrict__ rects, int nZ,
+                                                   const int* __restrict__ start, int Ny, int* __restrict__ sizes) {
+    const int zi = blockIdx.x, lane = threadIdx.x;
     if (zi > nZ) return;
     if (zi == nZ) {
-        sizes[zi] = 0;
+        if (lane == 0) sizes[zi] = 0;
         return;
     }
     const ZoneRect r = rects[zi];
     int P = 0;
-    for (int gx = r.xs; gx < r.xe; ++gx) P += start[(r.ye - 1 + Ny * gx) + 1] - start[r.ys + Ny * gx];
-    sizes[zi] = P;
+    for (int gx = r.xs + lane; gx < r.xe; gx += 64) P += start[(r.ye - 1 + Ny * gx) + 1] - start[r.ys + Ny * gx];
+    for (int o = 32; o > 0; o >>= 1) P += __shfl_xor(P, o, 64);
+    if (lane == 0) sizes[zi] = P;
 }
 
 /* One zone per workgroup of one wavefront.  IT = uint16_t keeps the working
@@ -826,21 +829,24 @@ __device__ __forceinline__ void zone_pass_body(const ZoneArgs& a, int zi, long l
     }
 }
 
+/* A workgroup takes zones blockIdx.x, + gridDim.x, ... (the grid is
+ * smaller than the zone count when few zones need this form). */
 template <class IT, bool kLds>
 __global__ void __launch_bounds__(ZONE_TPB) k_zone(ZoneArgs a) {
     extern __shared__ __align__(16) unsigned char lds[];
     __shared__ ZoneShared sh;
-    const int zi = blockIdx.x;
-    if (zi >= a.nZones) return;
-    const int P = a.zsize[zi];
-    if (P < a.pMin || P > a.pMax) return; /* uniform: another launch takes this zone */
     const int lane = threadIdx.x;
-    const long long off = a.zoff[zi];
-    if (off + (long long)P > a.scratchCap) {
-        if (lane == 0) atomicOr(a.err, 2);
-        return;
+    for (int zi = blockIdx.x; zi < a.nZones; zi += gridDim.x) {
+        const int P = a.zsize[zi];
+        if (P < a.pMin || P > a.pMax) continue; /* uniform: another launch takes this zone */
+        const long long off = a.zoff[zi];
+        if (off + (long long)P > a.scratchCap) {
+            if (lane == 0) atomicOr(a.err, 2);
+            continue;
+        }
+        zone_pass_body<IT, kLds>(a, zi, off, lane, lds, &sh);
+        ZBAR(); /* the next zone reuses the working set */
     }
-    zone_pass_body<IT, kLds>(a, zi, off, lane, lds, &sh);
 }
 
 /* The momentum correction of the zones k_zone left (zone_pass.cpp:870-922),
@@ -940,13 +946,14 @@ template <class IT, bool kLds>
 __global__ void __launch_bounds__(FINISH_TPB) k_zone_finish(ZoneArgs a) {
     extern __shared__ __align__(16) unsigned char lds[];
     __shared__ int queue[FINISH_TPB];
-    const int zi = blockIdx.x;
-    if (zi >= a.nZones) return;
-    const int P = a.zsize[zi];
-    if (P < a.pMin || P > a.pMax) return;
-    const long long off = a.zoff[zi];
-    if (off + (long long)P > a.scratchCap) return;
-    zone_finish_body<IT, kLds, FINISH_TPB, 4>(a, zi, off, (int)threadIdx.x, lds, FINISH_LDS, queue);
+    for (int zi = blockIdx.x; zi < a.nZones; zi += gridDim.x) {
+        const int P = a.zsize[zi];
+        if (P < a.pMin || P > a.pMax) continue;
+        const long long off = a.zoff[zi];
+        if (off + (long long)P > a.scratchCap) continue;
+        zone_finish_body<IT, kLds, FINISH_TPB, 4>(a, zi, off, (int)threadIdx.x, lds, FINISH_LDS, queue);
+        __syncthreads(); /* the next zone reuses the staging */
+    }
 }
 
 /* The passes and the correction of a zone in one workgroup of FUSED_TPB
@@ -1771,8 +1778,7 @@ struct mpcd_gpu_sim {
     void zone_pass(uint64_t seedBase, const ZoneRect* rectsDev, int nZ, int maxC, int maxW, const int* ownerDev,
                    const unsigned char* coverDev, const int* overDev, int nOverCells, int single,
                    const double* inPx = nullptr, const double* inPy = nullptr) {
-        hipLaunchKernelGGL(k_zone_sizes, dim3(nblk(nZ + 1)), dim3(TPB), 0, stream, rectsDev, nZ, bin.start, p.Ny,
-                           zsize);
+        hipLaunchKernelGGL(k_zone_sizes, dim3(nZ + 1), dim3(64), 0, stream, rectsDev, nZ, bin.start, p.Ny, zsize);
         const int nt = (nZ + 1 + SCAN_TILE - 1) / SCAN_TILE;
         hipLaunchKernelGGL(k_scan_tiles, dim3(nt), dim3(TPB), 0, stream, zsize, nZ + 1, zoff, tileSum,
                            (const int*)nullptr, (int*)nullptr);
@@ -1834,13 +1840,15 @@ struct mpcd_gpu_sim {
                 hipLaunchKernelGGL((k_zone<uint16_t, true>), dim3(nZ), dim3(ZONE_TPB), ZONE_LDS_FAST, stream, a);
             check_launch();
         }
-        /* zones the uint16_t path does not take (none, as a rule) */
+        /* zones the uint16_t path does not take (none, as a rule: a small grid
+         * strides over the zones then; all zones when the 16-bit form is out) */
+        const int nBig = fast ? std::min(nZ, 64) : nZ;
         a.pMin = fast ? 65535 : 0;
         a.pMax = 0x7fffffff;
         const size_t fix32 = zone_fixed_bytes<int>(maxC, maxW, zp.useLocal);
         if (fix32 <= (size_t)ZONE_LDS_CAP) {
             a.ldsBytes = ZONE_LDS_CAP;
-            hipLaunchKernelGGL((k_zone<int, true>), dim3(nZ), dim3(ZONE_TPB), ZONE_LDS_CAP, stream, a);
+            hipLaunchKernelGGL((k_zone<int, true>), dim3(nBig), dim3(ZONE_TPB), ZONE_LDS_CAP, stream, a);
         } else {
             const size_t need = fix32 * (size_t)nZ;
             if (need > zscratchCap) {
@@ -1850,7 +1858,7 @@ struct mpcd_gpu_sim {
             }
             a.gscratch = zscratch;
             a.gstride = fix32;
-            hipLaunchKernelGGL((k_zone<int, false>), dim3(nZ), dim3(ZONE_TPB), 0, stream, a);
+            hipLaunchKernelGGL((k_zone<int, false>), dim3(nBig), dim3(ZONE_TPB), 0, stream, a);
         }
         pend(MPCD_PROF_ZONE_KERNEL);
         check_launch();
@@ -1867,9 +1875,9 @@ struct mpcd_gpu_sim {
         a.pMin = fast ? 65535 : 0;
         a.pMax = 0x7fffffff;
         if (cellB32 + 4096 <= (size_t)FINISH_LDS)
-            hipLaunchKernelGGL((k_zone_finish<int, true>), dim3(nZ), dim3(FINISH_TPB), FINISH_LDS, stream, a);
+            hipLaunchKernelGGL((k_zone_finish<int, true>), dim3(nBig), dim3(FINISH_TPB), FINISH_LDS, stream, a);
         else
-            hipLaunchKernelGGL((k_zone_finish<int, false>), dim3(nZ), dim3(FINISH_TPB), 0, stream, a);
+            hipLaunchKernelGGL((k_zone_finish<int, false>), dim3(nBig), dim3(FINISH_TPB), 0, stream, a);
         pend(MPCD_PROF_ZONE_FINISH);
         check_launch();
         if (!single && coverDev && nOverCells > 0) {
