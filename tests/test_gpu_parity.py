@@ -447,3 +447,22 @@ def test_clustered_steps_in_one_call(gpu):
     xa, va = a.get_state()
     xb, vb = b.get_state()
     assert bits(xa, xb) and bits(va, vb)
+
+
+@pytest.mark.parametrize("layout", [LAYOUT_BASE, LAYOUT_SHIFTED])
+def test_zone_pass_big_zones(gpu, oracle, layout):
+    """25x25-cell zones of about 69,000 particles (110 per cell): the cell
+    arrays fit the 16-bit form but the particle count does not, so these
+    zones take the 32-bit kernels on their small strided grid while the
+    smaller edge zones of the shifted layout take the fused kernel."""
+    p = wide_params(Nx=50, Ny=25, gamma=110.0, zoneTileNx=25, zoneTileNy=25, maxRedistribPasses=1)
+    x, v = wide_post_collision(oracle, p, 13)
+    xo, vo = x.copy(), v.copy()
+    mo, owo = oracle.zone_pass(CTR, p, xo, vo, layout, 88172645463325252)
+    xg, vg = x.copy(), v.copy()
+    mg, owg = gpu.zone_pass(p, xg, vg, layout, 88172645463325252)
+    print("big zones moved", mo["nParticlesMovedDense"], mo["nParticlesMovedSparse"], flush=True)
+    assert np.array_equal(owg, owo)
+    assert mg["nParticlesMovedDense"] == mo["nParticlesMovedDense"]
+    assert mg["nParticlesMovedSparse"] == mo["nParticlesMovedSparse"]
+    assert bits(xg, xo) and bits(vg, vo)
