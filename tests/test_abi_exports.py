@@ -56,3 +56,18 @@ def test_error_path_without_gpu_is_an_error_code(lib):
         assert n.value >= 0
     else:
         assert rc < 0 and lib.mpcd_gpu_last_error()
+
+
+def test_profile_classes_match_header():
+    """The Python binding names one timing class per MPCD_PROF_* entry, in
+    the header's order."""
+    import re
+
+    from src_incompressible_openmp_amd import gpu
+
+    text = open(os.path.join(INCLUDE, "mpcd_gpu.h")).read()
+    entries = dict((k, int(v)) for k, v in re.findall(r"MPCD_PROF_([A-Z_]+)\s*=\s*(\d+)", text))
+    n = entries.pop("CLASSES")
+    assert n == len(gpu.PROF_CLASSES)
+    for name, idx in entries.items():
+        assert gpu.PROF_CLASSES[idx] == name.lower(), (name, idx)
