@@ -203,14 +203,15 @@ __global__ void __launch_bounds__(TPB) k_hist(const int* __restrict__ cid, int n
 /* exclusive scan, level 1: each block scans SCAN_TILE entries */
 __global__ void __launch_bounds__(TPB) k_scan_tiles(const int* __restrict__ in, int m, int* __restrict__ out,
                                                     int* __restrict__ tileSum, const int* __restrict__ in2 = nullptr,
-                                                    int* __restrict__ zero = nullptr) {
-    /* in2 != nullptr: scans in + in2; zero != nullptr: cleared over the range */
+                                                    int* __restrict__ zero = nullptr,
+                                                    const int* __restrict__ sub = nullptr) {
+    /* in2 != nullptr: scans in + in2 (- sub); zero != nullptr: cleared over the range */
     __shared__ int s[TPB];
     const int base = blockIdx.x * SCAN_TILE + threadIdx.x * 4;
     int v[4];
     int tot = 0;
     for (int k = 0; k < 4; ++k) {
-        v[k] = (base + k < m) ? in[base + k] + (in2 ? in2[base + k] : 0) : 0;
+        v[k] = (base + k < m) ? in[base + k] + (in2 ? in2[base + k] : 0) - (sub ? sub[base + k] : 0) : 0;
         tot += v[k];
     }
     s[threadIdx.x] = tot;
@@ -413,31 +414,13 @@ __global__ void __launch_bounds__(SEG_CELLS) k_segsort(const int* __restrict__ s
 
 /* ---- incremental unshifted binning ---- */
 
-/* The new cell of each slot against the binned one: a particle that
- * changed cell leaves its old cell's count and takes a rank among its new
- * cell's arrivals (rnk = -1 for a stayer). */
-__global__ void k_cells_delta(Geom g, const double* __restrict__ x, const double* __restrict__ y,
-                              int* __restrict__ cid, int* __restrict__ cnt, int* __restrict__ inc,
-                              int* __restrict__ rnk) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= g.n) return;
-    const int cn = cell_of(g, x[i], y[i]);
-    const int co = cid[i];
-    int r = -1;
-    if (cn != co) {
-        cid[i] = cn;
-        atomicSub(&cnt[co], 1);
-        r = atomicAdd(&inc[cn], 1);
-    }
-    rnk[i] = r;
-}
-
 /* Arrivals go to the tail of their new cell's run (after the stayers), in
  * arrival order; the merge sorts them. */
 constexpr int PLACE_PER = 4; /* slots per thread (one 16-byte load of ranks) */
 __global__ void k_place_arrivals(const int* __restrict__ cid, const int* __restrict__ pid,
                                  const int* __restrict__ rnk, int n, const int* __restrict__ startB,
-                                 const int* __restrict__ stay, unsigned long long* __restrict__ keyB) {
+                                 const int* __restrict__ cnt, const int* __restrict__ dout,
+                                 unsigned long long* __restrict__ keyB) {
     const int i0 = (blockIdx.x * blockDim.x + threadIdx.x) * PLACE_PER;
     if (i0 >= n) return;
     int r[PLACE_PER];
@@ -454,7 +437,7 @@ __global__ void k_place_arrivals(const int* __restrict__ cid, const int* __restr
         if (r[u] < 0) continue;
         const int i = i0 + u;
         const int c = cid[i];
-        keyB[startB[c] + stay[c] + r[u]] = ((unsigned long long)(unsigned)pid[i] << 32) | (unsigned)i;
+        keyB[startB[c] + cnt[c] - dout[c] + r[u]] = ((unsigned long long)(unsigned)pid[i] << 32) | (unsigned)i;
     }
 }
 
@@ -550,7 +533,8 @@ __global__ void __launch_bounds__(REB_CELLS) k_rebin_merge(const int* __restrict
                                                            const unsigned long long* __restrict__ key,
                                                            const int* __restrict__ cid, int Nc, int cpb,
                                                            const int* __restrict__ startB, int* __restrict__ cnt,
-                                                           int* __restrict__ inc, unsigned long long* keyB,
+                                                           int* __restrict__ inc, int* __restrict__ dout,
+                                                           unsigned long long* keyB,
                                                            int* __restrict__ listB, int* __restrict__ err) {
     __shared__ unsigned long long so[REB_CAP];
     const int c0 = blockIdx.x * cpb;
@@ -563,8 +547,9 @@ __global__ void __launch_bounds__(REB_CELLS) k_rebin_merge(const int* __restrict
     if (mine) {
         const int b = start[c], mo = start[c + 1] - b;
         nb = startB[c];
-        const int ms = cnt[c], ki = inc[c];
+        const int ms = cnt[c] - dout[c], ki = inc[c];
         inc[c] = 0; /* clear for the next update */
+        dout[c] = 0;
         m = ms + ki;
         if (lds)
             rebin_cell(c, b, mo, nb, ms, ki, key, cid, keyB, so, nb - q0, err);
@@ -1350,6 +1335,7 @@ struct Binning {
     int *startB = nullptr, *listB = nullptr, *inc = nullptr, *rnk = nullptr;
     unsigned long long* keyB = nullptr;
     int* cntS = nullptr; /* counts of the shifted collision cells */
+    int* dout = nullptr; /* departures per cell recorded by a zone pass (zero between updates) */
 };
 
 }  // namespace
@@ -1490,6 +1476,7 @@ struct mpcd_gpu_sim {
         bin.rnk = dalloc<int>(n);
         bin.keyB = dalloc<unsigned long long>(n);
         bin.cntS = dalloc<int>((size_t)Nc + 1);
+        bin.dout = dalloc<int>((size_t)Nc + 1);
         nTiles = (int)((Nc + 1 + SCAN_TILE - 1) / SCAN_TILE);
         tileSum = dalloc<int>(nTiles);
         stage = dalloc<double>(2 * (size_t)n);
@@ -1509,6 +1496,7 @@ struct mpcd_gpu_sim {
         hipLaunchKernelGGL(k_iota, dim3(nblk(n)), dim3(TPB), 0, stream, pid, n);
         hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, stream, err, 1, 0); /* no error flagged yet */
         hipLaunchKernelGGL(k_fill_int, dim3(nblk(Nc + 1)), dim3(TPB), 0, stream, bin.inc, Nc + 1, 0);
+        hipLaunchKernelGGL(k_fill_int, dim3(nblk(Nc + 1)), dim3(TPB), 0, stream, bin.dout, Nc + 1, 0);
         check_launch();
         /* constants */
         bc = make_bc_params(p);
@@ -1573,7 +1561,7 @@ struct mpcd_gpu_sim {
     }
 
     ~mpcd_gpu_sim() {
-        void* ptrs[] = {x, y, vx, vy, vx2, vy2, x2, y2, pid, pid2, cid, cidS, bin.cnt, bin.start, bin.cursor, bin.list, bin.key, bin.startB, bin.listB, bin.inc, bin.rnk, bin.keyB, bin.cntS, tileSum, next,
+        void* ptrs[] = {x, y, vx, vy, vx2, vy2, x2, y2, pid, pid2, cid, cidS, bin.cnt, bin.start, bin.cursor, bin.list, bin.key, bin.startB, bin.listB, bin.inc, bin.rnk, bin.keyB, bin.cntS, bin.dout, tileSum, next,
                         tmpg, zsize, zoff, owner[0], owner[1], cover[0], cover[1], overCells[0], overCells[1], stage,
                         refPx, refPy, refUx, refUy, pdriveRaw, redPx, redPy, redUx, redUy, duRX, duRY, duEX, duEY,
                         pdrive, cellTmp, mask, part, scal, err, wallPart, rects[0], rects[1], zout, zscratch, duG, recG, zstate,
@@ -1662,26 +1650,23 @@ struct mpcd_gpu_sim {
         check_launch();
         bin_cells(cid);
     }
-    /* The unshifted binning of the current positions, from the unshifted
-     * binning of earlier positions of the same slots (cid and bin hold it):
-     * only the particles that changed cell are placed, and each cell's run
-     * is the merge of its stayers and its arrivals, in pid order.  The
-     * result equals bin_unshifted()'s. */
-    void rebin_unshifted() {
-        /* bin.inc is zero here: cleared at creation and by every merge */
-        hipLaunchKernelGGL(k_cells_delta, dim3(nblk(n)), dim3(TPB), 0, stream, g, x, y, cid, bin.cnt, bin.inc,
-                           bin.rnk);
+    /* The unshifted binning after a zone pass that recorded its cell changes
+     * (bin.rnk set to -1 before it, then each moved particle's new cell in
+     * cid, its departure in dout, its rank among its new cell's arrivals):
+     * the runs are the old runs' stayers merged with the arrivals, in pid
+     * order, the same binning a counting sort of the new cells gives. */
+    void rebin_recorded() {
         hipLaunchKernelGGL(k_scan_tiles, dim3(nTiles), dim3(TPB), 0, stream, bin.cnt, Nc + 1, bin.startB, tileSum,
-                           bin.inc);
+                           bin.inc, (int*)nullptr, bin.dout);
         hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(TPB), 0, stream, tileSum, nTiles);
         hipLaunchKernelGGL(k_scan_add, dim3(nblk(Nc + 1)), dim3(TPB), 0, stream, bin.startB, Nc + 1, tileSum);
         hipLaunchKernelGGL(k_place_arrivals, dim3(nblk((n + PLACE_PER - 1) / PLACE_PER)), dim3(TPB), 0, stream, cid, pid,
-                           bin.rnk, n, bin.startB, bin.cnt, bin.keyB);
+                           bin.rnk, n, bin.startB, bin.cnt, bin.dout, bin.keyB);
         /* cells per block: the block's runs fit the LDS staging at 1.25x the mean density */
         int cpb = REB_CELLS;
         while (cpb > 64 && (double)cpb * (double)n / (double)Nc * 1.25 > (double)REB_CAP) cpb >>= 1;
         hipLaunchKernelGGL(k_rebin_merge, dim3(nblk(Nc, cpb)), dim3(REB_CELLS), 0, stream, bin.start, bin.key, cid,
-                           Nc, cpb, bin.startB, bin.cnt, bin.inc, bin.keyB, bin.listB, err);
+                           Nc, cpb, bin.startB, bin.cnt, bin.inc, bin.dout, bin.keyB, bin.listB, err);
         check_launch();
         std::swap(bin.start, bin.startB);
         std::swap(bin.key, bin.keyB);
@@ -1777,7 +1762,7 @@ struct mpcd_gpu_sim {
      * particles, seeded with seedBase itself (run_zone_kernel_inplace). */
     void zone_pass(uint64_t seedBase, const ZoneRect* rectsDev, int nZ, int maxC, int maxW, const int* ownerDev,
                    const unsigned char* coverDev, const int* overDev, int nOverCells, int single,
-                   const double* inPx = nullptr, const double* inPy = nullptr) {
+                   const double* inPx = nullptr, const double* inPy = nullptr, bool record = false) {
         hipLaunchKernelGGL(k_zone_sizes, dim3(nZ + 1), dim3(64), 0, stream, rectsDev, nZ, bin.start, p.Ny, zsize);
         const int nt = (nZ + 1 + SCAN_TILE - 1) / SCAN_TILE;
         hipLaunchKernelGGL(k_scan_tiles, dim3(nt), dim3(TPB), 0, stream, zsize, nZ + 1, zoff, tileSum,
@@ -1805,6 +1790,14 @@ struct mpcd_gpu_sim {
         a.in.vsy = vy2;
         a.in.inPx = inPx;
         a.in.inPy = inPy;
+        if (record) {
+            /* moved particles record their cell changes for rebin_recorded() */
+            hipLaunchKernelGGL(k_fill_int, dim3(nblk(n)), dim3(TPB), 0, stream, bin.rnk, n, -1);
+            a.in.dcid = cid;
+            a.in.dout = bin.dout;
+            a.in.dinc = bin.inc;
+            a.in.drnk = bin.rnk;
+        }
         a.rects = rectsDev;
         a.nZones = nZ;
         a.seedBase = seedBase;
@@ -1887,9 +1880,10 @@ struct mpcd_gpu_sim {
         }
     }
 
-    void layout_pass(int L, uint64_t seedBase, const double* inPx = nullptr, const double* inPy = nullptr) {
+    void layout_pass(int L, uint64_t seedBase, const double* inPx = nullptr, const double* inPy = nullptr,
+                     bool record = false) {
         zone_pass(seedBase, rects[L], nz[L], maxCells[L], maxCols[L], owner[L], cover[L], overCells[L], nOver[L], 0,
-                  inPx, inPy);
+                  inPx, inPy, record);
     }
 
     void pass_metrics(int nZ, mpcd_zone_pass_metrics* m) {
@@ -2100,20 +2094,20 @@ struct mpcd_gpu_sim {
         /* the base pass's input is the state the reference fields were
          * summed over (same runs, same order): the correction takes its
          * input sums from them */
-        layout_pass(0, zs, refPx, refPy);
+        layout_pass(0, zs, refPx, refPy, true);
         pend(MPCD_PROF_ZONE_PASS);
         if (sm) pass_metrics(nz[0], &sm->base);
         /* the passes move a few particles: the binning is updated */
         pbeg(MPCD_PROF_BIN);
-        rebin_unshifted();
+        rebin_recorded();
         pend(MPCD_PROF_BIN);
         if (stages && stages[2]) get_state(stages[2], stages[3]);
         pbeg(MPCD_PROF_ZONE_PASS);
-        layout_pass(1, zs + 4099ULL);
+        layout_pass(1, zs + 4099ULL, nullptr, nullptr, true);
         pend(MPCD_PROF_ZONE_PASS);
         if (sm) pass_metrics(nz[1], &sm->shifted);
         pbeg(MPCD_PROF_BIN);
-        rebin_unshifted();
+        rebin_recorded();
         pend(MPCD_PROF_BIN);
         if (stages && stages[4]) get_state(stages[4], stages[5]);
         pbeg(MPCD_PROF_CLOSURE);

@@ -192,6 +192,10 @@ struct ZoneIn {
     /* per-cell velocity sums of the pass input (each cell's run summed in
      * order from zero), or nullptr: the correction sums the runs itself */
     const double *inPx = nullptr, *inPy = nullptr;
+    /* non-null: each moved particle's cell change is recorded for the
+     * binning update (cell of each slot, departures and arrivals per cell,
+     * rank among the new cell's arrivals) */
+    int *dcid = nullptr, *dout = nullptr, *dinc = nullptr, *drnk = nullptr;
 };
 
 /* Scalars the lanes share (LDS on the device). */
@@ -214,6 +218,7 @@ struct ZoneCtxT {
     const unsigned long long* gkey;
     double *x, *y, *vx, *vy, *vsx, *vsy;
     const double *inPx, *inPy; /* input run sums per global cell, or nullptr */
+    int *dcid, *dout, *dinc, *drnk; /* cell-change record, or nullptr */
     IT* nxL;     /* links of k < nL */
     IT* nxG;     /* links of k >= nL (zone's global region, indexed by k) */
     IT* tmpG;    /* donor lists longer than ZONE_TMP_L (zone's global region) */
@@ -608,8 +613,25 @@ ZHD void place_moved(Z& z, int lc, int slot, uint32_t pid, int tag) {
     if (tag == ZM_SPARSE) mpcd_draw_sparse_pos(z.seed, pid, &ux, &uy);
     else mpcd_draw_dense_pos(z.seed, tag, pid, &ux, &uy);
     const int ix = lc / z.zh;
-    z.x[slot] = (double)(z.xs + ix) * z.g.dxc + z.g.dxc * ux;
-    z.y[slot] = (double)(z.ys + (lc - ix * z.zh)) * z.g.dyc + z.g.dyc * uy;
+    const double nx = (double)(z.xs + ix) * z.g.dxc + z.g.dxc * ux;
+    const double ny = (double)(z.ys + (lc - ix * z.zh)) * z.g.dyc + z.g.dyc * uy;
+    z.x[slot] = nx;
+    z.y[slot] = ny;
+    if (z.dcid) {
+        /* the cell the binning update will see (a particle is placed once
+         * per pass); the pass's own counts stay untouched */
+        const int cn = cell_of(z.g, nx, ny), co = z.dcid[slot];
+        if (cn != co) {
+            z.dcid[slot] = cn;
+#if defined(__HIP_DEVICE_COMPILE__)
+            atomicAdd(&z.dout[co], 1);
+            z.drnk[slot] = atomicAdd(&z.dinc[cn], 1);
+#else
+            ++z.dout[co];
+            z.drnk[slot] = z.dinc[cn]++;
+#endif
+        }
+    }
 }
 
 /* the bulk/skip masks of a transfer (zone_pass.cpp:492-525) */
@@ -1782,6 +1804,10 @@ ZHD void zone_init(ZoneCtxT<IT>& z, const ZoneIn& in, ZoneRect r, int zid, uint6
     z.gkey = in.gkey;
     z.inPx = in.inPx;
     z.inPy = in.inPy;
+    z.dcid = in.dcid;
+    z.dout = in.dout;
+    z.dinc = in.dinc;
+    z.drnk = in.drnk;
     z.x = in.x;
     z.y = in.y;
     z.vx = in.vx;
