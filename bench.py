@@ -211,7 +211,7 @@ def main() -> int:
 
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
-              f"(the multi-GPU decomposition is described in DESIGN.md, not implemented)", file=sys.stderr)
+              f"(the decomposition into x-slabs over several GPUs is designed, not implemented)", file=sys.stderr)
         return 2
 
     prm = poiseuille_params(Nx=args.nx, Ny=args.ny, gamma=float(args.ppc))
