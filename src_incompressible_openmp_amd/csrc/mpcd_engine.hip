@@ -534,7 +534,7 @@ __global__ void __launch_bounds__(REB_CELLS) k_rebin_merge(const int* __restrict
                                                            const int* __restrict__ cid, int Nc, int cpb,
                                                            const int* __restrict__ startB, int* __restrict__ cnt,
                                                            int* __restrict__ inc, int* __restrict__ dout,
-                                                           unsigned long long* keyB,
+                                                           int* __restrict__ rnk, unsigned long long* keyB,
                                                            int* __restrict__ listB, int* __restrict__ err) {
     __shared__ unsigned long long so[REB_CAP];
     const int c0 = blockIdx.x * cpb;
@@ -550,6 +550,7 @@ __global__ void __launch_bounds__(REB_CELLS) k_rebin_merge(const int* __restrict
         const int ms = cnt[c] - dout[c], ki = inc[c];
         inc[c] = 0; /* clear for the next update */
         dout[c] = 0;
+        for (int j = 0; j < ki; ++j) rnk[(int)(unsigned)(keyB[nb + ms + j] & 0xffffffffull)] = -1;
         m = ms + ki;
         if (lds)
             rebin_cell(c, b, mo, nb, ms, ki, key, cid, keyB, so, nb - q0, err);
@@ -1497,6 +1498,7 @@ struct mpcd_gpu_sim {
         hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, stream, err, 1, 0); /* no error flagged yet */
         hipLaunchKernelGGL(k_fill_int, dim3(nblk(Nc + 1)), dim3(TPB), 0, stream, bin.inc, Nc + 1, 0);
         hipLaunchKernelGGL(k_fill_int, dim3(nblk(Nc + 1)), dim3(TPB), 0, stream, bin.dout, Nc + 1, 0);
+        hipLaunchKernelGGL(k_fill_int, dim3(nblk(n)), dim3(TPB), 0, stream, bin.rnk, n, -1);
         check_launch();
         /* constants */
         bc = make_bc_params(p);
@@ -1651,7 +1653,7 @@ struct mpcd_gpu_sim {
         bin_cells(cid);
     }
     /* The unshifted binning after a zone pass that recorded its cell changes
-     * (bin.rnk set to -1 before it, then each moved particle's new cell in
+     * (bin.rnk -1 for every slot before it, then each moved particle's new cell in
      * cid, its departure in dout, its rank among its new cell's arrivals):
      * the runs are the old runs' stayers merged with the arrivals, in pid
      * order, the same binning a counting sort of the new cells gives. */
@@ -1666,7 +1668,7 @@ struct mpcd_gpu_sim {
         int cpb = REB_CELLS;
         while (cpb > 64 && (double)cpb * (double)n / (double)Nc * 1.25 > (double)REB_CAP) cpb >>= 1;
         hipLaunchKernelGGL(k_rebin_merge, dim3(nblk(Nc, cpb)), dim3(REB_CELLS), 0, stream, bin.start, bin.key, cid,
-                           Nc, cpb, bin.startB, bin.cnt, bin.inc, bin.dout, bin.keyB, bin.listB, err);
+                           Nc, cpb, bin.startB, bin.cnt, bin.inc, bin.dout, bin.rnk, bin.keyB, bin.listB, err);
         check_launch();
         std::swap(bin.start, bin.startB);
         std::swap(bin.key, bin.keyB);
@@ -1791,8 +1793,9 @@ struct mpcd_gpu_sim {
         a.in.inPx = inPx;
         a.in.inPy = inPy;
         if (record) {
-            /* moved particles record their cell changes for rebin_recorded() */
-            hipLaunchKernelGGL(k_fill_int, dim3(nblk(n)), dim3(TPB), 0, stream, bin.rnk, n, -1);
+            /* moved particles record their cell changes for rebin_recorded()
+             * (bin.rnk is -1 for every slot here: set at creation, and each
+             * merge resets the ranks of the arrivals it took) */
             a.in.dcid = cid;
             a.in.dout = bin.dout;
             a.in.dinc = bin.inc;
