@@ -670,8 +670,9 @@ __global__ void __launch_bounds__(TPB) k_sum2(const double* __restrict__ a, cons
 }
 
 /* single-block reduction of `np` partial tuples of width w into out[0..w) */
-__global__ void __launch_bounds__(TPB) k_reduce_parts(const double* __restrict__ part, int np, int w,
-                                                      double* __restrict__ out) {
+/* column sums of part[np][w] by one block (fixed order: strided partial
+ * sums, then a tree), written by thread 0 */
+__device__ inline void reduce_parts_block(const double* __restrict__ part, int np, int w, double* __restrict__ out) {
     __shared__ double s[TPB];
     for (int k = 0; k < w; ++k) {
         double acc = 0.0;
@@ -685,6 +686,11 @@ __global__ void __launch_bounds__(TPB) k_reduce_parts(const double* __restrict__
         if (threadIdx.x == 0) out[k] = s[0];
         __syncthreads();
     }
+}
+
+__global__ void __launch_bounds__(TPB) k_reduce_parts(const double* __restrict__ part, int np, int w,
+                                                      double* __restrict__ out) {
+    reduce_parts_block(part, np, w, out);
 }
 
 __global__ void k_sub_mean(double* __restrict__ vx, double* __restrict__ vy, int n, const double* __restrict__ sums) {
@@ -1031,8 +1037,9 @@ __global__ void __launch_bounds__(TPB) k_closure_repair(ClosureArgs a) {
 
 /* beta from (num, den) (liquid_closure.cpp:364-377): scal[0..1] = num, den;
  * writes scal[2] = betaOpt, scal[3] = betaApplied. */
-__global__ void k_beta(double* scal, int useOptimal, double betaRepair) {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+/* beta from the repair sums scal[0] (num), scal[1] (den): scal[2] the
+ * optimum, scal[3] the applied value */
+__device__ inline void beta_from_sums(double* scal, int useOptimal, double betaRepair) {
     const double num = scal[0], den = scal[1];
     double bopt = 0.0;
     if (isfinite(num) && isfinite(den) && den > 0.0) bopt = dmax(num / den, 0.0);
@@ -1042,6 +1049,13 @@ __global__ void k_beta(double* scal, int useOptimal, double betaRepair) {
     else app = (bopt < 0.0) ? 0.0 : ((2.0 < bopt) ? 2.0 : bopt);
     scal[2] = bopt;
     scal[3] = app;
+}
+
+/* the repair sums and beta in one block (thread 0 wrote the sums) */
+__global__ void __launch_bounds__(TPB) k_reduce_beta(const double* __restrict__ part, int np, double* scal,
+                                                     int useOptimal, double betaRepair) {
+    reduce_parts_block(part, np, 2, scal);
+    if (threadIdx.x == 0) beta_from_sums(scal, useOptimal, betaRepair);
 }
 
 /* pdrive with the interzone cross smoothing (liquid_closure.cpp:401-419) */
@@ -2006,8 +2020,8 @@ struct mpcd_gpu_sim {
         ca.part = part;
         const int nbc = (int)nblk(Nc);
         hipLaunchKernelGGL(k_closure_repair, dim3(nbc), dim3(TPB), 0, stream, ca);
-        hipLaunchKernelGGL(k_reduce_parts, dim3(1), dim3(TPB), 0, stream, part, nbc, 2, scal);
-        hipLaunchKernelGGL(k_beta, dim3(1), dim3(64), 0, stream, scal, p.useOptimalBetaRepair, p.betaRepair);
+        hipLaunchKernelGGL(k_reduce_beta, dim3(1), dim3(TPB), 0, stream, part, nbc, scal, p.useOptimalBetaRepair,
+                           p.betaRepair);
         hipLaunchKernelGGL(k_pdrive, dim3(nbc), dim3(TPB), 0, stream, g, pdriveRaw, mask, p.smoothPdriveInterzoneBlend,
                            anyMask, pdrive);
         const double dtr = p.dt / rho_target();
