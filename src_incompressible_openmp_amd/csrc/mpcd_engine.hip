@@ -263,6 +263,26 @@ __global__ void k_scan_add(int* __restrict__ out, int m, const int* __restrict__
     if (i < m) out[i] += tileSum[i / SCAN_TILE];
 }
 
+/* levels 2 and 3 in one: each block sums the totals of the tiles before
+ * its own (integers, any order) and adds them (blocks of TPB elements lie
+ * in one tile) */
+__global__ void __launch_bounds__(TPB) k_scan_add_prefix(int* __restrict__ out, int m,
+                                                         const int* __restrict__ tileSum) {
+    static_assert(SCAN_TILE % TPB == 0, "a block lies in one tile");
+    __shared__ int s[TPB];
+    const int i = blockIdx.x * TPB + (int)threadIdx.x;
+    const int t = (blockIdx.x * TPB) / SCAN_TILE;
+    int acc = 0;
+    for (int j = (int)threadIdx.x; j < t; j += TPB) acc += tileSum[j];
+    s[threadIdx.x] = acc;
+    __syncthreads();
+    for (int st = TPB / 2; st > 0; st >>= 1) {
+        if (threadIdx.x < (unsigned)st) s[threadIdx.x] += s[threadIdx.x + st];
+        __syncthreads();
+    }
+    if (i < m) out[i] += s[0];
+}
+
 /* Scatter (pid, slot) keys into their cell's run: one returning atomic per
  * key group, all groups in one instruction, the group base passed back from
  * its leader. */
@@ -1653,8 +1673,7 @@ struct mpcd_gpu_sim {
         /* the scan clears the scatter's cursors */
         hipLaunchKernelGGL(k_scan_tiles, dim3(nTiles), dim3(TPB), 0, stream, cnt, Nc + 1, bin.start, tileSum,
                            (const int*)nullptr, bin.cursor);
-        hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(TPB), 0, stream, tileSum, nTiles);
-        hipLaunchKernelGGL(k_scan_add, dim3(nblk(Nc + 1)), dim3(TPB), 0, stream, bin.start, Nc + 1, tileSum);
+        hipLaunchKernelGGL(k_scan_add_prefix, dim3(nblk(Nc + 1)), dim3(TPB), 0, stream, bin.start, Nc + 1, tileSum);
         hipLaunchKernelGGL(k_scatter, dim3(bin_blocks(n)), dim3(TPB), 0, stream, key, pid, n, bin.start, bin.cursor,
                            bin.key);
         hipLaunchKernelGGL(k_segsort, dim3(nblk(Nc, SEG_CELLS)), dim3(SEG_CELLS), 0, stream, bin.start, Nc, bin.key,
@@ -1674,8 +1693,7 @@ struct mpcd_gpu_sim {
     void rebin_recorded() {
         hipLaunchKernelGGL(k_scan_tiles, dim3(nTiles), dim3(TPB), 0, stream, bin.cnt, Nc + 1, bin.startB, tileSum,
                            bin.inc, (int*)nullptr, bin.dout);
-        hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(TPB), 0, stream, tileSum, nTiles);
-        hipLaunchKernelGGL(k_scan_add, dim3(nblk(Nc + 1)), dim3(TPB), 0, stream, bin.startB, Nc + 1, tileSum);
+        hipLaunchKernelGGL(k_scan_add_prefix, dim3(nblk(Nc + 1)), dim3(TPB), 0, stream, bin.startB, Nc + 1, tileSum);
         hipLaunchKernelGGL(k_place_arrivals, dim3(nblk((n + PLACE_PER - 1) / PLACE_PER)), dim3(TPB), 0, stream, cid, pid,
                            bin.rnk, n, bin.startB, bin.cnt, bin.dout, bin.keyB);
         /* cells per block: the block's runs fit the LDS staging at 1.25x the mean density */
