@@ -843,14 +843,34 @@ __device__ __forceinline__ void zone_pass_body(const ZoneArgs& a, int zi, long l
 
 /* A workgroup takes zones blockIdx.x, + gridDim.x, ... (the grid is
  * smaller than the zone count when few zones need this form). */
+/* The zones block b takes, b + gridDim.x * k, 64 at a time: each lane reads
+ * one zone's size and a ballot names the zones in this launch's range (the
+ * others go to another launch), so a small grid finds its few zones in one
+ * pass of loads.  Returns false when the block has no zone left. */
+__device__ inline bool next_zone(const ZoneArgs& a, int& k0, unsigned long long& todo, int& zi) {
+    while (!todo) {
+        if ((long long)blockIdx.x + (long long)k0 * gridDim.x >= a.nZones) return false;
+        const int ln = (int)(threadIdx.x & 63);
+        const long long zl = (long long)blockIdx.x + (long long)(k0 + ln) * gridDim.x;
+        const int pl = zl < a.nZones ? a.zsize[zl] : -1;
+        todo = __ballot(zl < a.nZones && pl >= a.pMin && pl <= a.pMax);
+        k0 += 64;
+    }
+    const int j = __builtin_ctzll(todo);
+    todo &= todo - 1ull;
+    zi = (int)((long long)blockIdx.x + (long long)(k0 - 64 + j) * gridDim.x);
+    return true;
+}
+
 template <class IT, bool kLds>
 __global__ void __launch_bounds__(ZONE_TPB) k_zone(ZoneArgs a) {
     extern __shared__ __align__(16) unsigned char lds[];
     __shared__ ZoneShared sh;
     const int lane = threadIdx.x;
-    for (int zi = blockIdx.x; zi < a.nZones; zi += gridDim.x) {
+    int k0 = 0, zi = 0;
+    unsigned long long todo = 0ull;
+    while (next_zone(a, k0, todo, zi)) {
         const int P = a.zsize[zi];
-        if (P < a.pMin || P > a.pMax) continue; /* uniform: another launch takes this zone */
         const long long off = a.zoff[zi];
         if (off + (long long)P > a.scratchCap) {
             if (lane == 0) atomicOr(a.err, 2);
@@ -958,9 +978,10 @@ template <class IT, bool kLds>
 __global__ void __launch_bounds__(FINISH_TPB) k_zone_finish(ZoneArgs a) {
     extern __shared__ __align__(16) unsigned char lds[];
     __shared__ int queue[FINISH_TPB];
-    for (int zi = blockIdx.x; zi < a.nZones; zi += gridDim.x) {
+    int k0 = 0, zi = 0;
+    unsigned long long todo = 0ull; /* the same in every wavefront of the block */
+    while (next_zone(a, k0, todo, zi)) {
         const int P = a.zsize[zi];
-        if (P < a.pMin || P > a.pMax) continue;
         const long long off = a.zoff[zi];
         if (off + (long long)P > a.scratchCap) continue;
         zone_finish_body<IT, kLds, FINISH_TPB, 4>(a, zi, off, (int)threadIdx.x, lds, FINISH_LDS, queue);
