@@ -1457,6 +1457,7 @@ struct mpcd_gpu_sim {
     BCParams bc{};
     double xmax = 0, ymax = 0;
     int fuseNext = -1; /* step whose stream the current kick performs (-1: none) */
+    bool recDirty = false; /* a recording pass was launched and its update has not run */
     int streamed = -1; /* step whose stream has been performed by a kick (-1: none) */
     int cellCap = 0; /* LDS entries of a per-cell block (k_collide, k_cell_fields) */
 
@@ -1723,6 +1724,7 @@ struct mpcd_gpu_sim {
         hipLaunchKernelGGL(k_rebin_merge, dim3(nblk(Nc, cpb)), dim3(REB_CELLS), 0, stream, bin.start, bin.key, cid,
                            Nc, cpb, bin.startB, bin.cnt, bin.inc, bin.dout, bin.rnk, bin.keyB, bin.listB, err);
         check_launch();
+        recDirty = false;
         std::swap(bin.start, bin.startB);
         std::swap(bin.key, bin.keyB);
         std::swap(bin.list, bin.listB);
@@ -1848,7 +1850,14 @@ struct mpcd_gpu_sim {
         if (record) {
             /* moved particles record their cell changes for rebin_recorded()
              * (bin.rnk is -1 for every slot here: set at creation, and each
-             * merge resets the ranks of the arrivals it took) */
+             * merge resets the ranks of the arrivals it took; after an
+             * interrupted pass-and-update the record is cleared again) */
+            if (recDirty) {
+                hipLaunchKernelGGL(k_fill_int, dim3(nblk(n)), dim3(TPB), 0, stream, bin.rnk, n, -1);
+                hipLaunchKernelGGL(k_fill_int, dim3(nblk(Nc + 1)), dim3(TPB), 0, stream, bin.inc, Nc + 1, 0);
+                hipLaunchKernelGGL(k_fill_int, dim3(nblk(Nc + 1)), dim3(TPB), 0, stream, bin.dout, Nc + 1, 0);
+            }
+            recDirty = true;
             a.in.dcid = cid;
             a.in.dout = bin.dout;
             a.in.dinc = bin.inc;
