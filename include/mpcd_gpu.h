@@ -122,6 +122,20 @@ typedef struct mpcd_occupancy_stats {
     int32_t nDense, nSparse, nEmptyActive, _pad;
 } mpcd_occupancy_stats;
 
+/* Diagnostics of one resident time step, reduced on the device: the
+ * occupancy statistics after each zone pass (the driver's baseOccStd,
+ * shiftedOccStd, ... columns and ZonePassMetrics' after-fields), the
+ * particle means of the step's output state (mean_kinetic, mean_component,
+ * main_benchmark_poiseuille_openmp_profiled_shifted_profiled.cpp:105-116)
+ * and the means of its cell fields (the closure's outFields; Qx =
+ * mean(Ux) Ly, :124-126). */
+typedef struct mpcd_step_diag {
+    mpcd_occupancy_stats base;
+    mpcd_occupancy_stats shifted;
+    double meanKinetic, meanVelocityX, meanVelocityY;
+    double meanUx, meanUy, Qx;
+} mpcd_step_diag;
+
 /* Metrics of one resident time step. */
 typedef struct mpcd_step_metrics {
     mpcd_wall_info wall;
@@ -196,7 +210,7 @@ int mpcd_gpu_zone_pass_gauss_seidel(const mpcd_params* p, double* x, double* v, 
 
 /* occupancy statistics from Nc cell counts (compute_occ_std_from_counts,
  * compute_out_band_from_counts, count_dense/sparse_from_counts,
- * count_empty_active: zone_pass.cpp:975-1044); host computation */
+ * count_empty_active: zone_pass.cpp:975-1044), reduced on the GPU */
 int mpcd_gpu_occupancy_stats(const mpcd_params* p, const int32_t* counts, mpcd_occupancy_stats* out);
 
 /* replaces mpcd::run_liquid_closure (liquid_closure.h:46-48) */
@@ -209,25 +223,45 @@ int mpcd_gpu_liquid_closure(const mpcd_params* p, const double* xRef, const doub
  * with the state kept in HBM between steps ---- */
 int mpcd_gpu_sim_create(const mpcd_params* p, mpcd_gpu_sim** out);
 int mpcd_gpu_sim_destroy(mpcd_gpu_sim* s);
+/* x or v may be NULL: that array keeps each particle's current value */
 int mpcd_gpu_sim_set_state(mpcd_gpu_sim* s, const double* x, const double* v);
 int mpcd_gpu_sim_get_state(mpcd_gpu_sim* s, double* x, double* v);
 /* steps first_step .. first_step+n_steps-1 with the driver's per-step seeds;
  * `last` (nullable) receives the last step's metrics */
 int mpcd_gpu_sim_step(mpcd_gpu_sim* s, int first_step, int n_steps, mpcd_step_metrics* last);
-/* same, enqueued without a host wait; mpcd_gpu_sim_check then waits and
- * reports a wall-mode error raised by any enqueued step */
+/* same, enqueued without a host wait; mpcd_gpu_sim_check then waits,
+ * reports an error (wall mode, capacity, binning update) raised by any step
+ * enqueued since the last check, and clears it */
 int mpcd_gpu_sim_step_async(mpcd_gpu_sim* s, int first_step, int n_steps);
 int mpcd_gpu_sim_check(mpcd_gpu_sim* s);
 /* one step that also returns the ref / after-base / red states */
 int mpcd_gpu_sim_step_stages(mpcd_gpu_sim* s, int step, double* x_ref, double* v_ref, double* x_base, double* v_base,
                              double* x_red, double* v_red, mpcd_step_metrics* m);
+/* one step (metrics as mpcd_gpu_sim_step_stages, m nullable) that also
+ * reduces its diagnostics on the device: the metrics row of the benchmark
+ * driver (main_benchmark_poiseuille_openmp_profiled_shifted_profiled.cpp:285-337) */
+int mpcd_gpu_sim_step_diag(mpcd_gpu_sim* s, int step, mpcd_step_metrics* m, mpcd_step_diag* d);
+/* compute_cell_fields (common_grid.h:10-13) of the resident state: every
+ * field of `out` that is not NULL, rhoTarget filled with the given value */
+int mpcd_gpu_sim_cell_fields(mpcd_gpu_sim* s, double rhoTarget, mpcd_cell_fields* out);
 int mpcd_gpu_sim_synchronize(mpcd_gpu_sim* s);
 int mpcd_gpu_device_synchronize(void);
-/* enable (1) / disable (0) per-class kernel timing; resets the totals */
+/* timing with events on the simulation's stream: enable is a mask of
+ * MPCD_PROFILE_KERNELS (per-class kernel times, mpcd_gpu_sim_profile_read)
+ * and MPCD_PROFILE_PHASES (per-phase step times, mpcd_gpu_sim_phase_read);
+ * 0 disables; every call resets the totals */
+enum { MPCD_PROFILE_KERNELS = 1, MPCD_PROFILE_PHASES = 2 };
 int mpcd_gpu_sim_profile(mpcd_gpu_sim* s, int enable);
 /* total milliseconds and launch-group counts per MPCD_PROF_* class since the
  * last reset (arrays of MPCD_PROF_CLASSES entries) */
 int mpcd_gpu_sim_profile_read(mpcd_gpu_sim* s, double* total_ms, int32_t* launches);
+/* milliseconds, summed over the steps profiled since the last reset, of the
+ * step's phases as the benchmark driver times them
+ * (main_benchmark_poiseuille_openmp_profiled_shifted_profiled.cpp:240-279):
+ * ms4 = reference build (force, stream, walls, collision, reference
+ * fields), base pass, shifted pass, closure (with the next step's stream
+ * when the kick performs it); each pass includes its binning update */
+int mpcd_gpu_sim_phase_read(mpcd_gpu_sim* s, double* ms4);
 /* the hipStream_t the simulation launches on */
 int mpcd_gpu_sim_stream(mpcd_gpu_sim* s, void** stream);
 

@@ -1136,6 +1136,39 @@ static double mean_of(const std::vector<double>& a) {
     return std::accumulate(a.begin(), a.end(), 0.0) / (double)a.size();
 }
 
+/* ============ occupancy statistics of a pass: zone_pass.cpp:975-1044 ============
+ * (every cell is active, zone_pass.cpp:128-130; the mean and the variance
+ * in two passes in cell order, as compute_occ_std_from_counts sums them) */
+mpcd_occupancy_stats occupancy_stats(const P& p, const std::vector<int>& cnt) {
+    mpcd_occupancy_stats o{};
+    const size_t Nc = (size_t)p.Nc;
+    double mean = 0.0;
+    for (size_t c = 0; c < Nc; ++c) mean += (double)cnt[c];
+    mean /= (double)Nc;
+    double acc = 0.0;
+    for (size_t c = 0; c < Nc; ++c) {
+        const double d = (double)cnt[c] - mean;
+        acc += d * d;
+    }
+    o.occStd = std::sqrt(acc / (double)Nc) / std::max(p.gamma, 1e-30);
+    const auto Nsm = smooth_counts(p, cnt);
+    std::vector<double> lowLoc, highLoc;
+    threshold_maps(p, Nsm, lowLoc, highLoc);
+    double highThr, lowThrBulk;
+    occupancy_thresholds(p, highThr, lowThrBulk);
+    int nOut = 0;
+    for (size_t c = 0; c < Nc; ++c) {
+        const bool bulk = cnt[c] > 0;
+        const double lo = bulk ? lowThrBulk : lowLoc[c], hi = bulk ? highThr : highLoc[c];
+        if ((double)cnt[c] > hi || (cnt[c] > 0 && (double)cnt[c] < lo)) ++nOut;
+        if ((double)cnt[c] > hi) ++o.nDense;
+        if (cnt[c] > 0 && (double)cnt[c] < lo) ++o.nSparse;
+        if (cnt[c] == 0) ++o.nEmptyActive;
+    }
+    o.outBand = (double)nOut / (double)Nc;
+    return o;
+}
+
 ClosureOut liquid_closure(const P& p, const std::vector<double>& xRef, const std::vector<double>& vRef,
                           const std::vector<double>& xRed, const std::vector<double>& vRed,
                           std::vector<double>& xOut, std::vector<double>& vOut) {

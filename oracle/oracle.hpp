@@ -208,5 +208,6 @@ ClosureOut liquid_closure(const P& p, const std::vector<double>& xRef, const std
                           std::vector<double>& xOut, std::vector<double>& vOut);
 std::vector<int32_t> interzone_mask(const P& p);
 double rho_target(const P& p);
+mpcd_occupancy_stats occupancy_stats(const P& p, const std::vector<int>& cnt);
 
 }  // namespace orc

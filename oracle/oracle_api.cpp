@@ -127,6 +127,12 @@ extern "C" int oracle_cell_fields(const mpcd_params* p, const double* x, const d
     OR_CATCH
 }
 
+extern "C" int oracle_occupancy_stats(const mpcd_params* p, const int32_t* counts, mpcd_occupancy_stats* out) {
+    OR_TRY
+    *out = occupancy_stats(*p, std::vector<int>(counts, counts + p->Nc));
+    OR_CATCH
+}
+
 extern "C" int oracle_layout(const mpcd_params* p, int mode, int32_t* owner, int32_t* nzones) {
     OR_TRY
     const Layout L = build_layout(*p, mode);

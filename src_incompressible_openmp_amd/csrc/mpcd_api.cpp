@@ -15,6 +15,7 @@
 
 #include "boundary_conditions.h"
 #include "common_grid.h"
+#include "gpu_params.h"
 #include "liquid_closure.h"
 #include "mpcd_gpu.h"
 #include "srd_collision.h"
@@ -39,7 +40,9 @@ int bc_mode(const std::string& s) {
     return MPCD_BC_INVALID; /* an error only if a particle reaches that wall */
 }
 
-mpcd_params to_c(const Params& q) {
+}  // namespace
+
+mpcd_params to_gpu_params(const Params& q) {
     mpcd_params p{};
     p.Nx = q.Nx;
     p.Ny = q.Ny;
@@ -96,6 +99,10 @@ mpcd_params to_c(const Params& q) {
     p.smoothPdriveIncludeShiftedLayouts = q.smoothPdriveIncludeShiftedLayouts;
     return p;
 }
+
+namespace {
+
+mpcd_params to_c(const Params& q) { return to_gpu_params(q); }
 
 /* "single_full" (or the params flag), "shifted", anything else the base
  * layout (zone_pass.cpp:177-196) */

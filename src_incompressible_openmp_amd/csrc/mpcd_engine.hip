@@ -17,6 +17,7 @@
 #include <mutex>
 #include <stdexcept>
 #include <string>
+#include <array>
 #include <vector>
 
 #include "mpcd_core.h"
@@ -59,6 +60,17 @@ __global__ void k_interleave(const double* __restrict__ x, const double* __restr
         const size_t p = (size_t)pid[s];
         a[2 * p] = x[s];
         a[2 * p + 1] = y[s];
+    }
+}
+
+/* scatter by pid: ox[pid[s]] = x[s] (the slot order back to particle-id order) */
+__global__ void k_scatter_pid(const double* __restrict__ x, const double* __restrict__ y, const int* __restrict__ pid,
+                              int n, double* __restrict__ ox, double* __restrict__ oy) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s < n) {
+        const int q = pid[s];
+        ox[q] = x[s];
+        oy[q] = y[s];
     }
 }
 
@@ -719,6 +731,114 @@ __global__ void k_sub_mean(double* __restrict__ vx, double* __restrict__ vy, int
         const double mx = sums[0] / (double)n, my = sums[1] / (double)n;
         vx[i] -= mx;
         vy[i] -= my;
+    }
+}
+
+/* ---- diagnostics: occupancy statistics and state means ---- */
+
+/* Per-block partials of the occupancy statistics of a count grid
+ * (compute_occ_std_from_counts, compute_out_band_from_counts,
+ * count_dense_from_counts, count_sparse_from_counts, count_empty_active:
+ * zone_pass.cpp:975-1044; every cell is active, zone_pass.cpp:128-130).
+ * part[b][6] = sum (N - gamma), sum (N - gamma)^2, out-of-band, dense,
+ * sparse, empty.  An occupied cell takes the bulk thresholds, an empty one
+ * the fluid-fraction map of its 3x3 binomial smoothing (no wrap): an empty
+ * cell is never out of band, dense or sparse, so the map only has to exist
+ * for the comparison to read like the reference's. */
+constexpr int OCC_W = 6;
+/* slots of a simulation's diagnostics buffer */
+constexpr int DIAG_OCC_BASE = 0, DIAG_OCC_SHIFTED = 5, DIAG_PSUM = 10, DIAG_USUM = 13, DIAG_SLOTS = 16;
+__global__ void __launch_bounds__(TPB) k_occupancy(const int* __restrict__ cnt, int Nx, int Ny, ZoneParams zp,
+                                                   double* __restrict__ part) {
+    __shared__ double sh[OCC_W][TPB];
+    const int Nc = Nx * Ny;
+    double a[OCC_W] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    for (int c = blockIdx.x * TPB + (int)threadIdx.x; c < Nc; c += gridDim.x * TPB) {
+        const int m = cnt[c];
+        const double d = (double)m - zp.gamma;
+        a[0] += d;
+        a[1] += d * d;
+        double lo = zp.lowThrBulk, hi = zp.highThr;
+        if (m == 0) {
+            lo = zp.lowLoc1;
+            hi = zp.highLoc1;
+            if (zp.useLocal) {
+                const int ix = c / Ny, iy = c - ix * Ny;
+                long long num = 0;
+                int den = 0;
+                for (int dx = -1; dx <= 1; ++dx)
+                    for (int dy = -1; dy <= 1; ++dy) {
+                        const int jx = ix + dx, jy = iy + dy;
+                        if (jx < 0 || jx >= Nx || jy < 0 || jy >= Ny) continue;
+                        const int w = (dx == 0 ? 2 : 1) * (dy == 0 ? 2 : 1);
+                        num += (long long)w * cnt[jy + Ny * jx];
+                        den += w;
+                    }
+                const double t = zp.fluidFracGain * (((double)num / (double)den) / dmax(zp.gamma, 1e-12));
+                const double gl = zp.gamma * ((t < 0.0) ? 0.0 : ((1.0 < t) ? 1.0 : t));
+                lo = dmax(zp.lowFrac * gl, zp.lowThrFloor);
+                hi = dmax(zp.highFrac * gl, zp.highThrFloor);
+            }
+        }
+        const bool dense = (double)m > hi, sparse = m > 0 && (double)m < lo;
+        a[2] += (dense || sparse) ? 1.0 : 0.0;
+        a[3] += dense ? 1.0 : 0.0;
+        a[4] += sparse ? 1.0 : 0.0;
+        a[5] += m == 0 ? 1.0 : 0.0;
+    }
+    for (int k = 0; k < OCC_W; ++k) sh[k][threadIdx.x] = a[k];
+    __syncthreads();
+    for (int st = TPB / 2; st > 0; st >>= 1) {
+        if (threadIdx.x < (unsigned)st)
+            for (int k = 0; k < OCC_W; ++k) sh[k][threadIdx.x] += sh[k][threadIdx.x + st];
+        __syncthreads();
+    }
+    if (threadIdx.x < OCC_W) part[(size_t)blockIdx.x * OCC_W + threadIdx.x] = sh[threadIdx.x][0];
+}
+
+/* out[0..5) = occStd, outBand, nDense, nSparse, nEmptyActive from the
+ * partials of k_occupancy.  With an integer gamma every partial is an exact
+ * integer, so the sums are exact and the variance is
+ * sum d^2 / Nc - (sum d / Nc)^2 with d = N - gamma. */
+__global__ void __launch_bounds__(TPB) k_occupancy_final(const double* __restrict__ part, int np, int Nc, double gamma,
+                                                         double* __restrict__ out) {
+    __shared__ double tot[OCC_W];
+    reduce_parts_block(part, np, OCC_W, tot);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const double m = tot[0] / (double)Nc;
+        double var = tot[1] / (double)Nc - m * m;
+        if (var < 0.0) var = 0.0;
+        out[0] = sqrt(var) / dmax(gamma, 1e-30);
+        out[1] = tot[2] / (double)Nc;
+        out[2] = tot[3];
+        out[3] = tot[4];
+        out[4] = tot[5];
+    }
+}
+
+/* deterministic block partials of (vx, vy, 0.5 |v|^2) */
+__global__ void __launch_bounds__(TPB) k_sum_ke(const double* __restrict__ vx, const double* __restrict__ vy, int n,
+                                                double* __restrict__ part) {
+    __shared__ double s0[TPB], s1[TPB], s2[TPB];
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const double a = (i < n) ? vx[i] : 0.0, b = (i < n) ? vy[i] : 0.0;
+    s0[threadIdx.x] = a;
+    s1[threadIdx.x] = b;
+    s2[threadIdx.x] = 0.5 * (a * a + b * b);
+    __syncthreads();
+    for (int st = TPB / 2; st > 0; st >>= 1) {
+        if (threadIdx.x < (unsigned)st) {
+            s0[threadIdx.x] += s0[threadIdx.x + st];
+            s1[threadIdx.x] += s1[threadIdx.x + st];
+            s2[threadIdx.x] += s2[threadIdx.x + st];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        part[3 * (size_t)blockIdx.x] = s0[0];
+        part[3 * (size_t)blockIdx.x + 1] = s1[0];
+        part[3 * (size_t)blockIdx.x + 2] = s2[0];
     }
 }
 
@@ -1435,6 +1555,12 @@ struct mpcd_gpu_sim {
     int* surFlag = nullptr;
     int nLearned = 0, nPlain = 0, nFallback = 0;
     int* err = nullptr;
+    /* diagnostics: device results of the last diagnosed step, and the cell
+     * fields of the current state (allocated on first use) */
+    bool wantDiag = false;
+    double* diagDev = nullptr; /* DIAG_* slots */
+    int* outN = nullptr;
+    double* outF = nullptr; /* 9 x Nc: Ux, Uy, Px, Py, T, rho, Pkin, Pvir, P */
     WallAcc* wallPart = nullptr;
     ZoneRect* rects[2] = {nullptr, nullptr};
     int* owner[2] = {nullptr, nullptr};
@@ -1486,9 +1612,40 @@ struct mpcd_gpu_sim {
         HIP_OK(hipEventRecord(b, stream));
         ev[c].back().second = b;
     }
+    /* phase boundaries of each profiled step: start, after the reference
+     * build (stream, collision, reference fields), after the base pass and
+     * its binning update, after the shifted pass and its update, after the
+     * closure (the closure phase holds the next step's stream when the kick
+     * performs it) */
+    std::vector<std::array<hipEvent_t, 5>> phaseEv;
+    bool phases = false;
+    double phaseAcc[4] = {0, 0, 0, 0};
+    void phase_mark(int k) {
+        if (!phases) return;
+        if (k == 0) phaseEv.push_back({});
+        hipEvent_t e = take_event();
+        HIP_OK(hipEventRecord(e, stream));
+        phaseEv.back()[(size_t)k] = e;
+    }
     void prof_reset() {
         for (auto& v : ev) v.clear();
+        phaseEv.clear();
+        for (double& a : phaseAcc) a = 0.0;
         evUsed = 0;
+    }
+    /* totals since the last reset; the events read are returned to the pool
+     * when no kernel-class timing holds events */
+    void phase_read(double* ms4) {
+        HIP_OK(hipStreamSynchronize(stream));
+        for (const auto& a : phaseEv)
+            for (int k = 0; k < 4; ++k) {
+                float f = 0.f;
+                HIP_OK(hipEventElapsedTime(&f, a[(size_t)k], a[(size_t)k + 1]));
+                phaseAcc[k] += (double)f;
+            }
+        phaseEv.clear();
+        if (!prof) evUsed = 0;
+        for (int k = 0; k < 4; ++k) ms4[k] = phaseAcc[k];
     }
     void prof_read(double* ms, int* count) {
         HIP_OK(hipStreamSynchronize(stream));
@@ -1547,8 +1704,9 @@ struct mpcd_gpu_sim {
         tgtBuf = dalloc<float>(2 * (size_t)Nc);
         surFlag = dalloc<int>(1);
         scal = dalloc<double>(64);
+        diagDev = dalloc<double>(DIAG_SLOTS);
         err = dalloc<int>(1);
-        ensure_parts((size_t)std::max(nblk(n) * 6, nblk(Nc) * 3) + 64);
+        ensure_parts(std::max(std::max((size_t)nblk(n) * 6, (size_t)nblk(Nc) * 3), std::min<size_t>(nblk(Nc), 512) * OCC_W) + 64);
         wallPart = dalloc<WallAcc>(nblk(n));
         hipLaunchKernelGGL(k_iota, dim3(nblk(n)), dim3(TPB), 0, stream, pid, n);
         hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, stream, err, 1, 0); /* no error flagged yet */
@@ -1623,7 +1781,7 @@ struct mpcd_gpu_sim {
                         tmpg, zsize, zoff, owner[0], owner[1], cover[0], cover[1], overCells[0], overCells[1], stage,
                         refPx, refPy, refUx, refUy, pdriveRaw, redPx, redPy, redUx, redUy, duRX, duRY, duEX, duEY,
                         pdrive, cellTmp, mask, part, scal, err, wallPart, rects[0], rects[1], zout, zscratch, duG, recG, zstate,
-                        redN, redT, featBuf, tgtBuf, surFlag};
+                        redN, redT, featBuf, tgtBuf, surFlag, diagDev, outN, outF};
         for (void* q : ptrs)
             if (q) (void)hipFree(q);
         for (hipEvent_t e : evPool) (void)hipEventDestroy(e);
@@ -1669,11 +1827,31 @@ struct mpcd_gpu_sim {
         HIP_OK(hipMemcpyAsync(h, stage, 2 * (size_t)n * sizeof(double), hipMemcpyDeviceToHost, stream));
         HIP_OK(hipStreamSynchronize(stream));
     }
+    /* Slots are reset to particle-id order (pid = iota).  An array the
+     * caller does not upload (null) is kept: it is first scattered from the
+     * current slot order into particle-id order, so every particle keeps its
+     * own position or velocity. */
     void set_state(const double* hx, const double* hv) {
+        if (!hx || !hv) {
+            double* kx = hx ? vx : x;
+            double* ky = hx ? vy : y;
+            double* tx = hx ? vx2 : x2;
+            double* ty = hx ? vy2 : y2;
+            hipLaunchKernelGGL(k_scatter_pid, dim3(nblk(n)), dim3(TPB), 0, stream, kx, ky, pid, n, tx, ty);
+            check_launch();
+            if (hx) {
+                std::swap(vx, vx2);
+                std::swap(vy, vy2);
+            } else {
+                std::swap(x, x2);
+                std::swap(y, y2);
+            }
+        }
         hipLaunchKernelGGL(k_iota, dim3(nblk(n)), dim3(TPB), 0, stream, pid, n);
         check_launch();
         if (hx) upload_xy(hx, x, y);
         if (hv) upload_xy(hv, vx, vy);
+        streamed = -1;
         HIP_OK(hipStreamSynchronize(stream));
     }
     void get_state(double* hx, double* hv) {
@@ -1734,10 +1912,13 @@ struct mpcd_gpu_sim {
         hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, stream, err, 1, 0);
         check_launch();
     }
+    /* reads the error flag and clears it (in stream order), so a reported
+     * error is reported once */
     void check_err() {
         int h = 0;
         HIP_OK(hipMemcpyAsync(&h, err, sizeof(int), hipMemcpyDeviceToHost, stream));
         HIP_OK(hipStreamSynchronize(stream));
+        if (h) clear_err();
         if (h & 1) throw ApiError(MPCD_E_BC_MODE, "Unsupported boundary mode at a wall a particle crossed");
         if (h & 2) throw ApiError(MPCD_E_CAPACITY, "zone scratch capacity exceeded");
         if (h & 4) throw ApiError(MPCD_E_INVALID, "binning update: a cell's stayers do not match its count");
@@ -2122,6 +2303,70 @@ struct mpcd_gpu_sim {
         }
     }
 
+    /* ---- diagnostics (device reductions; the host reads a few scalars) ---- */
+
+    /* occupancy statistics of the count grid `cnt` into diagDev[slot..slot+5) */
+    void occupancy(const int* cnt, int slot) {
+        const int nb = (int)std::min<size_t>(nblk(Nc), 512);
+        ensure_parts((size_t)nb * OCC_W);
+        hipLaunchKernelGGL(k_occupancy, dim3(nb), dim3(TPB), 0, stream, cnt, p.Nx, p.Ny, zp, part);
+        hipLaunchKernelGGL(k_occupancy_final, dim3(1), dim3(TPB), 0, stream, part, nb, Nc, p.gamma, diagDev + slot);
+        check_launch();
+    }
+
+    /* all cell fields of the current state (common_grid.cpp:32-100) into
+     * outN / outF, from a fresh unshifted binning of the current positions */
+    void fields_current() {
+        if (!outN) {
+            outN = dalloc<int>((size_t)Nc);
+            outF = dalloc<double>(9 * (size_t)Nc);
+        }
+        bin_unshifted();
+        const size_t C = (size_t)Nc;
+        FieldsOut f{outN, outF, outF + C, outF + 2 * C, outF + 3 * C, outF + 4 * C, outF + 5 * C, outF + 6 * C,
+                    outF + 7 * C, outF + 8 * C};
+        hipLaunchKernelGGL(k_cell_fields, dim3(nblk(Nc, CELL_BLK)), dim3(CELL_BLK), cellCap * 2 * sizeof(double), stream,
+                           bin.start, bin.list, Nc, vx, vy, vc(), p.Kvirial, rho_target(), f, cellCap);
+        check_launch();
+    }
+
+    /* sums over the particles of the current state (vx, vy, kinetic energy)
+     * and over the cells of its fields (Ux, Uy) into diagDev */
+    void state_diag() {
+        const int nb = (int)nblk(n);
+        hipLaunchKernelGGL(k_sum_ke, dim3(nb), dim3(TPB), 0, stream, vx, vy, n, part);
+        hipLaunchKernelGGL(k_reduce_parts, dim3(1), dim3(TPB), 0, stream, part, nb, 3, diagDev + DIAG_PSUM);
+        fields_current();
+        const int nbc = (int)nblk(Nc);
+        hipLaunchKernelGGL(k_sum2, dim3(nbc), dim3(TPB), 0, stream, outF, outF + Nc, Nc, part);
+        hipLaunchKernelGGL(k_reduce_parts, dim3(1), dim3(TPB), 0, stream, part, nbc, 2, diagDev + DIAG_USUM);
+        check_launch();
+    }
+
+    void read_diag(mpcd_step_diag* d) {
+        double h[DIAG_SLOTS];
+        HIP_OK(hipMemcpyAsync(h, diagDev, sizeof(h), hipMemcpyDeviceToHost, stream));
+        HIP_OK(hipStreamSynchronize(stream));
+        mpcd_occupancy_stats* o[2] = {&d->base, &d->shifted};
+        const int slot[2] = {DIAG_OCC_BASE, DIAG_OCC_SHIFTED};
+        for (int k = 0; k < 2; ++k) {
+            const double* q = h + slot[k];
+            o[k]->occStd = q[0];
+            o[k]->outBand = q[1];
+            o[k]->nDense = (int32_t)q[2];
+            o[k]->nSparse = (int32_t)q[3];
+            o[k]->nEmptyActive = (int32_t)q[4];
+            o[k]->_pad = 0;
+        }
+        const double inv = 1.0 / (double)std::max(n, 1);
+        d->meanVelocityX = h[DIAG_PSUM] * inv;
+        d->meanVelocityY = h[DIAG_PSUM + 1] * inv;
+        d->meanKinetic = h[DIAG_PSUM + 2] * inv;
+        d->meanUx = h[DIAG_USUM] / (double)Nc;
+        d->meanUy = h[DIAG_USUM + 1] / (double)Nc;
+        d->Qx = d->meanUx * p.Ly;
+    }
+
     /* one full time step with the benchmark driver's seeds
      * (main_benchmark_poiseuille_openmp_profiled_shifted_profiled.cpp:240-279) */
     /* next >= 0: the step that follows in the same call (its stream is
@@ -2130,6 +2375,7 @@ struct mpcd_gpu_sim {
         check_pass_modes();
         const uint64_t us = (uint64_t)s;
         const bool pre = streamed == s && !sm;
+        phase_mark(0);
         if (pre) {
             streamed = -1; /* the previous step's kick streamed this one */
         } else {
@@ -2154,6 +2400,7 @@ struct mpcd_gpu_sim {
         pbeg(MPCD_PROF_CELL_FIELDS);
         ref_fields();
         pend(MPCD_PROF_CELL_FIELDS);
+        phase_mark(1);
         const uint64_t zs = 88172645463325252ULL + 10007ULL * us;
         pbeg(MPCD_PROF_ZONE_PASS);
         /* the base pass's input is the state the reference fields were
@@ -2166,6 +2413,8 @@ struct mpcd_gpu_sim {
         pbeg(MPCD_PROF_BIN);
         rebin_recorded();
         pend(MPCD_PROF_BIN);
+        phase_mark(2);
+        if (wantDiag) occupancy(bin.cnt, DIAG_OCC_BASE); /* the counts after the base pass */
         if (stages && stages[2]) get_state(stages[2], stages[3]);
         pbeg(MPCD_PROF_ZONE_PASS);
         layout_pass(1, zs + 4099ULL, nullptr, nullptr, true);
@@ -2174,6 +2423,8 @@ struct mpcd_gpu_sim {
         pbeg(MPCD_PROF_BIN);
         rebin_recorded();
         pend(MPCD_PROF_BIN);
+        phase_mark(3);
+        if (wantDiag) occupancy(bin.cnt, DIAG_OCC_SHIFTED);
         if (stages && stages[4]) get_state(stages[4], stages[5]);
         pbeg(MPCD_PROF_CLOSURE);
         fuseNext = (stages || sm) ? -1 : next;
@@ -2185,6 +2436,8 @@ struct mpcd_gpu_sim {
         }
         fuseNext = -1;
         pend(MPCD_PROF_CLOSURE);
+        phase_mark(4);
+        if (wantDiag) state_diag();
     }
 };
 
@@ -2277,6 +2530,71 @@ int mpcd_gpu_sim_step_stages(mpcd_gpu_sim* s, int step, double* x_ref, double* v
     });
 }
 
+int mpcd_gpu_sim_step_diag(mpcd_gpu_sim* s, int step, mpcd_step_metrics* m, mpcd_step_diag* d) {
+    return guarded([&] {
+        if (!d) throw ApiError(MPCD_E_INVALID, "null diagnostics");
+        mpcd_step_metrics local{};
+        s->clear_err();
+        s->streamed = -1;
+        s->wantDiag = true;
+        try {
+            s->step(step, m ? m : &local, nullptr);
+        } catch (...) {
+            s->wantDiag = false;
+            throw;
+        }
+        s->wantDiag = false;
+        s->read_diag(d);
+        s->check_err();
+    });
+}
+
+int mpcd_gpu_sim_cell_fields(mpcd_gpu_sim* s, double rhoTarget, mpcd_cell_fields* out) {
+    return guarded([&] {
+        if (!out) throw ApiError(MPCD_E_INVALID, "null cell fields");
+        if (s->streamed >= 0) throw ApiError(MPCD_E_INVALID, "internal: a fused stream was left unused");
+        s->fields_current();
+        const size_t C = (size_t)s->Nc;
+        if (out->N) HIP_OK(hipMemcpyAsync(out->N, s->outN, C * sizeof(int), hipMemcpyDeviceToHost, s->stream));
+        double* hs[9] = {out->Ux, out->Uy, out->Px, out->Py, out->T, out->rho, out->Pkin, out->Pvir, out->P};
+        for (int k = 0; k < 9; ++k)
+            if (hs[k])
+                HIP_OK(hipMemcpyAsync(hs[k], s->outF + (size_t)k * C, C * sizeof(double), hipMemcpyDeviceToHost,
+                                      s->stream));
+        HIP_OK(hipStreamSynchronize(s->stream));
+        if (out->rhoTarget)
+            for (size_t c = 0; c < C; ++c) out->rhoTarget[c] = rhoTarget;
+    });
+}
+
+int mpcd_gpu_occupancy_stats(const mpcd_params* p, const int32_t* counts, mpcd_occupancy_stats* out) {
+    return guarded([&] {
+        if (!p || !counts || !out) throw ApiError(MPCD_E_INVALID, "null argument");
+        if (p->Nx <= 0 || p->Ny <= 0 || p->Nc != p->Nx * p->Ny) throw ApiError(MPCD_E_INVALID, "bad grid size");
+        const ZoneParams zp = make_zone_params(*p);
+        const int Nc = p->Nc;
+        const int nb = (int)std::min<size_t>(nblk(Nc), 512);
+        int* dc = dalloc<int>((size_t)Nc);
+        double* dpart = dalloc<double>((size_t)nb * OCC_W + 8);
+        hipStream_t st = nullptr;
+        HIP_OK(hipMemcpy(dc, counts, (size_t)Nc * sizeof(int), hipMemcpyHostToDevice));
+        hipLaunchKernelGGL(k_occupancy, dim3(nb), dim3(TPB), 0, st, dc, p->Nx, p->Ny, zp, dpart);
+        hipLaunchKernelGGL(k_occupancy_final, dim3(1), dim3(TPB), 0, st, dpart, nb, Nc, p->gamma,
+                           dpart + (size_t)nb * OCC_W);
+        HIP_OK(hipGetLastError());
+        double h[5];
+        HIP_OK(hipMemcpy(h, dpart + (size_t)nb * OCC_W, sizeof(h), hipMemcpyDeviceToHost));
+        HIP_OK(hipFree(dc));
+        HIP_OK(hipFree(dpart));
+        out->occStd = h[0];
+        out->outBand = h[1];
+        out->nDense = (int32_t)h[2];
+        out->nSparse = (int32_t)h[3];
+        out->nEmptyActive = (int32_t)h[4];
+        out->_pad = 0;
+    });
+}
+
 int mpcd_gpu_device_synchronize(void) {
     return guarded([&] { HIP_OK(hipDeviceSynchronize()); });
 }
@@ -2285,12 +2603,20 @@ int mpcd_gpu_sim_profile(mpcd_gpu_sim* s, int enable) {
     return guarded([&] {
         HIP_OK(hipStreamSynchronize(s->stream));
         s->prof_reset();
-        s->prof = enable != 0;
+        s->prof = (enable & MPCD_PROFILE_KERNELS) != 0;
+        s->phases = (enable & MPCD_PROFILE_PHASES) != 0;
     });
 }
 
 int mpcd_gpu_sim_profile_read(mpcd_gpu_sim* s, double* total_ms, int32_t* launches) {
     return guarded([&] { s->prof_read(total_ms, launches); });
+}
+
+int mpcd_gpu_sim_phase_read(mpcd_gpu_sim* s, double* ms4) {
+    return guarded([&] {
+        if (!ms4) throw ApiError(MPCD_E_INVALID, "null argument");
+        s->phase_read(ms4);
+    });
 }
 
 int mpcd_gpu_sim_synchronize(mpcd_gpu_sim* s) {

@@ -202,9 +202,14 @@ class OccupancyStats(ctypes.Structure):
                 ("nSparse", ctypes.c_int32), ("nEmptyActive", ctypes.c_int32), ("_pad", ctypes.c_int32)]
 
 
+class StepDiag(ctypes.Structure):
+    _fields_ = [("base", OccupancyStats), ("shifted", OccupancyStats)] + [
+        (k, ctypes.c_double) for k in ("meanKinetic", "meanVelocityX", "meanVelocityY", "meanUx", "meanUy", "Qx")]
+
+
 def occupancy_stats(p: MpcdParams, counts: np.ndarray) -> Dict[str, float]:
     """occStd, outBand, nDense, nSparse, nEmptyActive of a zone pass's before
-    or after counts (host computation in the library)."""
+    or after counts (reduced on the GPU)."""
     c = np.ascontiguousarray(counts, np.int32)
     if c.size != p.Nc:
         raise ValueError("counts must have Nc entries")
@@ -250,12 +255,15 @@ class Sim:
         except Exception:
             pass
 
-    def set_state(self, x: np.ndarray, v: np.ndarray) -> None:
-        x = np.ascontiguousarray(x, np.float64)
-        v = np.ascontiguousarray(v, np.float64)
-        if x.size != 2 * self.p.n or v.size != 2 * self.p.n:
-            raise ValueError("state size does not match params.n")
-        _check(load().mpcd_gpu_sim_set_state(self._h, _ptr(x), _ptr(v)))
+    def set_state(self, x, v) -> None:
+        """Uploads positions and velocities (interleaved, particle-id order);
+        either may be None to keep each particle's current value."""
+        x = None if x is None else np.ascontiguousarray(x, np.float64)
+        v = None if v is None else np.ascontiguousarray(v, np.float64)
+        for a in (x, v):
+            if a is not None and a.size != 2 * self.p.n:
+                raise ValueError("state size does not match params.n")
+        _check(load().mpcd_gpu_sim_set_state(self._h, None if x is None else _ptr(x), None if v is None else _ptr(v)))
 
     def get_state(self):
         x, v = np.zeros(2 * self.p.n), np.zeros(2 * self.p.n)
@@ -314,6 +322,24 @@ class Sim:
         s = ctypes.c_void_p()
         _check(load().mpcd_gpu_sim_stream(self._h, ctypes.byref(s)))
         return s.value or 0
+
+    def step_diag(self, step: int):
+        """One step with its metrics and its device-reduced diagnostics
+        (occupancy after each pass, means of the output state and of its
+        cell fields): returns (metrics, diag) dictionaries."""
+        m, d = StepMetrics(), StepDiag()
+        _check(load().mpcd_gpu_sim_step_diag(self._h, step, ctypes.byref(m), ctypes.byref(d)))
+        return struct_to_dict(m), struct_to_dict(d)
+
+    def cell_fields(self, rho_target: float):
+        """compute_cell_fields of the resident state: dict of Nc arrays."""
+        Nc = self.p.Nc
+        arrs = {"N": np.zeros(Nc, np.int32)}
+        for k in FIELD_NAMES:
+            arrs[k] = np.zeros(Nc)
+        f = CellFieldsC(_ptr(arrs["N"], ctypes.c_int32), *[_ptr(arrs[k]) for k in FIELD_NAMES])
+        _check(load().mpcd_gpu_sim_cell_fields(self._h, ctypes.c_double(rho_target), ctypes.byref(f)))
+        return arrs
 
     def step_stages(self, step: int):
         n2 = 2 * self.p.n

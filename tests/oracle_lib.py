@@ -147,6 +147,14 @@ class Oracle:
                    "wall": struct_to_dict(wi)})
         return st
 
+    def occupancy_stats(self, p: MpcdParams, counts: np.ndarray):
+        from src_incompressible_openmp_amd.gpu import OccupancyStats
+
+        c = np.ascontiguousarray(counts, np.int32)
+        out = OccupancyStats()
+        self._check(self.lib.oracle_occupancy_stats(ctypes.byref(p), _ptr(c, ctypes.c_int32), ctypes.byref(out)))
+        return struct_to_dict(out)
+
     def interzone_mask(self, p: MpcdParams) -> np.ndarray:
         out = np.zeros(p.Nc, np.int32)
         self._check(self.lib.oracle_interzone_mask(ctypes.byref(p), _ptr(out, ctypes.c_int32)))

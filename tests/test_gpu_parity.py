@@ -486,3 +486,56 @@ def test_medium_grid_steps_in_one_call(gpu):
     xa, va = a.get_state()
     xb, vb = b.get_state()
     assert bits(xa, xb) and bits(va, vb)
+
+
+def test_async_error_reported_once(gpu):
+    """A wall-mode error raised inside an asynchronous batch is reported by
+    the next check and cleared by it: a later check, and a later clean
+    batch, report nothing."""
+    prm = case_params("small")
+    prm["boundary_bottom"] = "sticky"  # particles reach the bottom wall within a step
+    p = to_struct(prm)
+    x, v, _, _ = case_state("small")
+    sim = gpu.Sim(p)
+    try:
+        sim.set_state(x, v)
+        sim.step_async(1, 1)
+        with pytest.raises(gpu.MpcdError) as e:
+            sim.check()
+        assert e.value.code == -3
+        sim.check()  # the flag was cleared by the first check
+    finally:
+        sim.close()
+    prm["boundary_bottom"] = "thermalize"
+    sim = gpu.Sim(to_struct(prm))
+    try:
+        sim.set_state(x, v)
+        sim.step_async(1, 2)
+        sim.check()
+    finally:
+        sim.close()
+
+
+def test_set_state_keeps_the_array_not_uploaded(gpu):
+    """After steps have put the slots in cell order, uploading only v (or
+    only x) keeps every particle's own position (or velocity)."""
+    p = to_struct(case_params("small"))
+    x, v, _, _ = case_state("small")
+    sim = gpu.Sim(p)
+    try:
+        sim.set_state(x, v)
+        sim.step(1, 2)
+        x1, v1 = sim.get_state()
+        vnew = v1[::-1].copy()
+        sim.set_state(None, vnew)
+        x2, v2 = sim.get_state()
+        assert bits(x2, x1) and bits(v2, vnew)
+        sim.step(3, 1)
+        x3, v3 = sim.get_state()
+        xnew = x3.copy()
+        xnew[0::2] = x3[0::2][::-1]  # particles swap positions, velocities stay theirs
+        sim.set_state(xnew, None)
+        x4, v4 = sim.get_state()
+        assert bits(x4, xnew) and bits(v4, v3)
+    finally:
+        sim.close()

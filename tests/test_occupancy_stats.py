@@ -1,6 +1,7 @@
-"""The occupancy statistics of a zone pass (host computation in the library,
-mpcd_occupancy_stats) against the values the compiled original wrote in its
-Gauss-Seidel pass runout (tests/golden, 6 significant digits)."""
+"""The oracle's occupancy statistics of a zone pass (occupancy_stats,
+restating zone_pass.cpp:975-1044) against the values the compiled original
+wrote in its Gauss-Seidel pass runout (tests/golden, 6 significant digits).
+The GPU reduction is compared with this oracle in tests/test_gpu_diag.py."""
 import numpy as np
 import pytest
 
@@ -15,31 +16,37 @@ def counts(p, x):
     return np.bincount(iy + p.Ny * ix, minlength=p.Nc).astype(np.int32)
 
 
-@pytest.fixture(scope="module")
-def lib():
-    from src_incompressible_openmp_amd import build, gpu
-
-    import os
-    if not os.path.exists(build.LIB):
-        build.build()
-    return gpu
-
-
 @pytest.mark.parametrize("case", SMALL_CASES)
 @pytest.mark.parametrize("layout", ["base", "shifted"])
-def test_pass_occupancy_stats_match_original(lib, case, layout):
+def test_pass_occupancy_stats_match_original(oracle, case, layout):
     g = load(case)
     p = to_struct(case_params(case))
     r = kv(g, f"s2{layout}")
     for tag, x in (("Before", g["x0"]), ("After", g[f"s2{layout}_x"])):
-        s = lib.occupancy_stats(p, counts(p, x))
+        s = oracle.occupancy_stats(p, counts(p, x))
         assert close6(s["occStd"], r["occStd" + tag]), tag
         assert close6(s["outBand"], r["outBand" + tag]), tag
         assert s["nDense"] == r["nDense" + tag] and s["nSparse"] == r["nSparse" + tag], tag
         assert s["nEmptyActive"] == r["nEmptyActive" + tag], tag
 
 
-def test_local_fluid_fraction_thresholds_change_empty_cells(lib):
+def test_occupancy_closed_forms(oracle):
+    """Uniform counts: occStd 0 and nothing out of band; one cell raised by
+    k over a uniform grid of Nc cells: occStd = k sqrt(Nc-1)/Nc/gamma."""
+    p = to_struct(case_params("small"))
+    g = int(p.gamma)
+    c = np.full(p.Nc, g, np.int32)
+    s = oracle.occupancy_stats(p, c)
+    assert s["occStd"] == 0.0 and s["outBand"] == 0.0 and s["nDense"] == 0 and s["nSparse"] == 0
+    k = 7
+    c[3] += k
+    s = oracle.occupancy_stats(p, c)
+    want = k * np.sqrt(p.Nc - 1) / p.Nc / p.gamma
+    assert abs(s["occStd"] - want) <= 1e-14 * want
+    assert s["nDense"] == 1 and s["outBand"] == 1.0 / p.Nc
+
+
+def test_local_fluid_fraction_thresholds_change_empty_cells(oracle):
     """With local thresholds an empty cell next to a full region gets the
     smoothed (not the constant) empty-cell thresholds."""
     prm = case_params("small")
@@ -47,5 +54,5 @@ def test_local_fluid_fraction_thresholds_change_empty_cells(lib):
     p = to_struct(prm)
     c = np.full(p.Nc, int(p.gamma), np.int32)
     c[0] = 0
-    s = lib.occupancy_stats(p, c)
+    s = oracle.occupancy_stats(p, c)
     assert s["nEmptyActive"] == 1 and s["nDense"] == 0
