@@ -188,6 +188,24 @@ int mpcd_gpu_srd_collision_step(const mpcd_params* p, double* v, const int32_t* 
 int mpcd_gpu_compute_cell_fields(const mpcd_params* p, const double* x, const double* v, double rhoTarget,
                                  mpcd_cell_fields* out);
 
+/* No counterpart in the reference: compute_cell_fields (as
+ * mpcd_gpu_compute_cell_fields above, same results bit for bit) for callers
+ * that cannot allocate device memory -- code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc.  x, v (interleaved, 2n
+ * doubles) and every non-NULL array of `out` are DEVICE pointers the caller
+ * owns; the work is enqueued on `stream` (a hipStream_t, NULL for the null
+ * stream) and the call returns without waiting.  It allocates nothing: its
+ * working memory is a static device array of the library (about 39 MB per
+ * GPU, held until the process ends) that bounds the problem to
+ * MPCD_NOALLOC_MAX_PARTICLES particles and MPCD_NOALLOC_MAX_CELLS cells
+ * (larger sizes return MPCD_E_INVALID before anything is enqueued).  Every
+ * call on one GPU shares that array: calls that overlap on one GPU (on
+ * different streams, or from several threads) race. */
+#define MPCD_NOALLOC_MAX_PARTICLES (1 << 20)
+#define MPCD_NOALLOC_MAX_CELLS (1 << 18)
+int mpcd_gpu_compute_cell_fields_noalloc(const mpcd_params* p, const double* x, const double* v, double rhoTarget,
+                                         const mpcd_cell_fields* out, void* stream);
+
 /* replaces mpcd::build_zone_layout (zone_pass.h:60); rects4 (4 per zone:
  * xs, xe, ys, ye) may be NULL */
 int mpcd_gpu_build_zone_layout(const mpcd_params* p, int mode, int32_t* ownerMap, int32_t* nZones, int32_t* rects4);

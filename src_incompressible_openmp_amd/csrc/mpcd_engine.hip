@@ -2442,6 +2442,61 @@ struct mpcd_gpu_sim {
 };
 
 /* ===================================================================== */
+/*          compute_cell_fields without allocation (static scratch)      */
+/* ===================================================================== */
+
+namespace {
+
+/* Scratch of mpcd_gpu_compute_cell_fields_noalloc: a file-scope device
+ * array (each GPU the library runs on holds its own copy until the process
+ * ends).  Regions, each 256-byte aligned: vx, vy (f64), cid, pid, list
+ * (i32) and the sort keys (u64) per particle; counts, starts, cursors (i32)
+ * per cell + 1; the scan's tile totals. */
+constexpr size_t NA_N = MPCD_NOALLOC_MAX_PARTICLES, NA_C = (size_t)MPCD_NOALLOC_MAX_CELLS + 1;
+constexpr size_t na_al(size_t b) { return (b + 255) & ~(size_t)255; }
+constexpr size_t NA_VX = 0, NA_VY = NA_VX + na_al(8 * NA_N), NA_CID = NA_VY + na_al(8 * NA_N),
+                 NA_PID = NA_CID + na_al(4 * NA_N), NA_LIST = NA_PID + na_al(4 * NA_N),
+                 NA_KEY = NA_LIST + na_al(4 * NA_N), NA_CNT = NA_KEY + na_al(8 * NA_N),
+                 NA_START = NA_CNT + na_al(4 * NA_C), NA_CUR = NA_START + na_al(4 * NA_C),
+                 NA_TILE = NA_CUR + na_al(4 * NA_C),
+                 NA_BYTES = NA_TILE + na_al(4 * ((NA_C + SCAN_TILE - 1) / SCAN_TILE));
+
+static __device__ __align__(256) unsigned char scratch[NA_BYTES];
+
+/* per-device address of `scratch` (the host-side name is not a device
+ * address), taken once per device with that device current */
+constexpr int NA_MAX_DEV = 64;
+std::mutex g_na_mu;
+unsigned char* g_na_ptr[NA_MAX_DEV] = {};
+
+unsigned char* noalloc_scratch() {
+    int dev = 0;
+    HIP_OK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= NA_MAX_DEV) throw ApiError(MPCD_E_INVALID, "device index beyond the scratch table");
+    std::lock_guard<std::mutex> lk(g_na_mu);
+    if (!g_na_ptr[dev]) {
+        void* ptr = nullptr;
+        HIP_OK(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+        if (!ptr) throw ApiError(MPCD_E_HIP, "hipGetSymbolAddress returned no address for the scratch array");
+        g_na_ptr[dev] = (unsigned char*)ptr;
+    }
+    return g_na_ptr[dev];
+}
+
+/* unshifted cells of interleaved positions (common_grid.cpp:9-19) */
+__global__ void k_cells_xy(Geom g, const double* __restrict__ xy, int* __restrict__ cid) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < g.n) cid[i] = cell_of(g, xy[2 * (size_t)i], xy[2 * (size_t)i + 1]);
+}
+
+__global__ void k_fill_double(double* a, int n, double v) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) a[i] = v;
+}
+
+}  // namespace
+
+/* ===================================================================== */
 /*                                C ABI                                  */
 /* ===================================================================== */
 
@@ -2700,6 +2755,60 @@ int mpcd_gpu_compute_cell_fields(const mpcd_params* p, const double* x, const do
             for (int c = 0; c < s.Nc; ++c) out->rhoTarget[c] = rhoTarget;
         for (auto q : d) HIP_OK(hipFree(q));
         HIP_OK(hipFree(dN));
+    });
+}
+
+int mpcd_gpu_compute_cell_fields_noalloc(const mpcd_params* p, const double* x, const double* v, double rhoTarget,
+                                         const mpcd_cell_fields* out, void* stream) {
+    return guarded([&] {
+        if (!p || !x || !v || !out) throw ApiError(MPCD_E_INVALID, "null argument");
+        validate(*p);
+        if (p->n > MPCD_NOALLOC_MAX_PARTICLES || p->Nc > MPCD_NOALLOC_MAX_CELLS)
+            throw ApiError(MPCD_E_INVALID, "compute_cell_fields_noalloc: at most " +
+                                               std::to_string(MPCD_NOALLOC_MAX_PARTICLES) + " particles and " +
+                                               std::to_string(MPCD_NOALLOC_MAX_CELLS) + " cells (got n=" +
+                                               std::to_string(p->n) + ", Nc=" + std::to_string(p->Nc) + ")");
+        unsigned char* b = noalloc_scratch();
+        hipStream_t st = (hipStream_t)stream;
+        const int n = p->n, Nc = p->Nc;
+        const Geom g = make_geom(*p);
+        double* vx = (double*)(b + NA_VX);
+        double* vy = (double*)(b + NA_VY);
+        int* cid = (int*)(b + NA_CID);
+        int* pid = (int*)(b + NA_PID);
+        int* list = (int*)(b + NA_LIST);
+        unsigned long long* key = (unsigned long long*)(b + NA_KEY);
+        int* cnt = (int*)(b + NA_CNT);
+        int* start = (int*)(b + NA_START);
+        int* cursor = (int*)(b + NA_CUR);
+        int* tileSum = (int*)(b + NA_TILE);
+        const int nTiles = (int)((Nc + 1 + SCAN_TILE - 1) / SCAN_TILE);
+        /* the scratch keeps the last call's contents: everything read is
+         * written first (counts cleared by a fill kernel, not a memset) */
+        if (n > 0) {
+            hipLaunchKernelGGL(k_deinterleave, dim3(nblk(n)), dim3(TPB), 0, st, v, n, vx, vy);
+            hipLaunchKernelGGL(k_cells_xy, dim3(nblk(n)), dim3(TPB), 0, st, g, x, cid);
+            hipLaunchKernelGGL(k_iota, dim3(nblk(n)), dim3(TPB), 0, st, pid, n);
+        }
+        hipLaunchKernelGGL(k_fill_int, dim3(nblk(Nc + 1)), dim3(TPB), 0, st, cnt, Nc + 1, 0);
+        if (n > 0) hipLaunchKernelGGL(k_hist, dim3(bin_blocks(n)), dim3(TPB), 0, st, cid, n, cnt);
+        hipLaunchKernelGGL(k_scan_tiles, dim3(nTiles), dim3(TPB), 0, st, cnt, Nc + 1, start, tileSum,
+                           (const int*)nullptr, cursor);
+        hipLaunchKernelGGL(k_scan_add_prefix, dim3(nblk(Nc + 1)), dim3(TPB), 0, st, start, Nc + 1, tileSum);
+        if (n > 0) {
+            hipLaunchKernelGGL(k_scatter, dim3(bin_blocks(n)), dim3(TPB), 0, st, cid, pid, n, start, cursor, key);
+            hipLaunchKernelGGL(k_segsort, dim3(nblk(Nc, SEG_CELLS)), dim3(SEG_CELLS), 0, st, start, Nc, key, list,
+                               (int)(n < (1 << 27)), 0);
+        }
+        const double mean = (double)n / (double)std::max(Nc, 1) * CELL_BLK;
+        const int cap = (int)std::min(4096.0, std::max(256.0, 1.15 * mean + 2.0 * CELL_BLK));
+        FieldsOut f{out->N, out->Ux, out->Uy, out->Px, out->Py, out->T, out->rho, out->Pkin, out->Pvir, out->P};
+        const double vc = (p->Lx / (double)p->Nx) * (p->Ly / (double)p->Ny);
+        hipLaunchKernelGGL(k_cell_fields, dim3(nblk(Nc, CELL_BLK)), dim3(CELL_BLK), cap * 2 * sizeof(double), st,
+                           start, list, Nc, vx, vy, vc, p->Kvirial, rhoTarget, f, cap);
+        if (out->rhoTarget)
+            hipLaunchKernelGGL(k_fill_double, dim3(nblk(Nc)), dim3(TPB), 0, st, out->rhoTarget, Nc, rhoTarget);
+        HIP_OK(hipGetLastError());
     });
 }
 
