@@ -73,12 +73,27 @@ def pmc_traffic(kernel: str, workload: str):
     return None
 
 
-def cpu_baseline(nx: int, ny: int, ppc: int, steps: int):
-    """The compiled original's benchmark driver on a bounded sample."""
+def host_info():
+    """The host the CPU baselines run on: the CPUs this process may use, the
+    count the OS reports, the model name and the OpenMP threads given."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    aff = len(os.sched_getaffinity(0))
+    return {"nproc": aff, "cpu_count": os.cpu_count(), "cpu_model": model,
+            "omp_num_threads_env": os.environ.get("OMP_NUM_THREADS")}
+
+
+def run_original(nx: int, ny: int, ppc: int, steps: int, threads: int) -> float:
+    """timePerStepMean of the compiled original's benchmark driver on a
+    synthetic channel of nx x ny cells."""
     exe = os.path.join(ROOT, "oracle", "_ref", "main_benchmark_poiseuille_openmp_profiled_shifted_profiled")
-    if not os.path.exists(exe):
-        return None
-    threads = max(1, min(16, os.cpu_count() or 1))
     prm = poiseuille_params(Nx=nx, Ny=ny, gamma=float(ppc))
     n = int(prm["n"])
     x, v, t, r0 = uniform_state(n, float(prm["Lx"]), float(prm["Ly"]), seed=3)
@@ -91,28 +106,68 @@ def cpu_baseline(nx: int, ny: int, ppc: int, steps: int):
         write_u8(os.path.join(d, "t.bin"), t, n)
         write_xy(os.path.join(d, "r0.bin"), r0, n)
         env = dict(os.environ, OMP_NUM_THREADS=str(threads))
-        t0 = time.perf_counter()
         subprocess.run([exe, os.path.join(d, "params.kv"), os.path.join(d, "x.bin"), os.path.join(d, "v.bin"),
                         os.path.join(d, "t.bin"), os.path.join(d, "r0.bin"), os.path.join(d, "o"), "1", "1",
                         str(threads)], check=True, env=env, stdout=subprocess.DEVNULL, timeout=600)
-        wall = time.perf_counter() - t0
         kv = {}
         with open(os.path.join(d, "o_runout.kv")) as fh:
             for line in fh:
                 if "=" in line:
                     k, val = line.strip().split("=", 1)
                     kv[k] = val
-        per_step = float(kv["timePerStepMean"])
+        return float(kv["timePerStepMean"])
     finally:
         shutil.rmtree(d, ignore_errors=True)
+
+
+def cpu_baseline(n_full: int, ppc: int, steps: int):
+    """The compiled original (OpenMP, every thread this process is given) on
+    two bounded samples, 128x64 and 256x128 cells.  Its step costs a*n +
+    b*n^2 (each zone of a pass copies the whole state,
+    zone_pass_openmp.cpp:67-72), so the two samples fix a and b and give the
+    extrapolated time of one step of the full workload; ``value`` is the
+    measured throughput of the larger sample."""
+    exe = os.path.join(ROOT, "oracle", "_ref", "main_benchmark_poiseuille_openmp_profiled_shifted_profiled")
+    if not os.path.exists(exe):
+        return None
+    host = host_info()
+    threads = int(host["omp_num_threads_env"] or host["nproc"])
+    samples = []
+    for (nx, ny) in ((128, 64), (256, 128)):
+        t = run_original(nx, ny, ppc, steps, threads)
+        samples.append({"cells": [nx, ny], "particles": nx * ny * ppc, "s_per_step": t,
+                        "particle_steps_per_s": nx * ny * ppc / t})
+    (n1, t1), (n2, t2) = [(q["particles"], q["s_per_step"]) for q in samples]
+    det = n1 * n2 * n2 - n2 * n1 * n1
+    a = (t1 * n2 * n2 - t2 * n1 * n1) / det
+    b = (n1 * t2 - n2 * t1) / det
+    t_full = a * n_full + b * n_full * n_full
     return {
-        "value": n / per_step, "unit": "particle-steps/s", "cores": threads, "kind": "reference",
-        "sample": (f"original OpenMP driver (main_benchmark_poiseuille_openmp_profiled_shifted_profiled, g++ -O3 "
-                   f"-fopenmp) on {nx}x{ny} cells x {ppc} ppc (n={n}), {steps} steps, same physics; "
-                   f"its cost per step grows ~quadratically with domain size, so at 1024x512 its throughput is "
-                   f"lower than this sample's"),
-        "s_per_step": per_step, "wall_s": wall,
+        "value": samples[1]["particle_steps_per_s"], "unit": "particle-steps/s", "cores": threads,
+        "kind": "reference",
+        "sample": (f"compiled original (main_benchmark_poiseuille_openmp_profiled_shifted_profiled, g++ -O3 -fopenmp, "
+                   f"{threads} OpenMP threads) on the same physics at 256x128 cells x {ppc} ppc (n={n2}), {steps} "
+                   f"steps; also 128x64 (n={n1}); value = the 256x128 sample's throughput"),
+        "samples": samples, "host": host,
+        "extrapolated_full_size": {"particles": n_full, "model": "t = a n + b n^2 fitted to the two samples",
+                                   "a": a, "b": b, "s_per_step": t_full, "particle_steps_per_s": n_full / t_full},
     }
+
+
+def cpu_port_full_size(p):
+    """One step of the oracle's tile-local restatement (C++, one thread) on
+    the full workload: a CPU baseline at the GPU's own size."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from tests.oracle_lib import CTR, get_oracle
+
+    x, v, _, _ = uniform_state(p.n, p.Lx, p.Ly, seed=1)
+    o = get_oracle()
+    t0 = time.perf_counter()
+    o.step(CTR, p, x, v, 1)
+    dt = time.perf_counter() - t0
+    return {"value": p.n / dt, "unit": "particle-steps/s", "cores": 1, "kind": "port", "s_per_step": dt,
+            "sample": f"one step of the full {p.Nx}x{p.Ny} workload by the oracle's tile-local C++ restatement "
+                      f"(oracle/, g++ -O2, one thread; O(n) per zone pass where the original is O(n^2))"}
 
 
 def learned_closure(sim, p, s0, args):
@@ -184,10 +239,9 @@ def main() -> int:
     ap.add_argument("--nx", type=int, default=1024)
     ap.add_argument("--ny", type=int, default=512)
     ap.add_argument("--ppc", type=int, default=20)
-    ap.add_argument("--cpu-nx", type=int, default=256)
-    ap.add_argument("--cpu-ny", type=int, default=128)
-    ap.add_argument("--cpu-steps", type=int, default=5)
+    ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-cpu-port", action="store_true", help="skip the one-step oracle timing at full size")
     ap.add_argument("--no-profile", action="store_true", help="time without per-kernel events")
     ap.add_argument("--no-learned", action="store_true", help="skip the closure-surrogate measurement")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -279,11 +333,16 @@ def main() -> int:
                         "moved_sparse": m["base"]["nParticlesMovedSparse"] + m["shifted"]["nParticlesMovedSparse"],
                         "beta_applied": m["closure"]["betaRepairApplied"]}
     if not args.no_cpu_baseline:
-        log(f"bench: CPU baseline {args.cpu_nx}x{args.cpu_ny} x {args.ppc}, {args.cpu_steps} steps")
-        cb = cpu_baseline(args.cpu_nx, args.cpu_ny, args.ppc, args.cpu_steps)
+        log(f"bench: CPU baseline (compiled original, 128x64 and 256x128 x {args.ppc}, {args.cpu_steps} steps each)")
+        cb = cpu_baseline(n, args.ppc, args.cpu_steps)
         out["cpu_baseline"] = cb
         if cb:
-            out["gpu_over_cpu_sample"] = value / cb["value"]
+            out["gpu_over_cpu_extrapolated_full_size"] = value / cb["extrapolated_full_size"]["particle_steps_per_s"]
+        if not args.no_cpu_port:
+            log("bench: CPU port, one step of the full workload")
+            cp = cpu_port_full_size(p)
+            out["cpu_port_full_size"] = cp
+            out["gpu_over_cpu_port_full_size"] = value / cp["value"]
     print(json.dumps(out), flush=True)
     return 0
 

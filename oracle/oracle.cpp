@@ -334,7 +334,10 @@ Layout build_layout(const P& p, int mode) {
     if (mode == MPCD_LAYOUT_SINGLE_FULL || p.zoneSingleFullDomainMode) {
         Zone z;
         z.id = 1;
-        z.mask.assign((size_t)p.Nc, 1);
+        z.x0 = 0;
+        z.x1 = p.Nx;
+        z.y0 = 0;
+        z.y1 = p.Ny;
         for (int c = 0; c < p.Nc; ++c) {
             z.cells.push_back(c);
             L.owner[(size_t)c] = 1;
@@ -353,12 +356,14 @@ Layout build_layout(const P& p, int mode) {
             const int y1 = std::min(y0 + p.zoneTileNy, p.Ny);
             Zone z;
             z.id = ++id;
-            z.mask.assign((size_t)p.Nc, 0);
+            z.x0 = x0;
+            z.x1 = x1;
+            z.y0 = y0;
+            z.y1 = y1;
             for (int ix = x0; ix < x1; ++ix)
                 for (int iy = y0; iy < y1; ++iy) {
                     const int c = iy + p.Ny * ix;
-                    z.cells.push_back(c);
-                    z.mask[(size_t)c] = 1;
+                    z.cells.push_back(c); /* ascending cell id */
                     L.owner[(size_t)c] = z.id;
                 }
             L.zones.push_back(std::move(z));
@@ -662,57 +667,129 @@ std::vector<int> rank_donors(const P& p, int cell, std::vector<int> donors, doub
 
 /* ============ zone kernel: zone_pass.cpp:669-973 ============ */
 
+namespace {
+
+/* 3x3 binomial smoothing of one cell, the terms of smooth_counts in its order */
+inline double smooth_at(const P& p, const std::vector<int>& cnt, int ix, int iy) {
+    static const double K[9] = {1.0 / 16.0, 2.0 / 16.0, 1.0 / 16.0, 2.0 / 16.0, 4.0 / 16.0,
+                                2.0 / 16.0, 1.0 / 16.0, 2.0 / 16.0, 1.0 / 16.0};
+    double acc = 0.0, ws = 0.0;
+    int kk = 0;
+    for (int dy = -1; dy <= 1; ++dy)
+        for (int dx = -1; dx <= 1; ++dx, ++kk) {
+            const int jy = iy + dy, jx = ix + dx;
+            if (jy < 0 || jy >= p.Ny || jx < 0 || jx >= p.Nx) continue;
+            acc += K[kk] * (double)cnt[(size_t)(jy + p.Ny * jx)];
+            ws += K[kk];
+        }
+    return acc / std::max(ws, 1e-30);
+}
+
+/* the empty-cell thresholds of one cell, the arithmetic of threshold_maps */
+inline void map_at(const P& p, double Nsm, double& lowLoc, double& highLoc) {
+    const double lowFrac = 1.0 / (1.0 + p.coef);
+    const double highFrac = 1.0 + p.coef;
+    double f = 1.0;
+    if (p.useLocalFluidFractionThresholds) {
+        const double fRaw = Nsm / std::max(p.gamma, 1e-12);
+        f = std::clamp(p.fluidFracGain * fRaw, 0.0, 1.0);
+    }
+    const double gl = p.gamma * f;
+    lowLoc = std::max(lowFrac * gl, p.lowThrFloor);
+    highLoc = std::max(highFrac * gl, p.highThrFloor);
+}
+
+/* Working arrays of the zone kernel, indexed by global cell id.  The
+ * full-domain form (zone_kernel) sizes a fresh set for one zone, as the
+ * reference's kernel builds its per-cell lists over the whole domain
+ * (zone_pass.cpp:680-685).  The tile-local form (zone_pass_snapshot) keeps
+ * one set for a pass, holding the pass input, and restores after each zone
+ * the entries that zone changed: a zone then costs what its cells and their
+ * ring of width 2 hold, not the domain. */
+struct ZoneWork {
+    std::vector<int> cnt, bulk, bulkBefore;
+    std::vector<std::vector<int>> parts;
+    std::vector<double> PbX, PbY, Nsm, lowLoc, highLoc, lowA, highA;
+    std::vector<char> recv, bb, skip;
+    std::vector<double> dU2, pxE, pyE, uxN, uyN;
+    void init(size_t Nc) {
+        cnt.assign(Nc, 0);
+        bulk.assign(Nc, 0);
+        bulkBefore.assign(Nc, 0);
+        parts.assign(Nc, {});
+        for (auto* a : {&PbX, &PbY, &Nsm, &lowLoc, &highLoc, &lowA, &highA, &dU2, &pxE, &pyE, &uxN, &uyN})
+            a->assign(Nc, 0.0);
+        recv.assign(Nc, 0);
+        bb.assign(Nc, 0);
+        skip.assign(Nc, 0);
+    }
+};
+
+/* every cell a zone kernel reads a count or a map of: the zone and the ring
+ * of width 2 around it (wrapped on periodic axes, clipped at walls) */
+std::vector<int> zone_region(const P& p, const Zone& z) {
+    std::vector<int> r;
+    for (int ix = z.x0 - 2; ix < z.x1 + 2; ++ix)
+        for (int iy = z.y0 - 2; iy < z.y1 + 2; ++iy) {
+            int jx = ix, jy = iy;
+            if (per_x(p)) jx = wrapi(jx, p.Nx);
+            else if (jx < 0 || jx >= p.Nx) continue;
+            if (per_y(p)) jy = wrapi(jy, p.Ny);
+            else if (jy < 0 || jy >= p.Ny) continue;
+            r.push_back(jy + p.Ny * jx);
+        }
+    std::sort(r.begin(), r.end());
+    r.erase(std::unique(r.begin(), r.end()), r.end());
+    return r;
+}
+
+/* The decisions and the momentum correction of run_zone_kernel
+ * (zone_pass.cpp:706-922) on W, which holds the counts and cell lists
+ * (ascending particle id) of the kernel's input.  Maps are built over
+ * `region`; the dense and low lists and the correction run over the zone's
+ * cells in ascending id, which is the reference's order over the whole
+ * domain with the zone mask (every receiving cell is a zone cell). */
 template <class R>
-ZoneStats zone_kernel(const P& p, std::vector<double>& x, std::vector<double>& v, const Zone& z, R& rng) {
-    ZoneStats out;
-    const size_t Nc = (size_t)p.Nc;
-    const auto& zmask = z.mask; /* active mask is all ones (:128-130), so zoneActive == zone mask */
-    const auto cid0 = cell_ids(p, x);
-    std::vector<std::vector<int>> parts(Nc);
-    for (int i = 0; i < p.n; ++i) parts[(size_t)cid0[(size_t)i]].push_back(i);
-    std::vector<int> cnt(Nc);
-    for (size_t c = 0; c < Nc; ++c) cnt[c] = (int)parts[c].size();
-    std::vector<double> PbX(Nc, 0.0), PbY(Nc, 0.0);
-    for (size_t c = 0; c < Nc; ++c) {
+void zone_core(const P& p, std::vector<double>& x, std::vector<double>& v, const Zone& z, R& rng, ZoneWork& W,
+               const std::vector<int>& region, ZoneStats& out) {
+    auto& cnt = W.cnt;
+    auto& parts = W.parts;
+    for (int c : z.cells) {
         double px = 0.0, py = 0.0;
-        for (int id : parts[c]) {
+        for (int id : parts[(size_t)c]) {
             px += v[2 * (size_t)id];
             py += v[2 * (size_t)id + 1];
         }
-        PbX[c] = px;
-        PbY[c] = py;
+        W.PbX[(size_t)c] = px;
+        W.PbY[(size_t)c] = py;
     }
-    std::vector<char> recv(Nc, 0), bb(Nc, 0), skip(Nc, 0);
-    double PgX0 = 0.0, PgY0 = 0.0, E0 = 0.0;
-    for (int i = 0; i < p.n; ++i) {
-        PgX0 += v[2 * (size_t)i];
-        PgY0 += v[2 * (size_t)i + 1];
-    }
-    for (int i = 0; i < p.n; ++i)
-        E0 += 0.5 * (v[2 * (size_t)i] * v[2 * (size_t)i] + v[2 * (size_t)i + 1] * v[2 * (size_t)i + 1]);
-
     double highThr, lowThrBulk;
     occupancy_thresholds(p, highThr, lowThrBulk);
-
-    std::vector<double> lowA(Nc, 0.0), highA(Nc, 0.0), Nsm, lowLoc, highLoc;
-    std::vector<int> bulk(Nc, 0);
+    auto& lowA = W.lowA;
+    auto& highA = W.highA;
+    auto& Nsm = W.Nsm;
+    auto& recv = W.recv;
+    auto& bb = W.bb;
+    auto& skip = W.skip;
     auto build_active_maps = [&]() {
-        Nsm = smooth_counts(p, cnt);
-        threshold_maps(p, Nsm, lowLoc, highLoc);
-        for (size_t c = 0; c < Nc; ++c) {
-            bulk[c] = cnt[c] > 0 ? 1 : 0;
-            lowA[c] = bulk[c] ? lowThrBulk : lowLoc[c];
-            highA[c] = bulk[c] ? highThr : highLoc[c];
+        for (int c : region) {
+            const int ix = c / p.Ny, iy = c - ix * p.Ny;
+            Nsm[(size_t)c] = smooth_at(p, cnt, ix, iy);
+            map_at(p, Nsm[(size_t)c], W.lowLoc[(size_t)c], W.highLoc[(size_t)c]);
+            W.bulk[(size_t)c] = cnt[(size_t)c] > 0 ? 1 : 0;
+            lowA[(size_t)c] = W.bulk[(size_t)c] ? lowThrBulk : W.lowLoc[(size_t)c];
+            highA[(size_t)c] = W.bulk[(size_t)c] ? highThr : W.highLoc[(size_t)c];
         }
     };
+    auto inz = [&](int c) { return z.in(c, p.Ny); };
 
     for (int pass = 1; pass <= p.maxRedistribPasses; ++pass) { /* :708-781 */
         int nHigh = 0, moved = 0;
         build_active_maps();
-        const std::vector<int> bulkBefore = bulk;
+        for (int c : z.cells) W.bulkBefore[(size_t)c] = W.bulk[(size_t)c];
         std::vector<int> dense;
-        for (size_t c = 0; c < Nc; ++c)
-            if (zmask[c] && (double)cnt[c] > highA[c]) dense.push_back((int)c);
+        for (int c : z.cells)
+            if ((double)cnt[(size_t)c] > highA[(size_t)c]) dense.push_back(c);
         if (!dense.empty()) rng.shuffle_dense(dense, pass);
         for (int c : dense) {
             const int nc = cnt[(size_t)c];
@@ -726,8 +803,7 @@ ZoneStats zone_kernel(const P& p, std::vector<double>& x, std::vector<double>& v
             const int needOut = std::max(0, (int)std::ceil((double)cnt[(size_t)c] - hl));
             if (needOut > 0) {
                 auto poor = admissible(p, c, cnt, lowA, 2);
-                poor.erase(std::remove_if(poor.begin(), poor.end(), [&](int q) { return !zmask[(size_t)q]; }),
-                           poor.end());
+                poor.erase(std::remove_if(poor.begin(), poor.end(), [&](int q) { return !inz(q); }), poor.end());
                 auto ordP = rank_dense(p, c, poor, cnt, lowA, pass, 0, rng);
                 int mv = 0;
                 auto rn = spread(p, x, parts, cnt, c, ordP, lowA, needOut, pass, rng, mv);
@@ -739,8 +815,7 @@ ZoneStats zone_kernel(const P& p, std::vector<double>& x, std::vector<double>& v
                 const int remaining = std::max(0, (int)std::ceil((double)cnt[(size_t)c] - hl));
                 if (remaining > 0) {
                     auto relief = admissible(p, c, cnt, highA, 2);
-                    relief.erase(std::remove_if(relief.begin(), relief.end(),
-                                                [&](int q) { return !zmask[(size_t)q]; }),
+                    relief.erase(std::remove_if(relief.begin(), relief.end(), [&](int q) { return !inz(q); }),
                                  relief.end());
                     auto ordR = rank_dense(p, c, relief, cnt, highA, pass, 1, rng);
                     rn = spread(p, x, parts, cnt, c, ordR, highA, remaining, pass, rng, mv);
@@ -752,7 +827,7 @@ ZoneStats zone_kernel(const P& p, std::vector<double>& x, std::vector<double>& v
                 }
             }
             for (int r : recvT) recv[(size_t)r] = 1;
-            transfer_masks(bb, skip, c, recvT, bulkBefore);
+            transfer_masks(bb, skip, c, recvT, W.bulkBefore);
             moved += movedCell;
             out.movedDense += movedCell;
         }
@@ -763,10 +838,10 @@ ZoneStats zone_kernel(const P& p, std::vector<double>& x, std::vector<double>& v
     }
 
     build_active_maps(); /* :783-794 */
-    const std::vector<int> bulkLow = bulk;
+    for (int c : z.cells) W.bulkBefore[(size_t)c] = W.bulk[(size_t)c]; /* the sparse phase's bulk snapshot */
     std::vector<int> low;
-    for (size_t c = 0; c < Nc; ++c)
-        if (zmask[c] && cnt[c] > 0 && (double)cnt[c] < lowA[c]) low.push_back((int)c);
+    for (int c : z.cells)
+        if (cnt[(size_t)c] > 0 && (double)cnt[(size_t)c] < lowA[(size_t)c]) low.push_back(c);
     if (!low.empty()) rng.shuffle_low(low);
 
     int nLow = 0, movedS = 0;
@@ -786,7 +861,7 @@ ZoneStats zone_kernel(const P& p, std::vector<double>& x, std::vector<double>& v
                     int jy, jx;
                     if (!shift_index(p, iy, ix, dy, dx, jy, jx)) continue;
                     const int q = jy + p.Ny * jx;
-                    if (zmask[(size_t)q]) neigh.push_back(q);
+                    if (inz(q)) neigh.push_back(q);
                 }
             neigh = unique_stable(neigh);
         }
@@ -795,7 +870,7 @@ ZoneStats zone_kernel(const P& p, std::vector<double>& x, std::vector<double>& v
         if (need <= 0) continue;
         std::vector<int> donors;
         for (int d : neigh)
-            if (zmask[(size_t)d] && cnt[(size_t)d] > lowInt) donors.push_back(d);
+            if (inz(d) && cnt[(size_t)d] > lowInt) donors.push_back(d);
         if (donors.empty()) continue;
         double gx, gy;
         inward_dir(p, c, Nsm, gx, gy);
@@ -819,7 +894,7 @@ ZoneStats zone_kernel(const P& p, std::vector<double>& x, std::vector<double>& v
             cnt[(size_t)d] = (int)parts[(size_t)d].size();
             cnt[(size_t)c] += nt;
             recv[(size_t)c] = 1;
-            transfer_masks(bb, skip, d, std::vector<int>{c}, bulkLow);
+            transfer_masks(bb, skip, d, std::vector<int>{c}, W.bulkBefore);
             movedS += nt;
             out.movedSparse += nt;
             need -= nt;
@@ -830,18 +905,11 @@ ZoneStats zone_kernel(const P& p, std::vector<double>& x, std::vector<double>& v
     out.stats[2] += (double)movedS;
     out.stats[3] += (double)movedS;
 
-    { /* :861-868: final high map for the diagnostics (the low map stays) */
-        Nsm = smooth_counts(p, cnt);
-        threshold_maps(p, Nsm, lowLoc, highLoc);
-        for (size_t c = 0; c < Nc; ++c) highA[c] = (cnt[c] > 0) ? highThr : highLoc[c];
-    }
-
-    /* momentum correction :870-909 */
-    std::vector<double> dU2(Nc, 0.0), pxE(Nc, 0.0), pyE(Nc, 0.0), uxN(Nc, 0.0), uyN(Nc, 0.0);
+    /* momentum correction :870-909 (over the zone-touched cells: the zone's,
+     * which hold every receiving cell) */
     double dUmax = 0.0;
-    for (size_t c = 0; c < Nc; ++c) {
-        if (!(zmask[c] || recv[c])) continue;
-        const auto& ids = parts[c];
+    for (int c : z.cells) {
+        const auto& ids = parts[(size_t)c];
         const int nc = (int)ids.size();
         if (nc == 0) continue;
         double pxA = 0.0, pyA = 0.0;
@@ -849,17 +917,17 @@ ZoneStats zone_kernel(const P& p, std::vector<double>& x, std::vector<double>& v
             pxA += v[2 * (size_t)id];
             pyA += v[2 * (size_t)id + 1];
         }
-        uxN[c] = pxA / (double)nc;
-        uyN[c] = pyA / (double)nc;
-        const double dPx = PbX[c] - pxA, dPy = PbY[c] - pyA;
+        W.uxN[(size_t)c] = pxA / (double)nc;
+        W.uyN[(size_t)c] = pyA / (double)nc;
+        const double dPx = W.PbX[(size_t)c] - pxA, dPy = W.PbY[(size_t)c] - pyA;
         const double dux = dPx / (double)nc, duy = dPy / (double)nc;
-        pxE[c] = dPx * dPx;
-        pyE[c] = dPy * dPy;
-        dU2[c] = dux * dux + duy * duy;
-        dUmax = std::max(dUmax, std::sqrt(dU2[c]));
+        W.pxE[(size_t)c] = dPx * dPx;
+        W.pyE[(size_t)c] = dPy * dPy;
+        W.dU2[(size_t)c] = dux * dux + duy * duy;
+        dUmax = std::max(dUmax, std::sqrt(W.dU2[(size_t)c]));
         bool corr = p.enableMomentumCorrectionPostRedistribution != 0;
-        if (corr && skip[c]) corr = false;
-        if (corr && recv[c] && !bb[c]) corr = false;
+        if (corr && skip[(size_t)c]) corr = false;
+        if (corr && recv[(size_t)c] && !bb[(size_t)c]) corr = false;
         if (corr && (dux != 0.0 || duy != 0.0))
             for (int id : ids) {
                 v[2 * (size_t)id] += dux;
@@ -868,15 +936,55 @@ ZoneStats zone_kernel(const P& p, std::vector<double>& x, std::vector<double>& v
     }
     double acc = 0.0;
     int nDU = 0;
-    for (double d : dU2)
+    for (int c : z.cells) {
+        const double d = W.dU2[(size_t)c];
         if (d > 0.0) {
             acc += d;
             ++nDU;
         }
+    }
     if (nDU > 0) {
         out.stats[4] = std::sqrt(acc / (double)nDU);
         out.stats[5] = dUmax;
     }
+}
+
+}  // namespace
+
+/* run_zone_kernel (zone_pass.cpp:669-973) on the whole domain: cell lists
+ * of every cell, maps over every cell, and the full-domain diagnostics
+ * (:924-970). */
+template <class R>
+ZoneStats zone_kernel(const P& p, std::vector<double>& x, std::vector<double>& v, const Zone& z, R& rng) {
+    ZoneStats out;
+    const size_t Nc = (size_t)p.Nc;
+    ZoneWork W;
+    W.init(Nc);
+    const auto cid0 = cell_ids(p, x);
+    for (int i = 0; i < p.n; ++i) W.parts[(size_t)cid0[(size_t)i]].push_back(i);
+    for (size_t c = 0; c < Nc; ++c) W.cnt[c] = (int)W.parts[c].size();
+    double PgX0 = 0.0, PgY0 = 0.0, E0 = 0.0;
+    for (int i = 0; i < p.n; ++i) {
+        PgX0 += v[2 * (size_t)i];
+        PgY0 += v[2 * (size_t)i + 1];
+    }
+    for (int i = 0; i < p.n; ++i)
+        E0 += 0.5 * (v[2 * (size_t)i] * v[2 * (size_t)i] + v[2 * (size_t)i + 1] * v[2 * (size_t)i + 1]);
+    std::vector<int> all(Nc);
+    for (size_t c = 0; c < Nc; ++c) all[c] = (int)c;
+    zone_core(p, x, v, z, rng, W, all, out);
+
+    double highThr, lowThrBulk;
+    occupancy_thresholds(p, highThr, lowThrBulk);
+    { /* :861-868: final high map for the diagnostics (the low map stays) */
+        const auto Nsm = smooth_counts(p, W.cnt);
+        std::vector<double> lowLoc, highLoc;
+        threshold_maps(p, Nsm, lowLoc, highLoc);
+        for (size_t c = 0; c < Nc; ++c) W.highA[c] = (W.cnt[c] > 0) ? highThr : highLoc[c];
+    }
+    const auto& cnt = W.cnt;
+    const auto& highA = W.highA;
+    const auto& lowA = W.lowA;
 
     /* diagnostics :924-970 */
     double PgX1 = 0.0, PgY1 = 0.0, E1 = 0.0;
@@ -907,9 +1015,9 @@ ZoneStats zone_kernel(const P& p, std::vector<double>& x, std::vector<double>& v
         out.diag[2] = std::sqrt(dX * dX + dY * dY);
         double aX = 0.0, aY = 0.0;
         int nX = 0, nY = 0;
-        for (double q : pxE)
+        for (double q : W.pxE)
             if (q > 0.0) { aX += q; ++nX; }
-        for (double q : pyE)
+        for (double q : W.pyE)
             if (q > 0.0) { aY += q; ++nY; }
         out.diag[3] = nX > 0 ? std::sqrt(aX / (double)nX) : 0.0;
         out.diag[4] = nY > 0 ? std::sqrt(aY / (double)nY) : 0.0;
@@ -919,8 +1027,8 @@ ZoneStats zone_kernel(const P& p, std::vector<double>& x, std::vector<double>& v
         int nV = 0;
         for (size_t c = 0; c < Nc; ++c)
             if (cnt[c] > 0) {
-                sUx += uxN[c] * uxN[c];
-                sUy += uyN[c] * uyN[c];
+                sUx += W.uxN[c] * W.uxN[c];
+                sUy += W.uyN[c] * W.uyN[c];
                 ++nV;
             }
         out.diag[7] = 0.0;
@@ -945,9 +1053,22 @@ static void check_pass_modes(const P& p) {
 
 /* ============ snapshot zone pass: zone_pass_openmp.cpp:34-172 ============ */
 
+static void add_zone_metrics(PassMetrics& m, const ZoneStats& st, double& acc2, int& nc, double& mx) {
+    m.movedDense += st.movedDense;
+    m.movedSparse += st.movedSparse;
+    if (st.stats[4] > 0.0) {
+        acc2 += st.stats[4] * st.stats[4];
+        ++nc;
+    }
+    mx = std::max(mx, st.stats[5]);
+}
+
+/* The reference's structure: each zone's kernel on a private copy of the
+ * whole pass input, then the serial merge (the last zone covering a
+ * particle's input cell wins, :145-153).  Cost O(n) per zone. */
 template <class R>
-PassMetrics zone_pass_snapshot(const P& p, std::vector<double>& x, std::vector<double>& v, int mode,
-                               uint64_t seedBase, std::vector<int32_t>* owner) {
+PassMetrics zone_pass_snapshot_full(const P& p, std::vector<double>& x, std::vector<double>& v, int mode,
+                                    uint64_t seedBase, std::vector<int32_t>* owner) {
     check_pass_modes(p);
     const Layout L = build_layout(p, mode);
     if (owner) *owner = L.owner;
@@ -963,19 +1084,65 @@ PassMetrics zone_pass_snapshot(const P& p, std::vector<double>& x, std::vector<d
         R rng(seedBase + 104729ULL * (uint64_t)(iz + 1));
         const ZoneStats st = zone_kernel(p, xl, vl, z, rng);
         for (int i = 0; i < p.n; ++i) {
-            if (!z.mask[(size_t)cid0[(size_t)i]]) continue;
+            if (!z.in(cid0[(size_t)i], p.Ny)) continue;
             x[2 * (size_t)i] = xl[2 * (size_t)i];
             x[2 * (size_t)i + 1] = xl[2 * (size_t)i + 1];
             v[2 * (size_t)i] = vl[2 * (size_t)i];
             v[2 * (size_t)i + 1] = vl[2 * (size_t)i + 1];
         }
-        m.movedDense += st.movedDense;
-        m.movedSparse += st.movedSparse;
-        if (st.stats[4] > 0.0) {
-            acc2 += st.stats[4] * st.stats[4];
-            ++nc;
+        add_zone_metrics(m, st, acc2, nc, mx);
+    }
+    m.corrRmsDU = nc > 0 ? std::sqrt(acc2 / (double)nc) : 0.0;
+    m.corrMaxDU = mx;
+    return m;
+}
+
+/* The same pass, tile-local: the cell lists and counts of the pass input
+ * are built once; a zone's kernel reads its own cells and their ring of
+ * width 2 and changes only its own cells and particles, which are merged
+ * into the output and restored to the pass input before the next zone.
+ * Bitwise equal to zone_pass_snapshot_full (tests/test_oracle_fast.py),
+ * O(n) per pass: the form that checks the 1024 x 512 grid. */
+template <class R>
+PassMetrics zone_pass_snapshot(const P& p, std::vector<double>& x, std::vector<double>& v, int mode,
+                               uint64_t seedBase, std::vector<int32_t>* owner) {
+    check_pass_modes(p);
+    const Layout L = build_layout(p, mode);
+    if (owner) *owner = L.owner;
+    const size_t Nc = (size_t)p.Nc;
+    const std::vector<double> x0 = x, v0 = v;
+    std::vector<double> xw = x0, vw = v0;
+    const auto cid0 = cell_ids(p, x0);
+    ZoneWork W;
+    W.init(Nc);
+    for (int i = 0; i < p.n; ++i) W.parts[(size_t)cid0[(size_t)i]].push_back(i);
+    for (size_t c = 0; c < Nc; ++c) W.cnt[c] = (int)W.parts[c].size();
+    const auto parts0 = W.parts;
+    const auto cnt0 = W.cnt;
+    PassMetrics m;
+    m.nZones = (int)L.zones.size();
+    double acc2 = 0.0, mx = 0.0;
+    int nc = 0;
+    for (size_t iz = 0; iz < L.zones.size(); ++iz) {
+        const Zone& z = L.zones[iz];
+        R rng(seedBase + 104729ULL * (uint64_t)(iz + 1));
+        ZoneStats st;
+        zone_core(p, xw, vw, z, rng, W, zone_region(p, z), st);
+        for (int c : z.cells) {
+            for (int i : parts0[(size_t)c]) {
+                for (size_t k = 2 * (size_t)i; k < 2 * (size_t)i + 2; ++k) {
+                    x[k] = xw[k];
+                    v[k] = vw[k];
+                    xw[k] = x0[k];
+                    vw[k] = v0[k];
+                }
+            }
+            W.cnt[(size_t)c] = cnt0[(size_t)c];
+            W.parts[(size_t)c] = parts0[(size_t)c];
+            W.recv[(size_t)c] = W.bb[(size_t)c] = W.skip[(size_t)c] = 0;
+            W.dU2[(size_t)c] = W.pxE[(size_t)c] = W.pyE[(size_t)c] = W.uxN[(size_t)c] = W.uyN[(size_t)c] = 0.0;
         }
-        mx = std::max(mx, st.stats[5]);
+        add_zone_metrics(m, st, acc2, nc, mx);
     }
     m.corrRmsDU = nc > 0 ? std::sqrt(acc2 / (double)nc) : 0.0;
     m.corrMaxDU = mx;
@@ -986,6 +1153,10 @@ template PassMetrics zone_pass_snapshot<StdRngZone>(const P&, std::vector<double
                                                     uint64_t, std::vector<int32_t>*);
 template PassMetrics zone_pass_snapshot<CtrRngZone>(const P&, std::vector<double>&, std::vector<double>&, int,
                                                     uint64_t, std::vector<int32_t>*);
+template PassMetrics zone_pass_snapshot_full<StdRngZone>(const P&, std::vector<double>&, std::vector<double>&, int,
+                                                         uint64_t, std::vector<int32_t>*);
+template PassMetrics zone_pass_snapshot_full<CtrRngZone>(const P&, std::vector<double>&, std::vector<double>&, int,
+                                                         uint64_t, std::vector<int32_t>*);
 
 /* ============ Gauss-Seidel zone pass: zone_pass.cpp:1096-1163 ============ */
 

@@ -151,10 +151,16 @@ struct Fields {
     std::vector<double> Ux, Uy, Px, Py, T, rho, Pkin, Pvir, P, rhoTarget;
 };
 
+/* a zone: the cells [x0,x1) x [y0,y1) (zone_pass.cpp:191-219), listed in
+ * ascending cell id */
 struct Zone {
     int id = 0;
+    int x0 = 0, x1 = 0, y0 = 0, y1 = 0;
     std::vector<int> cells;
-    std::vector<char> mask;
+    bool in(int c, int Ny) const {
+        const int ix = c / Ny, iy = c - ix * Ny;
+        return ix >= x0 && ix < x1 && iy >= y0 && iy < y1;
+    }
 };
 
 struct Layout {
@@ -200,6 +206,9 @@ ZoneStats zone_kernel(const P& p, std::vector<double>& x, std::vector<double>& v
 template <class R>
 PassMetrics zone_pass_snapshot(const P& p, std::vector<double>& x, std::vector<double>& v, int mode, uint64_t seedBase,
                                std::vector<int32_t>* owner);
+template <class R>
+PassMetrics zone_pass_snapshot_full(const P& p, std::vector<double>& x, std::vector<double>& v, int mode,
+                                    uint64_t seedBase, std::vector<int32_t>* owner);
 template <class R>
 PassMetrics zone_pass_gauss_seidel(const P& p, std::vector<double>& x, std::vector<double>& v, int mode,
                                    uint64_t seedBase);

@@ -193,6 +193,26 @@ extern "C" int oracle_zone_pass(int mode, const mpcd_params* p, double* x, doubl
     OR_CATCH
 }
 
+/* the snapshot pass in the reference's structure (a private copy of the
+ * domain per zone, O(n) per zone): the check of the tile-local form */
+extern "C" int oracle_zone_pass_full(int mode, const mpcd_params* p, double* x, double* v, int layout,
+                                     uint64_t seedBase, mpcd_zone_pass_metrics* m) {
+    OR_TRY
+    auto xv = vec(x, 2 * (size_t)p->n), vv = vec(v, 2 * (size_t)p->n);
+    const PassMetrics pm = (mode == 0) ? zone_pass_snapshot_full<StdRngZone>(*p, xv, vv, layout, seedBase, nullptr)
+                                       : zone_pass_snapshot_full<CtrRngZone>(*p, xv, vv, layout, seedBase, nullptr);
+    put(xv, x);
+    put(vv, v);
+    if (m) {
+        m->nZones = pm.nZones;
+        m->nParticlesMovedDense = pm.movedDense;
+        m->nParticlesMovedSparse = pm.movedSparse;
+        m->corrRmsDU = pm.corrRmsDU;
+        m->corrMaxDU = pm.corrMaxDU;
+    }
+    OR_CATCH
+}
+
 /* grids: 6 arrays of Nc doubles (pdriveRaw, pdrive, duRepairX, duRepairY,
  * duEOSX, duEOSY), any may be NULL; fields: 4 structs (ref, red, repaired,
  * out), any may be NULL. */

@@ -114,6 +114,14 @@ class Oracle:
                                               _ptr(owner, ctypes.c_int32)))
         return struct_to_dict(m), owner
 
+    def zone_pass_full(self, mode: int, p: MpcdParams, x, v, layout: int, seed_base: int):
+        """The snapshot pass with a private copy of the domain per zone (the
+        reference's structure), in place; returns the pass metrics."""
+        m = PassMetrics()
+        self._check(self.lib.oracle_zone_pass_full(mode, ctypes.byref(p), _ptr(x), _ptr(v), layout,
+                                                   ctypes.c_uint64(seed_base), ctypes.byref(m)))
+        return struct_to_dict(m)
+
     def closure(self, p: MpcdParams, x_ref, v_ref, x_red, v_red):
         n2 = 2 * p.n
         xo, vo, vrep = np.zeros(n2), np.zeros(n2), np.zeros(n2)
