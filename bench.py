@@ -170,48 +170,128 @@ def cpu_port_full_size(p):
                       f"(oracle/, g++ -O2, one thread; O(n) per zone pass where the original is O(n^2))"}
 
 
-def learned_closure(sim, p, s0, args):
-    """Trains the closure surrogate on the GPU from samples of the running
-    plain simulation, then times the same step with the surrogate (plain
-    closure every K-th step and on a failed residual check)."""
+def collect_samples(p_dict, forcings, per_forcing, warm, seed=1):
+    """Closure samples (features, the plain closure's applied correction)
+    from plain runs of the workload's grid, one run per body force, every
+    second step after `warm` steps."""
+    feats, tgts = [], []
+    for gx in forcings:
+        prm = dict(p_dict, bodyForceX=float(gx))
+        ps = to_struct(prm)
+        x, v, _, _ = uniform_state(ps.n, ps.Lx, ps.Ly, seed=seed)
+        sim = gpu.Sim(ps)
+        sim.set_state(x, v)
+        del x, v
+        sim.step(1, warm)
+        sim.closure_sample(True, 1.0)
+        s = warm + 1
+        for _ in range(per_forcing):
+            sim.step(s, 2)
+            s += 2
+            f, t = sim.closure_sample(True, 1.0, fetch=True)
+            feats.append(f)
+            tgts.append(t)
+        sim.close()
+    return np.stack(feats), np.stack(tgts)
+
+
+def parabola_fit(prof, Ly):
+    """Least-squares parabola plus slip, U(y) = a + b y (Ly - y), at the row
+    centres."""
+    Ny = prof.size
+    y = (np.arange(Ny) + 0.5) * (Ly / Ny)
+    A = np.stack([np.ones(Ny), y * (Ly - y)], axis=1)
+    coef, *_ = np.linalg.lstsq(A, prof, rcond=None)
+    return A @ coef
+
+
+def poiseuille_profiles(p_dict, theta, out_scale, K, steps, runs):
+    """The time- and x-averaged Ux(y) of the second half of `steps` steps of
+    a channel, plain and with the surrogate (parameters theta), from the
+    same `runs` initial states; summed on the device after every step."""
     from src_incompressible_openmp_amd import surrogate
 
-    log(f"bench: closure surrogate: sampling {args.train_samples} plain closures")
-    sim.closure_sample(True, 1.0)
-    feats, tgts = [], []
-    s = s0
-    for _ in range(args.train_samples):
-        sim.step(s, 2)
-        s += 2
-        f, t = sim.closure_sample(True, 1.0, fetch=True)
-        feats.append(f)
-        tgts.append(t)
-    sim.closure_sample(False)
-    feats = np.stack(feats)
-    tgts = np.stack(tgts)
+    ps = to_struct(p_dict)
+    net = surrogate.Net(ps.Nx, ps.Ny, True, False, max_batch=1)
+    net.set_params(theta)
+    out = {"plain": [], "learned": [], "none": []}
+    counts = {"learned": 0, "plain": 0, "fallback": 0}
+    half = steps // 2
+    for r in range(runs):
+        x, v, _, _ = uniform_state(ps.n, ps.Lx, ps.Ly, seed=100 + r)
+        for mode in ("plain", "learned", "none"):
+            sim = gpu.Sim(ps)
+            sim.set_state(x, v)
+            if mode == "learned":
+                sim.set_surrogate(net, K, out_scale, 50.0 * out_scale)
+            elif mode == "none":  # no closure correction on the surrogate's steps: the method without the network
+                sim.set_surrogate(net, K, 1e-300, 1.0)
+                sim.set_surrogate_check(1e300, 1e300)
+            sim.step(1, half)
+            sim.row_profile(1)
+            sim.step(half + 1, steps - half)
+            ux, _, cnt, _ = sim.row_profile(0, fetch=True)
+            out[mode].append(ux / np.maximum(cnt, 1.0))
+            if mode == "learned":
+                c = sim.surrogate_counts()
+                for k in counts:
+                    counts[k] += c[k]
+            sim.close()
+    net.close()
+    plain = np.mean(out["plain"], axis=0)
+    learned = np.mean(out["learned"], axis=0)
+    none = np.mean(out["none"], axis=0)
+    fit = parabola_fit(plain, ps.Ly)
+    nrm = np.linalg.norm(fit)
+    return {"grid": [ps.Nx, ps.Ny], "particles": ps.n, "steps": steps, "window": [half + 1, steps], "runs": runs,
+            "profile_err_plain": float(np.linalg.norm(plain - fit) / nrm),
+            "profile_err_learned": float(np.linalg.norm(learned - fit) / nrm),
+            "profile_err_no_correction": float(np.linalg.norm(none - fit) / nrm),
+            "qx_plain": float(plain.mean() * ps.Ly), "qx_learned": float(learned.mean() * ps.Ly),
+            "qx_no_correction": float(none.mean() * ps.Ly),
+            "qx_rel_diff": float(abs(learned.mean() - plain.mean()) / max(abs(plain.mean()), 1e-300)),
+            "profile_diff_learned_vs_plain": float(np.linalg.norm(learned - plain) / np.linalg.norm(plain)),
+            "profile_diff_no_correction_vs_plain": float(np.linalg.norm(none - plain) / np.linalg.norm(plain)),
+            "surrogate_steps": counts, "ux_plain": plain.tolist(), "ux_learned": learned.tolist(),
+            "ux_no_correction": none.tolist()}
+
+
+def learned_closure(sim, p, prm, s0, args, plain_value):
+    """Trains the closure surrogate on the GPU from plain closures of the
+    workload's grid at several body forces (1e-4 .. 1e-2), then times the
+    workload with the surrogate (plain closure every K-th step and on a
+    failed residual check, decided on the device), and measures the
+    Poiseuille profile error of plain and surrogate runs on a 50x50
+    channel of the same physics (where the flow develops within the run)."""
+    from src_incompressible_openmp_amd import surrogate
+
+    forcings = (1e-4, 1e-3, 1e-2)
+    log(f"bench: closure surrogate: {args.train_samples} samples per body force {forcings}")
+    t0 = time.perf_counter()
+    feats, tgts = collect_samples(prm, forcings, args.train_samples, warm=20)
+    t_sample = time.perf_counter() - t0
     scale = float(np.std(tgts)) or 1.0
     tgts /= scale
-    net = surrogate.Net(p.Nx, p.Ny, True, False, max_batch=2)
     rng = np.random.default_rng(1)
     theta = []
     for co, ci in ((32, 4), (32, 32), (32, 32), (2, 32)):
         b = np.sqrt(6.0 / (9 * ci))
         theta += [rng.uniform(-b, b, co * 9 * ci), np.zeros(co)]
+    held = np.arange(len(feats)) % 8 == 7  # every 8th sample held out
+    net = surrogate.Net(p.Nx, p.Ny, True, False, max_batch=4)
     net.set_params(np.concatenate(theta))
-    ntr = max(1, len(feats) - 4)
-    log(f"bench: training {args.train_iters} Adam steps on {ntr} samples")
+    log(f"bench: training {args.train_iters} Adam steps on {int((~held).sum())} samples")
     t0 = time.perf_counter()
-    losses = net.train(feats[:ntr], tgts[:ntr], batch=2, iters=args.train_iters, lr=2e-3, seed=1)
+    losses = net.train(feats[~held], tgts[~held], batch=4, iters=args.train_iters, lr=2e-3, seed=1)
     gpu.device_synchronize()
     t_train = time.perf_counter() - t0
-    held = feats[ntr:]
-    r2 = None
-    if len(held):
-        pred = net.forward(held)
-        r2 = float(1.0 - np.mean((pred - tgts[ntr:]) ** 2) / np.mean(tgts[ntr:] ** 2))
+    pred = net.forward(feats[held])
+    r2 = float(1.0 - np.mean((pred - tgts[held]) ** 2) / np.mean(tgts[held] ** 2))
+    theta_trained = net.get_params()
+    del feats, tgts
     sim.set_surrogate(net, args.learned_K, scale, 50.0 * scale)
-    sim.step(s, args.warmup)
-    s += args.warmup
+    sim.step(s0, args.warmup)
+    s = s0 + args.warmup
     gpu.device_synchronize()
     log(f"bench: timing {args.steps} steps with the surrogate")
     c0 = sim.surrogate_counts()
@@ -224,11 +304,17 @@ def learned_closure(sim, p, s0, args):
     c1 = sim.surrogate_counts()
     sim.set_surrogate(None, 0, 1.0, 1.0)
     net.close()
-    return {"value": p.n * args.steps / el, "ms_per_step": 1e3 * el / args.steps, "K": args.learned_K,
-            "steps_learned": c1["learned"] - c0["learned"], "steps_plain": c1["plain"] - c0["plain"],
-            "steps_fallback": c1["fallback"] - c0["fallback"], "train_samples": int(ntr),
-            "train_iters": args.train_iters, "train_s": t_train, "train_loss_first": float(losses[0]),
-            "train_loss_last": float(losses[-1]), "heldout_r2": r2, "out_scale": scale}
+    value = p.n * args.steps / el
+    log("bench: Poiseuille profile error, plain vs surrogate, 50x50 channel")
+    prof = poiseuille_profiles(dict(prm, Nx=50, Ny=50, Nc=2500, n=50 * 50 * int(prm["gamma"]), Lx=10.0, Ly=10.0),
+                               theta_trained, scale, args.learned_K, args.profile_steps, args.profile_runs)
+    return {"value": value, "ms_per_step": 1e3 * el / args.steps, "speedup_vs_plain": value / plain_value,
+            "K": args.learned_K, "steps_learned": c1["learned"] - c0["learned"],
+            "steps_plain": c1["plain"] - c0["plain"], "steps_fallback": c1["fallback"] - c0["fallback"],
+            "train_forcings": list(forcings), "train_samples": int((~held).sum()), "heldout_samples": int(held.sum()),
+            "train_iters": args.train_iters, "sample_s": t_sample, "train_s": t_train,
+            "train_loss_first": float(np.mean(losses[:10])), "train_loss_last": float(np.mean(losses[-10:])),
+            "heldout_r2": r2, "out_scale": scale, "profile": prof}
 
 
 def main() -> int:
@@ -248,8 +334,10 @@ def main() -> int:
                     help="only copy 83,886,080 bytes device to device 5 times (the counter calibration of the "
                          "traffic figures) and exit")
     ap.add_argument("--learned-K", type=int, default=50, help="plain closure every K-th step with the surrogate")
-    ap.add_argument("--train-samples", type=int, default=24)
-    ap.add_argument("--train-iters", type=int, default=150)
+    ap.add_argument("--train-samples", type=int, default=40, help="closure samples per body force")
+    ap.add_argument("--train-iters", type=int, default=600)
+    ap.add_argument("--profile-steps", type=int, default=3000)
+    ap.add_argument("--profile-runs", type=int, default=8)
     args = ap.parse_args()
 
     if args.copy_calibration:
@@ -294,10 +382,10 @@ def main() -> int:
     s_next = 1 + args.warmup + args.steps
     m = sim.step(s_next, 1, metrics=True)
     s_next += 1
-    learned = None if args.no_learned else learned_closure(sim, p, s_next, args)
+    value = n * args.steps / elapsed
+    learned = None if args.no_learned else learned_closure(sim, p, prm, s_next, args, value)
     sim.close()
 
-    value = n * args.steps / elapsed
     ms_per_step = 1e3 * elapsed / args.steps
     out = {
         "metric": METRIC, "value": value, "unit": "particle-steps/s", "n_gpus": 1, "steps": args.steps,
@@ -328,7 +416,6 @@ def main() -> int:
             for k in ALG_BYTES if per[k]["launches"]}
     if learned:
         out["learned"] = learned
-        out["learned"]["speedup_vs_plain"] = learned["value"] / value
     out["last_step"] = {"moved_dense": m["base"]["nParticlesMovedDense"] + m["shifted"]["nParticlesMovedDense"],
                         "moved_sparse": m["base"]["nParticlesMovedSparse"] + m["shifted"]["nParticlesMovedSparse"],
                         "beta_applied": m["closure"]["betaRepairApplied"]}

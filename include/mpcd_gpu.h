@@ -313,13 +313,28 @@ int mpcd_gpu_net_train(mpcd_gpu_net* net, const float* feat, const float* target
                        int iters, double lr, uint64_t seed, double* losses);
 /* the resident loop with the surrogate: steps s with s % K != 0 take the
  * surrogate's correction (outScale * network output) instead of the plain
- * closure; a prediction beyond duCap (or not finite) falls back to the plain
- * closure for that step.  K <= 0 or net NULL: plain closure every step. */
+ * closure, unless the residual check fails: a prediction beyond duCap (or
+ * not finite), a red state whose occupancy spread exceeds occTol times the
+ * last plain step's, or a corrected velocity whose divergence residual
+ * exceeds divTol times the plain closure's (mpcd_gpu_sim_set_surrogate_check;
+ * defaults 1.1 and 2.0).  The check runs on the device and a failed step
+ * takes the plain closure without a host round trip.  K <= 0 or net NULL:
+ * plain closure every step. */
 int mpcd_gpu_sim_set_surrogate(mpcd_gpu_sim* s, mpcd_gpu_net* net, int K, double outScale, double duCap);
+int mpcd_gpu_sim_set_surrogate_check(mpcd_gpu_sim* s, double occTol, double divTol);
+/* row profiles of the state after each step, summed on the device over the
+ * steps since enabling (for the Poiseuille profile): enable 1 clears and
+ * starts, 0 stops, -1 leaves the state as it is; rows3 (nullable, 3 x Ny)
+ * receives per cell row the sums of vx, of vx^2 + vy^2 and of the particle
+ * count, steps (nullable) the number of steps summed.  Steps of one call
+ * stream on their own while enabled. */
+int mpcd_gpu_sim_row_profile(mpcd_gpu_sim* s, int enable, double* rows3, int32_t* steps);
 /* steps taken with the surrogate, with the plain closure, and fallbacks */
 int mpcd_gpu_sim_surrogate_counts(mpcd_gpu_sim* s, int32_t* counts3);
 /* training samples: with enable, every plain closure also stores its
- * features [cell][4] and its applied correction / outScale [cell][2];
+ * features [cell][4] (those of the step's reference state: the cells after
+ * the collision, before the zone passes -- the state the closure's EOS drive
+ * is computed from) and its applied correction / outScale [cell][2];
  * feat/target (nullable) receive the last stored sample */
 int mpcd_gpu_sim_closure_sample(mpcd_gpu_sim* s, int enable, double outScale, float* feat, float* target);
 /* one 16x16x32 bfloat16 matrix-core product of 64 lanes x 8 operands each

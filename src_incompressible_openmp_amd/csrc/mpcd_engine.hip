@@ -79,6 +79,11 @@ __global__ void k_iota(int* a, int n) {
     if (i < n) a[i] = i;
 }
 
+__global__ void k_fill_double(double* a, int n, double v) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) a[i] = v;
+}
+
 __global__ void k_fill_int(int* a, int n, int v) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) a[i] = v;
@@ -747,7 +752,8 @@ __global__ void k_sub_mean(double* __restrict__ vx, double* __restrict__ vy, int
  * for the comparison to read like the reference's. */
 constexpr int OCC_W = 6;
 /* slots of a simulation's diagnostics buffer */
-constexpr int DIAG_OCC_BASE = 0, DIAG_OCC_SHIFTED = 5, DIAG_PSUM = 10, DIAG_USUM = 13, DIAG_SLOTS = 16;
+constexpr int DIAG_OCC_BASE = 0, DIAG_OCC_SHIFTED = 5, DIAG_PSUM = 10, DIAG_USUM = 13, DIAG_SUR_OCC = 16,
+              DIAG_SUR_DIV = 21, DIAG_SLOTS = 24;
 __global__ void __launch_bounds__(TPB) k_occupancy(const int* __restrict__ cnt, int Nx, int Ny, ZoneParams zp,
                                                    double* __restrict__ part) {
     __shared__ double sh[OCC_W][TPB];
@@ -1165,9 +1171,15 @@ struct ClosureArgs {
     const double *redPx, *redPy, *redUx, *redUy;
     double *duRX, *duRY;
     double* part; /* per block: num, den */
+    const int* gate; /* nullptr, or the kernel runs only when *gate != 0 (the surrogate's fallback) */
 };
 
+/* the surrogate's steps launch the plain closure predicated on the residual
+ * check's flag: with the flag clear every block leaves at once */
+__device__ inline bool gated_off(const int* gate) { return gate && *gate == 0; }
+
 __global__ void __launch_bounds__(TPB) k_closure_repair(ClosureArgs a) {
+    if (gated_off(a.gate)) return;
     __shared__ double s0[TPB], s1[TPB];
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     double num = 0.0, den = 0.0;
@@ -1214,16 +1226,17 @@ __device__ inline void beta_from_sums(double* scal, int useOptimal, double betaR
 
 /* the repair sums and beta in one block (thread 0 wrote the sums) */
 __global__ void __launch_bounds__(TPB) k_reduce_beta(const double* __restrict__ part, int np, double* scal,
-                                                     int useOptimal, double betaRepair) {
+                                                     int useOptimal, double betaRepair, const int* gate = nullptr) {
+    if (gated_off(gate)) return;
     reduce_parts_block(part, np, 2, scal);
     if (threadIdx.x == 0) beta_from_sums(scal, useOptimal, betaRepair);
 }
 
 /* pdrive with the interzone cross smoothing (liquid_closure.cpp:401-419) */
 __global__ void k_pdrive(Geom g, const double* __restrict__ raw, const int* __restrict__ mask, double blend,
-                         int anyMask, double* __restrict__ pd) {
+                         int anyMask, double* __restrict__ pd, const int* gate = nullptr) {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= g.Nc) return;
+    if (c >= g.Nc || gated_off(gate)) return;
     double v = raw[c];
     if (anyMask && mask[c] != 0) {
         const int ix = c / g.Ny, iy = c - ix * g.Ny;
@@ -1248,9 +1261,9 @@ __global__ void k_pdrive(Geom g, const double* __restrict__ raw, const int* __re
 
 /* duEOS = -(dt/rhoT) * grad(pdrive) * betaEOS (liquid_closure.cpp:243-295, :424-430) */
 __global__ void k_eos(Geom g, const double* __restrict__ p, double dtOverRho, double betaEOS, double* __restrict__ ex,
-                      double* __restrict__ ey) {
+                      double* __restrict__ ey, const int* gate = nullptr) {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= g.Nc) return;
+    if (c >= g.Nc || gated_off(gate)) return;
     const int ix = c / g.Ny, iy = c - ix * g.Ny;
     const double dx = g.dxc, dy = g.dyc;
     double gx, gy;
@@ -1391,18 +1404,36 @@ __global__ void __launch_bounds__(TPB) k_kick_reorder_stream(
 /* per-cell closure metrics partials: sum hypot(beta*duR), sum pdriveRaw, sum pdrive */
 /* ---- closure surrogate ---- */
 
-/* the surrogate's inputs from the red state's cell fields: occupancy
- * deviation (N - gamma)/gamma, cell velocity, and T/kBT - 1 (0 where the
- * temperature is undefined, N < 2) */
-__global__ void k_features(int Nc, const int* __restrict__ N, const double* __restrict__ Ux,
+/* The surrogate's input features of each cell: occupancy deviation
+ * (N - gamma)/gamma, the cell velocity's deviation from the mean of its 3x3
+ * neighbourhood (periodic axes wrap, walls clip; the local mean flow is
+ * removed, so the features do not depend on the flow's speed, only on its
+ * local structure: a uniform or linearly sheared flow gives zero), and
+ * T/kBT - 1 (0 where the temperature is undefined, N < 2). */
+__global__ void k_features(Geom g, const int* __restrict__ N, const double* __restrict__ Ux,
                            const double* __restrict__ Uy, const double* __restrict__ T, double gamma, double kBT,
                            float* __restrict__ feat) {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= Nc) return;
+    if (c >= g.Nc) return;
+    const int ix = c / g.Ny, iy = c - ix * g.Ny;
+    double sx = 0.0, sy = 0.0;
+    int m = 0;
+    for (int dx = -1; dx <= 1; ++dx)
+        for (int dy = -1; dy <= 1; ++dy) {
+            int jx = ix + dx, jy = iy + dy;
+            if (g.perx) jx = (jx + g.Nx) % g.Nx;
+            else if (jx < 0 || jx >= g.Nx) continue;
+            if (g.pery) jy = (jy + g.Ny) % g.Ny;
+            else if (jy < 0 || jy >= g.Ny) continue;
+            const int q = jy + g.Ny * jx;
+            sx += Ux[q];
+            sy += Uy[q];
+            ++m;
+        }
     const double t = T[c];
     feat[4 * (size_t)c] = (float)(((double)N[c] - gamma) / gamma);
-    feat[4 * (size_t)c + 1] = (float)Ux[c];
-    feat[4 * (size_t)c + 2] = (float)Uy[c];
+    feat[4 * (size_t)c + 1] = (float)(Ux[c] - sx / (double)m);
+    feat[4 * (size_t)c + 2] = (float)(Uy[c] - sy / (double)m);
     feat[4 * (size_t)c + 3] = (N[c] >= 2 && isfinite(t)) ? (float)(t / kBT - 1.0) : 0.0f;
 }
 
@@ -1466,6 +1497,141 @@ __global__ void k_learned_center(int Nc, const double* __restrict__ sums, double
     if (c >= Nc) return;
     duEX[c] -= sums[0] * invN;
     duEY[c] -= sums[1] * invN;
+}
+
+/* Removes the predicted correction's momentum line by line along a
+ * periodic axis: with axis 0 (x periodic) each cell row iy gets
+ * du_x -= (sum over the row of N du_x) / (particles of the row), with axis
+ * 1 (y periodic) each column gets the same for du_y.  The plain closure's
+ * pressure kick, -grad P along a periodic axis, moves no momentum from one
+ * line to another on average; a learned correction constrained the same way
+ * cannot carry flow momentum across the channel (which would act as a
+ * spurious viscosity on the flow profile).  One block per line. */
+__global__ void __launch_bounds__(TPB) k_learned_center_lines(Geom g, int axis, const int* __restrict__ N,
+                                                              double* __restrict__ du) {
+    __shared__ double s0[TPB], s1[TPB];
+    const int line = blockIdx.x;
+    const int len = axis == 0 ? g.Nx : g.Ny;
+    auto cell = [&](int k) { return axis == 0 ? line + g.Ny * k : k + g.Ny * line; };
+    double a = 0.0, m = 0.0;
+    for (int k = threadIdx.x; k < len; k += TPB) {
+        const int c = cell(k);
+        a += (double)N[c] * du[c];
+        m += (double)N[c];
+    }
+    s0[threadIdx.x] = a;
+    s1[threadIdx.x] = m;
+    __syncthreads();
+    for (int st = TPB / 2; st > 0; st >>= 1) {
+        if (threadIdx.x < (unsigned)st) {
+            s0[threadIdx.x] += s0[threadIdx.x + st];
+            s1[threadIdx.x] += s1[threadIdx.x + st];
+        }
+        __syncthreads();
+    }
+    const double shift = s1[0] > 0.0 ? s0[0] / s1[0] : 0.0;
+    for (int k = threadIdx.x; k < len; k += TPB) du[cell(k)] -= shift;
+}
+
+/* Divergence residual of a closure's correction: per cell the central
+ * (one-sided at walls) divergence of the corrected cell velocity
+ * U + beta duRepair + duEOS; per-block partial sums of its square. */
+__global__ void __launch_bounds__(TPB) k_div_residual(Geom g, const double* __restrict__ Ux,
+                                                      const double* __restrict__ Uy, const double* __restrict__ duRX,
+                                                      const double* __restrict__ duRY, const double* __restrict__ duEX,
+                                                      const double* __restrict__ duEY, const double* __restrict__ scal,
+                                                      double* __restrict__ part) {
+    __shared__ double sh[TPB];
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    double d2 = 0.0;
+    if (c < g.Nc) {
+        const double beta = scal[3];
+        auto ux = [&](int q) { return Ux[q] + beta * duRX[q] + duEX[q]; };
+        auto uy = [&](int q) { return Uy[q] + beta * duRY[q] + duEY[q]; };
+        const int ix = c / g.Ny, iy = c - ix * g.Ny;
+        double gx, gy;
+        if (g.perx) {
+            const int a = iy + g.Ny * ((ix + 1) % g.Nx), b = iy + g.Ny * ((ix - 1 + g.Nx) % g.Nx);
+            gx = (ux(a) - ux(b)) / (2.0 * g.dxc);
+        } else if (g.Nx < 2) {
+            gx = 0.0;
+        } else {
+            const int a = iy + g.Ny * min(ix + 1, g.Nx - 1), b = iy + g.Ny * max(ix - 1, 0);
+            gx = (ux(a) - ux(b)) / ((double)(min(ix + 1, g.Nx - 1) - max(ix - 1, 0)) * g.dxc);
+        }
+        if (g.pery) {
+            const int a = (iy + 1) % g.Ny + g.Ny * ix, b = (iy - 1 + g.Ny) % g.Ny + g.Ny * ix;
+            gy = (uy(a) - uy(b)) / (2.0 * g.dyc);
+        } else if (g.Ny < 2) {
+            gy = 0.0;
+        } else {
+            const int a = min(iy + 1, g.Ny - 1) + g.Ny * ix, b = max(iy - 1, 0) + g.Ny * ix;
+            gy = (uy(a) - uy(b)) / ((double)(min(iy + 1, g.Ny - 1) - max(iy - 1, 0)) * g.dyc);
+        }
+        d2 = (gx + gy) * (gx + gy);
+    }
+    sh[threadIdx.x] = d2;
+    __syncthreads();
+    for (int st = TPB / 2; st > 0; st >>= 1) {
+        if (threadIdx.x < (unsigned)st) sh[threadIdx.x] += sh[threadIdx.x + st];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
+}
+
+/* slots of the surrogate's device state */
+enum { SUR_OCC_REF = 0, SUR_DIV_REF = 1, SUR_HAVE_REF = 2, SUR_OCC = 3, SUR_DIV = 4, SUR_N_LEARNED = 5,
+       SUR_N_FALLBACK = 6, SUR_SLOTS = 8 };
+
+/* The residual check of a surrogate step, on the device (one thread).
+ * res[0] = occStd of the red state's counts, res[1] = sum of the squared
+ * divergence of its corrected velocity.  A plain step (plain = 1) records
+ * them as the reference; a learned step fails when its prediction was not
+ * finite or beyond the cap (flag already 1), when the red state's
+ * occupancy spread exceeds occTol times the reference's, or when the
+ * divergence residual of the predicted correction exceeds divTol times the
+ * plain closure's; the flag then sends the step to the plain closure (its
+ * kernels run predicated on it). */
+__global__ void k_sur_decide(const double* __restrict__ occ, const double* __restrict__ div, double* __restrict__ sur,
+                             int* __restrict__ flag, int plain, double occTol, double divTol) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    const double o = occ[0], d = sqrt(div[0]);
+    sur[SUR_OCC] = o;
+    sur[SUR_DIV] = d;
+    if (plain) {
+        sur[SUR_OCC_REF] = o;
+        sur[SUR_DIV_REF] = d;
+        sur[SUR_HAVE_REF] = 1.0;
+        return;
+    }
+    bool bad = *flag != 0;
+    if (sur[SUR_HAVE_REF] != 0.0) {
+        if (o > occTol * sur[SUR_OCC_REF]) bad = true;
+        if (d > divTol * sur[SUR_DIV_REF]) bad = true;
+    }
+    *flag = bad ? 1 : 0;
+    sur[bad ? SUR_N_FALLBACK : SUR_N_LEARNED] += 1.0;
+}
+
+/* Row profiles accumulated over steps: sum of vx, vx^2 + vy^2 and the
+ * particle count of each cell row (cell iy of the particle's y).  One block
+ * strides over the particles with the rows in LDS. */
+__global__ void __launch_bounds__(TPB) k_row_profile(Geom g, const double* __restrict__ y, const double* __restrict__ vx,
+                                                     const double* __restrict__ vy, double* __restrict__ acc) {
+    extern __shared__ double rows[]; /* 3 * Ny */
+    for (int r = threadIdx.x; r < 3 * g.Ny; r += TPB) rows[r] = 0.0;
+    __syncthreads();
+    for (int i = blockIdx.x * TPB + (int)threadIdx.x; i < g.n; i += gridDim.x * TPB) {
+        int iy = (int)floor(y[i] / g.dyc);
+        iy = iy < 0 ? 0 : (iy >= g.Ny ? g.Ny - 1 : iy);
+        const double a = vx[i], b = vy[i];
+        atomicAdd(&rows[iy], a);
+        atomicAdd(&rows[g.Ny + iy], a * a + b * b);
+        atomicAdd(&rows[2 * g.Ny + iy], 1.0);
+    }
+    __syncthreads();
+    for (int r = threadIdx.x; r < 3 * g.Ny; r += TPB)
+        if (rows[r] != 0.0) atomicAdd(&acc[r], rows[r]);
 }
 
 __global__ void __launch_bounds__(TPB) k_closure_cell_metrics(int Nc, const double* __restrict__ duRX,
@@ -1551,9 +1717,15 @@ struct mpcd_gpu_sim {
     bool wantSample = false;
     int* redN = nullptr;
     double* redT = nullptr;
+    int* refN = nullptr;    /* the reference state's counts and temperatures (the surrogate's features) */
+    double* refT = nullptr;
     float *featBuf = nullptr, *tgtBuf = nullptr;
     int* surFlag = nullptr;
-    int nLearned = 0, nPlain = 0, nFallback = 0;
+    double* surDev = nullptr; /* SUR_* slots */
+    double surOccTol = 1.1, surDivTol = 2.0;
+    int nPlain = 0;
+    double* rowAcc = nullptr; /* 3 x Ny row profiles, accumulated while rowSteps >= 0 */
+    int rowSteps = -1;
     int* err = nullptr;
     /* diagnostics: device results of the last diagnosed step, and the cell
      * fields of the current state (allocated on first use) */
@@ -1700,9 +1872,13 @@ struct mpcd_gpu_sim {
         mask = dalloc<int>(Nc);
         redN = dalloc<int>(Nc);
         redT = dalloc<double>(Nc);
+        refN = dalloc<int>(Nc);
+        refT = dalloc<double>(Nc);
         featBuf = dalloc<float>(4 * (size_t)Nc);
         tgtBuf = dalloc<float>(2 * (size_t)Nc);
         surFlag = dalloc<int>(1);
+        surDev = dalloc<double>(SUR_SLOTS);
+        HIP_OK(hipMemset(surDev, 0, SUR_SLOTS * sizeof(double)));
         scal = dalloc<double>(64);
         diagDev = dalloc<double>(DIAG_SLOTS);
         err = dalloc<int>(1);
@@ -1781,7 +1957,7 @@ struct mpcd_gpu_sim {
                         tmpg, zsize, zoff, owner[0], owner[1], cover[0], cover[1], overCells[0], overCells[1], stage,
                         refPx, refPy, refUx, refUy, pdriveRaw, redPx, redPy, redUx, redUy, duRX, duRY, duEX, duEY,
                         pdrive, cellTmp, mask, part, scal, err, wallPart, rects[0], rects[1], zout, zscratch, duG, recG, zstate,
-                        redN, redT, featBuf, tgtBuf, surFlag, diagDev, outN, outF};
+                        redN, redT, featBuf, tgtBuf, surFlag, diagDev, outN, outF, surDev, rowAcc, refN, refT};
         for (void* q : ptrs)
             if (q) (void)hipFree(q);
         for (hipEvent_t e : evPool) (void)hipEventDestroy(e);
@@ -1980,6 +2156,10 @@ struct mpcd_gpu_sim {
         f.Px = refPx;
         f.Py = refPy;
         f.P = pdriveRaw; /* P = Pkin + Pvir = pdriveRaw (liquid_closure.cpp:401-405) */
+        if (sur || wantSample) { /* the surrogate's features are the reference state's */
+            f.N = refN;
+            f.T = refT;
+        }
         hipLaunchKernelGGL(k_cell_fields, dim3(nblk(Nc, CELL_BLK)), dim3(CELL_BLK), cellCap * 2 * sizeof(double), stream,
                            bin.start, bin.list, Nc, vx, vy, vc(), p.Kvirial, rho_target(), f, cellCap);
         check_launch();
@@ -2188,51 +2368,7 @@ struct mpcd_gpu_sim {
         check_launch();
     }
 
-    void closure_learned() {
-        FieldsOut f{};
-        f.N = redN;
-        f.Ux = redUx;
-        f.Uy = redUy;
-        f.T = redT;
-        hipLaunchKernelGGL(k_cell_fields, dim3(nblk(Nc, CELL_BLK)), dim3(CELL_BLK), cellCap * 2 * sizeof(double), stream,
-                           bin.start, bin.list, Nc, vx, vy, vc(), p.Kvirial, rho_target(), f, cellCap);
-        const int nbc = (int)nblk(Nc);
-        hipLaunchKernelGGL(k_features, dim3(nbc), dim3(TPB), 0, stream, Nc, redN, redUx, redUy, redT, p.gamma, p.kBT,
-                           featBuf);
-        const float* pred = net_forward_device(sur, featBuf, stream);
-        hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, stream, surFlag, 1, 0);
-        hipLaunchKernelGGL(k_learned_du, dim3(nbc), dim3(TPB), 0, stream, Nc, pred, redN, surScale, surCap, duRX, duRY,
-                           duEX, duEY, scal, surFlag, part);
-        hipLaunchKernelGGL(k_reduce_parts, dim3(1), dim3(TPB), 0, stream, part, nbc, 2, scal + 8);
-        hipLaunchKernelGGL(k_learned_center, dim3(nbc), dim3(TPB), 0, stream, Nc, scal + 8, 1.0 / (double)n, duEX,
-                           duEY);
-        check_launch();
-        int bad = 0;
-        HIP_OK(hipMemcpyAsync(&bad, surFlag, sizeof(int), hipMemcpyDeviceToHost, stream));
-        HIP_OK(hipStreamSynchronize(stream));
-        if (bad) {
-            ++nFallback;
-            closure(nullptr);
-            return;
-        }
-        ++nLearned;
-        kick_reorder(nullptr);
-    }
-
-    /* closure: ref sums already in ref*; red binning current */
-    void closure(mpcd_closure_metrics* cm) {
-        if (!p.useLiquidClosure) throw ApiError(MPCD_E_INVALID, "run_liquid_closure requires useLiquidClosure=true");
-        FieldsOut f{};
-        f.Ux = redUx;
-        f.Uy = redUy;
-        f.Px = redPx;
-        f.Py = redPy;
-        if (wantSample) {
-            f.N = redN;
-            f.T = redT;
-        }
-        hipLaunchKernelGGL(k_cell_fields, dim3(nblk(Nc, CELL_BLK)), dim3(CELL_BLK), cellCap * 2 * sizeof(double), stream,
-                           bin.start, bin.list, Nc, vx, vy, vc(), p.Kvirial, rho_target(), f, cellCap);
+    ClosureArgs closure_args() {
         ClosureArgs ca{};
         ca.Nc = Nc;
         ca.gamma = p.gamma;
@@ -2247,16 +2383,93 @@ struct mpcd_gpu_sim {
         ca.duRX = duRX;
         ca.duRY = duRY;
         ca.part = part;
+        return ca;
+    }
+
+    /* the plain closure's grids (repair, beta, pressure drive, EOS
+     * gradient) from the reference and red fields; with a gate, predicated
+     * on it (the kernels leave at once when it is 0) */
+    void closure_grids(const int* gate) {
+        ClosureArgs ca = closure_args();
+        ca.gate = gate;
         const int nbc = (int)nblk(Nc);
         hipLaunchKernelGGL(k_closure_repair, dim3(nbc), dim3(TPB), 0, stream, ca);
         hipLaunchKernelGGL(k_reduce_beta, dim3(1), dim3(TPB), 0, stream, part, nbc, scal, p.useOptimalBetaRepair,
-                           p.betaRepair);
+                           p.betaRepair, gate);
         hipLaunchKernelGGL(k_pdrive, dim3(nbc), dim3(TPB), 0, stream, g, pdriveRaw, mask, p.smoothPdriveInterzoneBlend,
-                           anyMask, pdrive);
+                           anyMask, pdrive, gate);
         const double dtr = p.dt / rho_target();
-        hipLaunchKernelGGL(k_eos, dim3(nbc), dim3(TPB), 0, stream, g, pdrive, dtr, p.betaEOS, duEX, duEY);
+        hipLaunchKernelGGL(k_eos, dim3(nbc), dim3(TPB), 0, stream, g, pdrive, dtr, p.betaEOS, duEX, duEY, gate);
+        check_launch();
+    }
+
+    /* the residual check's inputs for the grids in place (occupancy of the
+     * red counts, divergence of the corrected red velocity) and the
+     * decision on the device (plain = 1: the plain closure's values become
+     * the reference) */
+    void sur_residual(int plain) {
+        occupancy(redN, DIAG_SUR_OCC);
+        const int nbc = (int)nblk(Nc);
+        hipLaunchKernelGGL(k_div_residual, dim3(nbc), dim3(TPB), 0, stream, g, redUx, redUy, duRX, duRY, duEX, duEY,
+                           scal, part);
+        hipLaunchKernelGGL(k_reduce_parts, dim3(1), dim3(TPB), 0, stream, part, nbc, 1, diagDev + DIAG_SUR_DIV);
+        hipLaunchKernelGGL(k_sur_decide, dim3(1), dim3(64), 0, stream, diagDev + DIAG_SUR_OCC, diagDev + DIAG_SUR_DIV,
+                           surDev, surFlag, plain, surOccTol, surDivTol);
+        check_launch();
+    }
+
+    /* A surrogate step: red fields -> features -> network -> the kick's
+     * grids, then the residual check on the device; the plain closure's
+     * kernels are launched predicated on its flag, so a failed check makes
+     * this step a plain one with no host round trip. */
+    void closure_learned() {
+        FieldsOut f{};
+        f.N = redN;
+        f.Ux = redUx;
+        f.Uy = redUy;
+        f.Px = redPx;
+        f.Py = redPy;
+        f.T = redT;
+        hipLaunchKernelGGL(k_cell_fields, dim3(nblk(Nc, CELL_BLK)), dim3(CELL_BLK), cellCap * 2 * sizeof(double), stream,
+                           bin.start, bin.list, Nc, vx, vy, vc(), p.Kvirial, rho_target(), f, cellCap);
+        const int nbc = (int)nblk(Nc);
+        hipLaunchKernelGGL(k_features, dim3(nbc), dim3(TPB), 0, stream, g, refN, refUx, refUy, refT, p.gamma, p.kBT,
+                           featBuf);
+        const float* pred = net_forward_device(sur, featBuf, stream);
+        hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, stream, surFlag, 1, 0);
+        hipLaunchKernelGGL(k_learned_du, dim3(nbc), dim3(TPB), 0, stream, Nc, pred, redN, surScale, surCap, duRX, duRY,
+                           duEX, duEY, scal, surFlag, part);
+        hipLaunchKernelGGL(k_reduce_parts, dim3(1), dim3(TPB), 0, stream, part, nbc, 2, scal + 8);
+        hipLaunchKernelGGL(k_learned_center, dim3(nbc), dim3(TPB), 0, stream, Nc, scal + 8, 1.0 / (double)n, duEX,
+                           duEY);
+        /* along a periodic axis the momentum is removed line by line */
+        if (g.perx) hipLaunchKernelGGL(k_learned_center_lines, dim3(p.Ny), dim3(TPB), 0, stream, g, 0, redN, duEX);
+        if (g.pery) hipLaunchKernelGGL(k_learned_center_lines, dim3(p.Nx), dim3(TPB), 0, stream, g, 1, redN, duEY);
+        check_launch();
+        sur_residual(0);
+        closure_grids(surFlag);
+        kick_reorder(nullptr);
+    }
+
+    /* closure: ref sums already in ref*; red binning current */
+    void closure(mpcd_closure_metrics* cm) {
+        if (!p.useLiquidClosure) throw ApiError(MPCD_E_INVALID, "run_liquid_closure requires useLiquidClosure=true");
+        FieldsOut f{};
+        f.Ux = redUx;
+        f.Uy = redUy;
+        f.Px = redPx;
+        f.Py = redPy;
+        if (wantSample || sur) {
+            f.N = redN;
+            f.T = redT;
+        }
+        hipLaunchKernelGGL(k_cell_fields, dim3(nblk(Nc, CELL_BLK)), dim3(CELL_BLK), cellCap * 2 * sizeof(double), stream,
+                           bin.start, bin.list, Nc, vx, vy, vc(), p.Kvirial, rho_target(), f, cellCap);
+        closure_grids(nullptr);
+        const int nbc = (int)nblk(Nc);
+        if (sur) sur_residual(1); /* the plain closure's residuals become the surrogate's reference */
         if (wantSample) {
-            hipLaunchKernelGGL(k_features, dim3(nbc), dim3(TPB), 0, stream, Nc, redN, redUx, redUy, redT, p.gamma, p.kBT,
+            hipLaunchKernelGGL(k_features, dim3(nbc), dim3(TPB), 0, stream, g, refN, refUx, refUy, refT, p.gamma, p.kBT,
                                featBuf);
             hipLaunchKernelGGL(k_closure_target, dim3(nbc), dim3(TPB), 0, stream, Nc, scal, duRX, duRY, duEX, duEY,
                                1.0 / surScale, tgtBuf);
@@ -2427,7 +2640,7 @@ struct mpcd_gpu_sim {
         if (wantDiag) occupancy(bin.cnt, DIAG_OCC_SHIFTED);
         if (stages && stages[4]) get_state(stages[4], stages[5]);
         pbeg(MPCD_PROF_CLOSURE);
-        fuseNext = (stages || sm) ? -1 : next;
+        fuseNext = (stages || sm || rowSteps >= 0) ? -1 : next;
         if (sur && surK > 0 && s % surK != 0) {
             closure_learned();
         } else {
@@ -2438,6 +2651,12 @@ struct mpcd_gpu_sim {
         pend(MPCD_PROF_CLOSURE);
         phase_mark(4);
         if (wantDiag) state_diag();
+        if (rowSteps >= 0) { /* row profiles of the state after the step */
+            hipLaunchKernelGGL(k_row_profile, dim3(std::min<unsigned>(nblk(n), 1024u)), dim3(TPB),
+                               3 * (size_t)p.Ny * sizeof(double), stream, g, y, vx, vy, rowAcc);
+            check_launch();
+            ++rowSteps;
+        }
     }
 };
 
@@ -2487,11 +2706,6 @@ unsigned char* noalloc_scratch() {
 __global__ void k_cells_xy(Geom g, const double* __restrict__ xy, int* __restrict__ cid) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < g.n) cid[i] = cell_of(g, xy[2 * (size_t)i], xy[2 * (size_t)i + 1]);
-}
-
-__global__ void k_fill_double(double* a, int n, double v) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) a[i] = v;
 }
 
 }  // namespace
@@ -2936,12 +3150,51 @@ int mpcd_gpu_sim_set_surrogate(mpcd_gpu_sim* s, mpcd_gpu_net* net, int K, double
     });
 }
 
+int mpcd_gpu_sim_set_surrogate_check(mpcd_gpu_sim* s, double occTol, double divTol) {
+    return guarded([&] {
+        if (!s) throw ApiError(MPCD_E_INVALID, "null simulation");
+        if (!(occTol > 0.0) || !(divTol > 0.0)) throw ApiError(MPCD_E_INVALID, "tolerances must be > 0");
+        s->surOccTol = occTol;
+        s->surDivTol = divTol;
+    });
+}
+
+int mpcd_gpu_sim_row_profile(mpcd_gpu_sim* s, int enable, double* rows3, int32_t* steps) {
+    return guarded([&] {
+        if (!s) throw ApiError(MPCD_E_INVALID, "null simulation");
+        if (3 * (size_t)s->p.Ny * sizeof(double) > 60 * 1024)
+            throw ApiError(MPCD_E_INVALID, "row profiles support at most 2560 rows");
+        if (rows3 || steps) {
+            HIP_OK(hipStreamSynchronize(s->stream));
+            if (rows3) {
+                if (s->rowAcc)
+                    HIP_OK(hipMemcpy(rows3, s->rowAcc, 3 * (size_t)s->p.Ny * sizeof(double), hipMemcpyDeviceToHost));
+                else
+                    std::fill(rows3, rows3 + 3 * (size_t)s->p.Ny, 0.0);
+            }
+            if (steps) *steps = s->rowSteps < 0 ? 0 : s->rowSteps;
+        }
+        if (enable > 0) {
+            if (!s->rowAcc) s->rowAcc = dalloc<double>(3 * (size_t)s->p.Ny);
+            hipLaunchKernelGGL(k_fill_double, dim3(nblk(3 * s->p.Ny)), dim3(TPB), 0, s->stream, s->rowAcc, 3 * s->p.Ny,
+                               0.0);
+            HIP_OK(hipGetLastError());
+            s->rowSteps = 0;
+        } else if (enable == 0) {
+            s->rowSteps = -1;
+        }
+    });
+}
+
 int mpcd_gpu_sim_surrogate_counts(mpcd_gpu_sim* s, int32_t* counts3) {
     return guarded([&] {
         if (!s || !counts3) throw ApiError(MPCD_E_INVALID, "null argument");
-        counts3[0] = s->nLearned;
+        double h[SUR_SLOTS];
+        HIP_OK(hipMemcpyAsync(h, s->surDev, sizeof(h), hipMemcpyDeviceToHost, s->stream));
+        HIP_OK(hipStreamSynchronize(s->stream));
+        counts3[0] = (int32_t)h[SUR_N_LEARNED];
         counts3[1] = s->nPlain;
-        counts3[2] = s->nFallback;
+        counts3[2] = (int32_t)h[SUR_N_FALLBACK];
     });
 }
 

@@ -302,6 +302,24 @@ class Sim:
         _check(load().mpcd_gpu_sim_set_surrogate(self._h, h, int(K), ctypes.c_double(out_scale),
                                                  ctypes.c_double(du_cap)))
 
+    def set_surrogate_check(self, occ_tol: float, div_tol: float) -> None:
+        """Tolerances of the surrogate's residual check (occupancy spread
+        and divergence residual relative to the last plain step)."""
+        _check(load().mpcd_gpu_sim_set_surrogate_check(self._h, ctypes.c_double(occ_tol), ctypes.c_double(div_tol)))
+
+    def row_profile(self, enable: int = -1, fetch: bool = False):
+        """Row profiles summed on the device after each step: enable 1 clears
+        and starts, 0 stops, -1 keeps; with fetch returns (sums of vx, sums of
+        |v|^2, particle counts) per cell row and the number of steps."""
+        rows = np.zeros(3 * self.p.Ny) if fetch else None
+        st = ctypes.c_int32(0)
+        _check(load().mpcd_gpu_sim_row_profile(self._h, int(enable), _ptr(rows) if fetch else None,
+                                               ctypes.byref(st) if fetch else None))
+        if not fetch:
+            return None
+        Ny = self.p.Ny
+        return rows[:Ny], rows[Ny:2 * Ny], rows[2 * Ny:], int(st.value)
+
     def surrogate_counts(self):
         c = np.zeros(3, np.int32)
         _check(load().mpcd_gpu_sim_surrogate_counts(self._h, _ptr(c, ctypes.c_int32)))

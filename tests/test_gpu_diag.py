@@ -160,3 +160,41 @@ def test_driver_matches_oracle_rows(gpu, oracle, tmp_path):
     for k in ("finalOccStd", "finalQx", "finalMeanKinetic", "timePerStepMean", "fracShiftedMerge"):
         assert k in kv, k
     assert abs(float(kv["finalMeanKinetic"]) - 0.5 * np.sum(v * v) / p.n) <= 5e-6
+
+
+def test_row_profiles(gpu):
+    """Row profiles summed on the device after each step equal the sums of
+    the states read back (float atomics: to 1e-9)."""
+    p = to_struct(case_params("small"))
+    x, v, _, _ = case_state("small")
+    sim = gpu.Sim(p)
+    try:
+        sim.set_state(x, v)
+        sim.row_profile(1)
+        want = np.zeros((3, p.Ny))
+        for s in range(1, 5):
+            sim.step(s)
+            xs, vs = sim.get_state()
+            iy = np.clip(np.floor(xs[1::2] / (p.Ly / p.Ny)).astype(np.int64), 0, p.Ny - 1)
+            want[0] += np.bincount(iy, weights=vs[0::2], minlength=p.Ny)
+            want[1] += np.bincount(iy, weights=vs[0::2] ** 2 + vs[1::2] ** 2, minlength=p.Ny)
+            want[2] += np.bincount(iy, minlength=p.Ny)
+        sim.step(5, 3)  # several steps in one call while enabled
+        ux, v2, cnt, steps = sim.row_profile(0, fetch=True)
+        assert steps == 7
+        sim2 = gpu.Sim(p)
+        sim2.set_state(x, v)
+        sim2.step(1, 4)
+        for s in (5, 6, 7):
+            sim2.step(s)
+            xs, vs = sim2.get_state()
+            iy = np.clip(np.floor(xs[1::2] / (p.Ly / p.Ny)).astype(np.int64), 0, p.Ny - 1)
+            want[0] += np.bincount(iy, weights=vs[0::2], minlength=p.Ny)
+            want[1] += np.bincount(iy, weights=vs[0::2] ** 2 + vs[1::2] ** 2, minlength=p.Ny)
+            want[2] += np.bincount(iy, minlength=p.Ny)
+        sim2.close()
+        assert np.array_equal(cnt, want[2])
+        assert np.max(np.abs(ux - want[0])) <= 1e-9 * max(1.0, np.max(np.abs(want[0])))
+        assert np.max(np.abs(v2 - want[1])) <= 1e-9 * np.max(want[1])
+    finally:
+        sim.close()

@@ -198,3 +198,41 @@ def test_surrogate_learns_the_closure_and_steps(S):
     assert abs(ke_l - ke_r) <= 0.05 * ke_r
     # mean flow: the two runs differ by thermal noise of the wall exchanges
     assert abs(np.mean(vl[0::2]) - np.mean(vr[0::2])) <= 0.03
+
+
+def test_failed_residual_checks_give_the_plain_run(S):
+    """The residual check decides on the device and a failed step runs the
+    plain closure's kernels (predicated on the check's flag) on the same
+    fields: a surrogate whose every prediction fails (beyond the cap, or
+    tolerances too tight to pass) gives the plain run bit for bit, with every
+    learned step counted as a fallback."""
+    from src_incompressible_openmp_amd import gpu
+    from src_incompressible_openmp_amd.fileio import uniform_state
+
+    p = closure_run_params()
+    x, v, _, _ = uniform_state(p.n, p.Lx, p.Ly, seed=5)
+    ref = gpu.Sim(p)
+    ref.set_state(x, v)
+    ref.step(5, 30)  # steps 5..34: the first is a plain one, so the check has its reference
+    xr, vr = ref.get_state()
+    net = S.Net(p.Nx, p.Ny, True, False, max_batch=1)
+    net.set_params(R.init_params(2))
+    for cap, occ_tol, div_tol in ((1e-30, 10.0, 10.0), (1e3, 1e-9, 1e-9)):
+        lrn = gpu.Sim(p)
+        lrn.set_state(x, v)
+        lrn.set_surrogate(net, 5, 0.01, cap)
+        lrn.set_surrogate_check(occ_tol, div_tol)
+        lrn.step(5, 30)
+        xl, vl = lrn.get_state()
+        c = lrn.surrogate_counts()
+        assert c == {"learned": 0, "plain": 6, "fallback": 24}, c
+        assert xl.tobytes() == xr.tobytes() and vl.tobytes() == vr.tobytes()
+        lrn.close()
+    # loose tolerances: a small, finite prediction passes
+    lrn = gpu.Sim(p)
+    lrn.set_state(x, v)
+    lrn.set_surrogate(net, 5, 1e-4, 1e3)
+    lrn.set_surrogate_check(1e3, 1e3)
+    lrn.step(5, 30)
+    c = lrn.surrogate_counts()
+    assert c["learned"] == 24 and c["fallback"] == 0, c
