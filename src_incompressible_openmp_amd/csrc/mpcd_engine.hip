@@ -1135,13 +1135,45 @@ __global__ void __launch_bounds__(FUSED_TPB, 4) k_zone_fused(ZoneArgs a) { /* 4 
         if (threadIdx.x == 0) atomicOr(a.err, 2);
         return;
     }
-    if (threadIdx.x < ZONE_TPB) {
-        /* the passes are the critical path: their wavefront issues ahead of
-         * the corrections of zones that are done */
-        __builtin_amdgcn_s_setprio(3);
-        zone_pass_body<IT, true>(a, zi, off, (int)threadIdx.x, lds, &sh);
-        __builtin_amdgcn_s_setprio(0);
+    /* the passes are the critical path: their wavefronts issue ahead of the
+     * corrections of zones that are done */
+    __builtin_amdgcn_s_setprio(3);
+    const int wave = (int)threadIdx.x >> 6, lane = (int)threadIdx.x & 63;
+    ZoneCtxT<IT> z;
+    const uint64_t seed = a.singleSeed ? a.seedBase : a.seedBase + 104729ULL * (uint64_t)(zi + 1);
+    zone_init(z, a.in, a.rects[zi], zi + 1, seed, &sh);
+    z.nxG = (IT*)a.nextG + off;
+    z.tmpG = (IT*)a.tmpG + off;
+    z.duG = a.duG + (size_t)zi * (size_t)a.duStride;
+    z.rec = a.rec + off;
+    zone_carve(z, lds, a.ldsBytes);
+    if (wave == 0) {
+        zone_setup(z, a.in, lane, ZONE_TPB);
+        ZBAR();
     }
+    __syncthreads();
+    z.P = sh.P;
+    z.allOwned = sh.allOwned;
+    for (int k = (int)threadIdx.x; k < z.P; k += FUSED_TPB) z.rec[k] = (unsigned char)ZM_NONE;
+    __syncthreads();
+    /* the dense passes on all four wavefronts, the sparse phase on one */
+    zone_dense_passes_mw(z, wave, lane);
+    if (wave == 0) {
+        zone_sparse_pass(z, lane, ZONE_TPB);
+        ZBAR();
+        zone_dump(z, zone_state_at<IT>(a.zstate + (size_t)zi * a.zsStride, z.C, z.zw), lane, ZONE_TPB);
+        if (lane == 0) {
+            ZoneOut o;
+            o.nHigh = sh.nHigh;
+            o.nLow = sh.nLow;
+            o.movedDense = sh.movedDense;
+            o.movedSparse = sh.movedSparse;
+            o.stats4 = 0.0;
+            o.stats5 = 0.0;
+            a.out[zi] = o;
+        }
+    }
+    __builtin_amdgcn_s_setprio(0);
     __syncthreads();
     zone_finish_body<IT, true, FUSED_TPB, 3>(a, zi, off, (int)threadIdx.x, lds, a.ldsBytes, queue);
 }

@@ -201,6 +201,7 @@ struct ZoneIn {
 /* Scalars the lanes share (LDS on the device). */
 struct ZoneShared {
     int nOrd, startBin, nr, stage, P, allOwned;
+    int mwCount, mwLevels; /* the multi-wavefront dense phase's list length and level count */
     int nHigh, nLow, movedDense, movedSparse;
     double stats4, stats5;
 };
@@ -1308,6 +1309,148 @@ ZHD void zone_dense_passes(Z& z, int lane, int nl) {
     ZBAR();
 #endif
 }
+
+#if defined(__HIP_DEVICE_COMPILE__)
+/* The dense passes on the four wavefronts of a 256-thread workgroup
+ * (device only).  A dense cell's spread reads counts within Chebyshev range 2
+ * of it and writes the lists, counts and flags of itself and of targets
+ * within range 2; its own count changes only by itself (a dense cell is above
+ * its high threshold, a target below it).  Cells of the shuffled list more
+ * than 4 cells apart therefore touch disjoint cells and commute.  Each cell
+ * gets a level, one more than the highest level of the earlier cells within
+ * range 4; the cells of one level run on the four wavefronts at once, level
+ * after level, so every conflicting pair keeps the list order and the
+ * result is the serial loop's bit for bit.  A cell whose movers exceed a
+ * wavefront takes the one-lane form, which uses the zone's shared scratch:
+ * it gets a level of its own, above every earlier cell and below every later
+ * one, and wavefront 0 runs it.  Waves 1-3 stage their movers in the one-lane
+ * form's candidate scratch, free while they run. */
+constexpr int ZONE_WAVES = 4;
+static_assert(24 * sizeof(Cand) >= (ZONE_WAVES - 1) * ZONE_MAX_MOVE * sizeof(int),
+              "the candidate scratch holds the other wavefronts' movers");
+
+template <class Z>
+ZHD bool dense_conflict(const Z& z, int a, int b) {
+    int dx = a / z.zh - b / z.zh, dy = a % z.zh - b % z.zh;
+    dx = dx < 0 ? -dx : dx;
+    dy = dy < 0 ? -dy : dy;
+    /* a zone spanning a periodic axis reaches around it */
+    if (z.g.perx && z.zw == z.g.Nx) dx = imin(dx, z.zw - dx);
+    if (z.g.pery && z.zh == z.g.Ny) dy = imin(dy, z.zh - dy);
+    return dx <= 4 && dy <= 4;
+}
+
+template <class Z>
+__device__ __forceinline__ void zone_dense_passes_mw(Z& z, int wave, int lane) {
+    typedef typename Z::idx_t IT;
+    const int tid = wave * 64 + lane;
+    int nHighW = 0, movedW = 0;
+    if (wave > 0) z.mvk = reinterpret_cast<int*>(z.cand) + (wave - 1) * ZONE_MAX_MOVE;
+    for (int pass = 1; pass <= z.zp.maxPasses; ++pass) {
+        for (int lc = tid; lc < z.C; lc += 64 * ZONE_WAVES) zone_maps(z, lc, 0);
+        __syncthreads();
+        if (wave == 0) {
+            const int m = zone_compact(z, lane, 64, ZC_DENSE);
+            ZBAR();
+            zone_shuffle(z.dl, m, lane, 64, ShufDraw{z.seed, ZS_DENSE, (uint32_t)pass, 0u});
+            /* levels in z.ext[0..m) (free until the sparse phase); 0 = skipped */
+            int floorLev = 0, maxLev = 0;
+            for (int k = 0; k < m; ++k) {
+                const int c = (int)z.dl[k];
+                const int nc = cnt_of(z, c);
+                const double hl = high_thr(z, c);
+                int lv = 0;
+                if (nc > 0 && (double)nc > hl) {
+                    const int needOut = imax(0, (int)MPCD_CEIL((double)nc - hl));
+                    const bool one = !(needOut > 0 && imin(needOut + 1, nc) <= ZONE_MAX_MOVE);
+                    int best = 0;
+                    if (one) {
+                        best = maxLev;
+                    } else {
+                        for (int j = lane; j < k; j += 64) {
+                            const int lj = iget(z.ext[j]);
+                            if (lj > best && dense_conflict(z, c, (int)z.dl[j])) best = lj;
+                        }
+                        for (int o = 32; o > 0; o >>= 1) best = imax(best, __shfl_xor(best, o, 64));
+                    }
+                    lv = imax(best, floorLev) + 1;
+                    if (one) floorLev = lv;
+                    maxLev = imax(maxLev, lv);
+                }
+                ZWSYNC();
+                if (lane == 0) z.ext[k] = (IT)lv;
+                ZWSYNC();
+            }
+            if (lane == 0) {
+                z.sh->mwCount = m;
+                z.sh->mwLevels = maxLev;
+            }
+        }
+        __syncthreads();
+        const int m = z.sh->mwCount, nLev = z.sh->mwLevels;
+        for (int l = 1; l <= nLev; ++l) {
+            int t = 0; /* cells of this level met so far, dealt round robin to the wavefronts */
+            for (int b0 = 0; b0 < m; b0 += 64) {
+                const int k = b0 + lane;
+                unsigned long long msk = wave_ballot(k < m && iget(z.ext[k]) == l);
+                while (msk) {
+                    const int bit = (int)__builtin_ctzll(msk);
+                    msk &= msk - 1ull;
+                    if ((t++ % ZONE_WAVES) != wave) continue;
+                    const int c = (int)z.dl[b0 + bit];
+                    const int nc = cnt_of(z, c);
+                    const double hl = high_thr(z, c);
+                    const int needOut = imax(0, (int)MPCD_CEIL((double)nc - hl));
+                    if (imin(needOut + 1, nc) <= ZONE_MAX_MOVE) {
+                        ++nHighW;
+                        movedW += dense_cell_wave(z, c, nc, hl, needOut, pass, lane);
+                        continue;
+                    }
+                    /* the one-lane form (alone in its level, on wavefront 0) */
+                    int moved = 0;
+                    if (lane == 0) {
+                        ++z.sh->nHigh;
+                        z.sh->nr = 0;
+                    }
+                    dense_targets(z, c, 0, pass, 0, lane, 64);
+                    if (lane == 0) {
+                        int nr = 0;
+                        moved += spread(z, c, needOut, pass, z.recvT, nr);
+                        z.sh->nr = nr;
+                    }
+                    ZBAR();
+                    const int remaining = imax(0, (int)MPCD_CEIL((double)cnt_of(z, c) - hl));
+                    if (remaining > 0) {
+                        dense_targets(z, c, 1, pass, 1, lane, 64);
+                        if (lane == 0) {
+                            int nr = z.sh->nr;
+                            moved += spread(z, c, remaining, pass, z.recvT, nr);
+                            z.sh->nr = nr;
+                        }
+                    }
+                    if (lane == 0) {
+                        const int nr = z.sh->nr;
+                        for (int r = 0; r < nr; ++r) fset(z, z.recvT[r], ZF_RECV);
+                        transfer_masks(z, c, z.recvT, nr);
+                        z.sh->movedDense += moved;
+                    }
+                    ZBAR();
+                }
+            }
+            __syncthreads();
+        }
+    }
+    if (lane == 0) {
+        atomicAdd(&z.sh->nHigh, nHighW);
+        atomicAdd(&z.sh->movedDense, movedW);
+    }
+    __syncthreads();
+}
+#else
+/* (the host passes of the compilers see a declaration only) */
+template <class Z>
+ZHD void zone_dense_passes_mw(Z&, int, int) {}
+#endif
 
 /* ---- sparse phase ---- */
 
