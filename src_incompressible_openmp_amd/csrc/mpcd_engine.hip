@@ -1156,11 +1156,10 @@ __global__ void __launch_bounds__(FUSED_TPB, 4) k_zone_fused(ZoneArgs a) { /* 4 
     z.allOwned = sh.allOwned;
     for (int k = (int)threadIdx.x; k < z.P; k += FUSED_TPB) z.rec[k] = (unsigned char)ZM_NONE;
     __syncthreads();
-    /* the dense passes on all four wavefronts, the sparse phase on one */
+    /* the dense passes and the sparse phase on all four wavefronts */
     zone_dense_passes_mw(z, wave, lane);
+    zone_sparse_pass_mw(z, wave, lane);
     if (wave == 0) {
-        zone_sparse_pass(z, lane, ZONE_TPB);
-        ZBAR();
         zone_dump(z, zone_state_at<IT>(a.zstate + (size_t)zi * a.zsStride, z.C, z.zw), lane, ZONE_TPB);
         if (lane == 0) {
             ZoneOut o;

@@ -245,7 +245,9 @@ struct ZoneCtxT {
     int* dord2;     /* 8 */
     double* nsm4;   /* 4 */
     int* mvk;          /* ZONE_MAX_MOVE: local indices of a materialised list's head run */
-    IT* tmpL;          /* ZONE_TMP_L */
+    IT* tmpL;          /* tmpCap entries (ZONE_TMP_L, or a wavefront's share in the multi-wavefront sparse phase) */
+    IT* lix;           /* per cell: its index in the multi-wavefront sparse phase's list, or the type's maximum */
+    int tmpCap;
     ZoneShared* sh;
 };
 
@@ -1527,24 +1529,105 @@ ZHD void donor_transfer(Z& z, int c, int d, int md, int nt, int cgc, T* tmp, int
         fset(z, c, ZF_RECV);
         const int rv1[1] = {c};
         transfer_masks(z, d, rv1, 1);
-        z.sh->movedSparse += nt;
     }
     ZBAR();
+}
+
+/* One low cell of the sparse phase (zone_pass.cpp:804-854): its donors among
+ * the zone neighbours at range 1, ranked, and the transfers; all lanes of
+ * one wavefront.  Returns the particles moved into c. */
+template <class Z>
+ZHD int sparse_cell(Z& z, int c, int nc, int lowInt, int lane, int nl) {
+    const int cgx = lc_gx(z, c), cgy = lc_gy(z, c), cgc = lc_gc(z, c);
+    /* zone neighbours at range 1 (local_neighbors_in_zone :263-278) */
+    for (int o = lane; o < 8; o += nl) {
+        const int f = o < 4 ? o : o + 1;
+        const int dy = f / 3 - 1, dx = f % 3 - 1;
+        int jx, jy, q = -1;
+        if (shift_idx(z, cgx, cgy, dx, dy, jx, jy)) q = zone_lc(z, jx, jy);
+        z.nbGc[o] = q;
+    }
+    ZBAR();
+    int nn = 0, nd = 0;
+    for (int b0 = 0; b0 < 8; b0 += nl) {
+        const int o = b0 + lane;
+        bool keep = false;
+        if (o < 8) {
+            const int q = z.nbGc[o];
+            keep = q >= 0;
+            if (z.zp.dedupe)
+                for (int t = 0; t < o && keep; ++t)
+                    if (z.nbGc[t] == q) keep = false;
+        }
+        const unsigned long long mk = wave_ballot(keep);
+        nn += wave_popc(mk);
+        const bool don = keep && cnt_of(z, z.nbGc[o]) > lowInt;
+        const unsigned long long mdn = wave_ballot(don);
+        if (don) z.dord[nd + wave_popc(mdn & lanes_below(lane))] = z.nbGc[o];
+        nd += wave_popc(mdn);
+    }
+    ZBAR();
+    if (nn == 0) return 0;
+    int need = imax(0, lowInt - nc);
+    if (need <= 0 || nd == 0) return 0;
+    double gxd, gyd;
+    inward_dir(z, cgx, cgy, lane, nl, gxd, gyd);
+    /* rank_admissible_neighbors_inward (zone_pass.cpp:636-667) */
+    zone_shuffle(z.dord, nd, lane, nl, ShufDraw{z.seed, ZS_DONOR, (uint32_t)cgc, 0u});
+    const double ndir = mpcd_hypot(gxd, gyd);
+    for (int t = lane; t < nd; t += nl) {
+        const int q = z.dord[t];
+        int ddx, ddy;
+        min_step(z, cgx, cgy, lc_gx(z, q), lc_gy(z, q), ddx, ddy);
+        const double nstep = mpcd_hypot((double)ddx, (double)ddy);
+        double align = 0.0;
+        if (ndir >= 1e-14 && nstep >= 1e-14) align = ((double)ddx * gxd + (double)ddy * gyd) / (ndir * nstep);
+        const double noise = 1e-9 * mpcd_draw_donor_noise(z.seed, (uint32_t)cgc, (uint32_t)lc_gc(z, q));
+        z.dsc[t] = 100.0 * align + nsm_ext(z, lc_gx(z, q), lc_gy(z, q)) + noise;
+    }
+    ZBAR();
+    for (int t = lane; t < nd; t += nl) { /* stable, descending score */
+        int r = 0;
+        for (int u = 0; u < nd; ++u)
+            if (z.dsc[u] > z.dsc[t] || (u < t && !(z.dsc[t] > z.dsc[u]))) ++r;
+        z.dord2[r] = z.dord[t];
+    }
+    ZBAR();
+    int moved = 0;
+    for (int t = 0; t < nd; ++t) {
+        const int d = z.dord2[t];
+        const int md = cnt_of(z, d);
+        if (md <= 0) continue;
+        const int avail = md - lowInt;
+        if (avail <= 0) continue;
+        const int nt = imin(imin(need, avail), md);
+        if (nt <= 0) continue;
+        if (md <= z.tmpCap) donor_transfer(z, c, d, md, nt, cgc, z.tmpL, lane, nl);
+        else donor_transfer(z, c, d, md, nt, cgc, z.tmpG, lane, nl);
+        moved += nt;
+        need -= nt;
+        if (need <= 0) break;
+    }
+    return moved;
+}
+
+/* the sparse phase's set-up: maps, and the counts it starts from (the zone
+ * and its ring) in z.ext; threads tid .. of nt */
+template <class Z>
+ZHD void sparse_setup(Z& z, int tid, int nt) {
+    for (int lc = tid; lc < z.C; lc += nt) zone_maps(z, lc, 0);
+    const int H = z.zh + 4, ne = (z.zw + 4) * H;
+    for (int e = tid; e < ne; e += nt) {
+        const int rx = e / H - 2, ry = e % H - 2;
+        const int h = halo_slot(z, rx, ry);
+        z.ext[e] = h >= 0 ? z.halo[h] : z.cnt[rx * z.zh + ry];
+    }
 }
 
 /* Sparse pass (zone_pass.cpp:783-854), all lanes. */
 template <class Z>
 ZHD void zone_sparse_pass(Z& z, int lane, int nl) {
-    typedef typename Z::idx_t IT;
-    for (int lc = lane; lc < z.C; lc += nl) zone_maps(z, lc, 0);
-    {
-        const int H = z.zh + 4, ne = (z.zw + 4) * H;
-        for (int e = lane; e < ne; e += nl) {
-            const int rx = e / H - 2, ry = e % H - 2;
-            const int h = halo_slot(z, rx, ry);
-            z.ext[e] = h >= 0 ? z.halo[h] : z.cnt[rx * z.zh + ry];
-        }
-    }
+    sparse_setup(z, lane, nl);
     ZBAR();
     const int m = zone_compact(z, lane, nl, ZC_LOW);
     ZBAR();
@@ -1555,77 +1638,190 @@ ZHD void zone_sparse_pass(Z& z, int lane, int nl) {
         const int nc = cnt_of(z, c);
         if (nc <= 0) continue;
         if (lane == 0) ++z.sh->nLow;
-        const int cgx = lc_gx(z, c), cgy = lc_gy(z, c), cgc = lc_gc(z, c);
-        /* zone neighbours at range 1 (local_neighbors_in_zone :263-278) */
-        for (int o = lane; o < 8; o += nl) {
-            const int f = o < 4 ? o : o + 1;
-            const int dy = f / 3 - 1, dx = f % 3 - 1;
-            int jx, jy, q = -1;
-            if (shift_idx(z, cgx, cgy, dx, dy, jx, jy)) q = zone_lc(z, jx, jy);
-            z.nbGc[o] = q;
-        }
-        ZBAR();
-        int nn = 0, nd = 0;
-        for (int b0 = 0; b0 < 8; b0 += nl) {
-            const int o = b0 + lane;
-            bool keep = false;
-            if (o < 8) {
-                const int q = z.nbGc[o];
-                keep = q >= 0;
-                if (z.zp.dedupe)
-                    for (int t = 0; t < o && keep; ++t)
-                        if (z.nbGc[t] == q) keep = false;
-            }
-            const unsigned long long mk = wave_ballot(keep);
-            nn += wave_popc(mk);
-            const bool don = keep && cnt_of(z, z.nbGc[o]) > lowInt;
-            const unsigned long long mdn = wave_ballot(don);
-            if (don) z.dord[nd + wave_popc(mdn & lanes_below(lane))] = z.nbGc[o];
-            nd += wave_popc(mdn);
-        }
-        ZBAR();
-        if (nn == 0) continue;
-        int need = imax(0, lowInt - nc);
-        if (need <= 0 || nd == 0) continue;
-        double gxd, gyd;
-        inward_dir(z, cgx, cgy, lane, nl, gxd, gyd);
-        /* rank_admissible_neighbors_inward (zone_pass.cpp:636-667) */
-        zone_shuffle(z.dord, nd, lane, nl,
-                     ShufDraw{z.seed, ZS_DONOR, (uint32_t)cgc, 0u});
-        const double ndir = mpcd_hypot(gxd, gyd);
-        for (int t = lane; t < nd; t += nl) {
-            const int q = z.dord[t];
-            int ddx, ddy;
-            min_step(z, cgx, cgy, lc_gx(z, q), lc_gy(z, q), ddx, ddy);
-            const double nstep = mpcd_hypot((double)ddx, (double)ddy);
-            double align = 0.0;
-            if (ndir >= 1e-14 && nstep >= 1e-14) align = ((double)ddx * gxd + (double)ddy * gyd) / (ndir * nstep);
-            const double noise = 1e-9 * mpcd_draw_donor_noise(z.seed, (uint32_t)cgc, (uint32_t)lc_gc(z, q));
-            z.dsc[t] = 100.0 * align + nsm_ext(z, lc_gx(z, q), lc_gy(z, q)) + noise;
-        }
-        ZBAR();
-        for (int t = lane; t < nd; t += nl) { /* stable, descending score */
-            int r = 0;
-            for (int u = 0; u < nd; ++u)
-                if (z.dsc[u] > z.dsc[t] || (u < t && !(z.dsc[t] > z.dsc[u]))) ++r;
-            z.dord2[r] = z.dord[t];
-        }
-        ZBAR();
-        for (int t = 0; t < nd; ++t) {
-            const int d = z.dord2[t];
-            const int md = cnt_of(z, d);
-            if (md <= 0) continue;
-            const int avail = md - lowInt;
-            if (avail <= 0) continue;
-            const int nt = imin(imin(need, avail), md);
-            if (nt <= 0) continue;
-            if (md <= ZONE_TMP_L) donor_transfer(z, c, d, md, nt, cgc, z.tmpL, lane, nl);
-            else donor_transfer(z, c, d, md, nt, cgc, z.tmpG, lane, nl);
-            need -= nt;
-            if (need <= 0) break;
-        }
+        const int moved = sparse_cell(z, c, nc, lowInt, lane, nl);
+        if (lane == 0) z.sh->movedSparse += moved;
     }
 }
+
+#if defined(__HIP_DEVICE_COMPILE__)
+/* The sparse phase on the four wavefronts (device only).  A low cell's
+ * transfers read and change only itself and its zone neighbours at range 1
+ * (its own count changes only by itself: a low cell is never a donor), and
+ * the smoothed counts they rank by are the phase's starting counts, so low
+ * cells more than 2 apart commute.  Rounds: every unfinished cell with no
+ * unfinished earlier cell within range 2 (found through lix, each cell's list
+ * index, cleared when the cell is finished) runs in this round, the k-th such
+ * cell on wavefront k mod 4.  A cell one of whose neighbours starts with more
+ * particles than a wavefront's share of the transfer scratch (ZF_ALONE)
+ * conflicts with every cell and runs alone when it is the first unfinished,
+ * on wavefront 0 (which has the full scratch and the global region).
+ * Conflicting cells keep the list order, so the result is the serial loop's
+ * bit for bit.  Each round marks its cells (ZF_READY) before any runs, and
+ * finished cells carry ZF_DONE, so every wavefront deals the same cells. */
+constexpr int ZONE_TMP_W = 112; /* transfer scratch of each of wavefronts 1..3 */
+enum : unsigned short { ZF_DONE = 512, ZF_READY = 1024, ZF_ALONE = 2048 };
+
+/* a low cell with a zone neighbour at range 1 that starts the phase with
+ * more particles than a wavefront's transfer scratch holds (donor lists only
+ * shrink in the phase) */
+template <class Z>
+ZHD bool sparse_alone(const Z& z, int c) {
+    const int H = z.zh + 4;
+    bool big = false;
+    int jx, jy;
+    for (int dx = -1; dx <= 1; ++dx)
+        for (int dy = -1; dy <= 1; ++dy)
+            if ((dx || dy) && shift_idx(z, lc_gx(z, c), lc_gy(z, c), dx, dy, jx, jy)) {
+                const int q = zone_lc(z, jx, jy);
+                if (q >= 0 && iget(z.ext[(q / z.zh + 2) * H + q % z.zh + 2]) > ZONE_TMP_W) big = true;
+            }
+    return big;
+}
+
+template <class Z>
+ZHD bool sparse_conflict(const Z& z, int a, int b) {
+    int dx = a / z.zh - b / z.zh, dy = a % z.zh - b % z.zh;
+    dx = dx < 0 ? -dx : dx;
+    dy = dy < 0 ? -dy : dy;
+    if (z.g.perx && z.zw == z.g.Nx) dx = imin(dx, z.zw - dx);
+    if (z.g.pery && z.zh == z.g.Ny) dy = imin(dy, z.zh - dy);
+    return dx <= 2 && dy <= 2;
+}
+
+template <class Z>
+__device__ __forceinline__ void zone_sparse_pass_mw(Z& z, int wave, int lane) {
+    typedef typename Z::idx_t IT;
+    static_assert(sizeof(IT) == 2, "the wavefronts' scratch shares are laid out for 16-bit indices");
+    const int tid = wave * 64 + lane;
+    if (wave > 0) {
+        /* this wavefront's scratch: 448 bytes of the dense phase's one-lane
+         * candidate and threshold arrays (cand, ordThr, nbTie, contiguous and
+         * idle now) */
+        unsigned char* b = reinterpret_cast<unsigned char*>(z.cand) + (wave - 1) * 448;
+        z.dsc = reinterpret_cast<double*>(b);
+        z.nsm4 = reinterpret_cast<double*>(b + 64);
+        z.nbGc = reinterpret_cast<int*>(b + 96);
+        z.dord = reinterpret_cast<int*>(b + 128);
+        z.dord2 = reinterpret_cast<int*>(b + 160);
+        z.tmpL = reinterpret_cast<IT*>(b + 224);
+        z.tmpCap = ZONE_TMP_W;
+    }
+    const IT none = (IT)~(IT)0;
+    sparse_setup(z, tid, 64 * ZONE_WAVES);
+    for (int lc = tid; lc < z.C; lc += 64 * ZONE_WAVES) {
+        z.fl[lc] = (unsigned short)(z.fl[lc] & ~(ZF_DONE | ZF_READY | ZF_ALONE));
+        z.lix[lc] = none;
+    }
+    __syncthreads();
+    if (wave == 0) {
+        const int m = zone_compact(z, lane, 64, ZC_LOW);
+        ZBAR();
+        zone_shuffle(z.dl, m, lane, 64, ShufDraw{z.seed, ZS_LOW, 0u, 0u});
+        if (lane == 0) z.sh->mwCount = m;
+    }
+    __syncthreads();
+    const int m = z.sh->mwCount;
+    for (int k = tid; k < m; k += 64 * ZONE_WAVES) {
+        const int c = (int)z.dl[k];
+        z.lix[c] = (IT)k;
+        if (sparse_alone(z, c)) z.fl[c] = (unsigned short)(z.fl[c] | ZF_ALONE);
+    }
+    __syncthreads();
+    const int lowInt = (int)MPCD_CEIL(z.zp.lowThrBulk);
+    const bool wx = z.g.perx && z.zw == z.g.Nx, wy = z.g.pery && z.zh == z.g.Ny;
+    int nLowW = 0, movedW = 0;
+    /* the first unfinished cell and the first unfinished one that runs alone */
+    auto scan = [&](int& done, int& first, int& firstAlone) {
+        done = 0;
+        first = m;
+        firstAlone = m;
+        for (int b0 = 0; b0 < m; b0 += 64) {
+            const int k = b0 + lane;
+            const unsigned short f = k < m ? z.fl[(int)z.dl[k]] : (unsigned short)ZF_DONE;
+            const unsigned long long dm = wave_ballot(f & ZF_DONE);
+            done += wave_popc(dm & (b0 + 64 <= m ? ~0ull : lanes_below(m - b0)));
+            const unsigned long long um = ~dm;
+            if (first == m && um) first = b0 + (int)__builtin_ctzll(um);
+            const unsigned long long am = wave_ballot(!(f & ZF_DONE) && (f & ZF_ALONE));
+            if (firstAlone == m && am) firstAlone = b0 + (int)__builtin_ctzll(am);
+        }
+    };
+    int done, first, firstAlone;
+    scan(done, first, firstAlone);
+    for (int round = 0; round < m && done < m; ++round) {
+        /* 1. the round's ready cells, marked ZF_READY: the first unfinished
+         * cell, and every cell before the first unfinished lone one that
+         * runs with others and has no unfinished earlier cell within range 2
+         * (lix of finished cells is cleared) */
+        for (int k = tid; k < m; k += 64 * ZONE_WAVES) {
+            const int c = (int)z.dl[k];
+            const unsigned short f = z.fl[c];
+            if (f & ZF_DONE) continue;
+            bool ready = k == first;
+            if (!ready && k < firstAlone && !(f & ZF_ALONE)) {
+                ready = true;
+                const int rx = c / z.zh, ry = c - rx * z.zh;
+                for (int dx = -2; dx <= 2 && ready; ++dx)
+                    for (int dy = -2; dy <= 2; ++dy) {
+                        int qx = rx + dx, qy = ry + dy;
+                        if (wx) qx = wrapi(qx, z.zw);
+                        if (wy) qy = wrapi(qy, z.zh);
+                        if (qx < 0 || qx >= z.zw || qy < 0 || qy >= z.zh) continue;
+                        if ((int)z.lix[qx * z.zh + qy] < k) {
+                            ready = false;
+                            break;
+                        }
+                    }
+            }
+            if (ready) z.fl[c] = (unsigned short)(f | ZF_READY);
+        }
+        __syncthreads();
+        /* 2. the ready cells, the t-th on wavefront t mod 4; ZF_READY stays
+         * as it is until every wavefront has dealt the round */
+        int t = 0;
+        for (int b0 = 0; b0 < m; b0 += 64) {
+            const int k = b0 + lane;
+            unsigned long long msk = wave_ballot(k < m && (z.fl[(int)z.dl[k]] & ZF_READY));
+            while (msk) {
+                const int bit = (int)__builtin_ctzll(msk);
+                msk &= msk - 1ull;
+                if ((t++ % ZONE_WAVES) != wave) continue;
+                const int c = (int)z.dl[b0 + bit];
+                const int nc = cnt_of(z, c);
+                if (nc > 0) {
+                    ++nLowW;
+                    movedW += sparse_cell(z, c, nc, lowInt, lane, 64);
+                }
+                ZWSYNC();
+                if (lane == 0) {
+                    z.fl[c] = (unsigned short)(z.fl[c] | ZF_DONE);
+                    z.lix[c] = none;
+                }
+                ZWSYNC();
+            }
+        }
+        __syncthreads();
+        /* 3. clear the round's marks; every wavefront scans the flags the
+         * same way */
+        for (int k = tid; k < m; k += 64 * ZONE_WAVES) {
+            const int c = (int)z.dl[k];
+            z.fl[c] = (unsigned short)(z.fl[c] & ~ZF_READY);
+        }
+        scan(done, first, firstAlone);
+        __syncthreads();
+    }
+    if (lane == 0) {
+        atomicAdd(&z.sh->nLow, nLowW);
+        atomicAdd(&z.sh->movedSparse, movedW);
+    }
+    for (int lc = tid; lc < z.C; lc += 64 * ZONE_WAVES)
+        z.fl[lc] = (unsigned short)(z.fl[lc] & ~(ZF_DONE | ZF_READY | ZF_ALONE));
+    __syncthreads();
+}
+#else
+template <class Z>
+ZHD void zone_sparse_pass_mw(Z&, int, int) {}
+#endif
 
 /* ---- set-up, correction, statistics ---- */
 
@@ -1879,6 +2075,7 @@ ZHD size_t zone_fixed_bytes(int C, int W, int useLocal) {
     b += (size_t)(W + 4) * (size_t)(C / (W > 0 ? W : 1) + 4) * sizeof(IT); /* ext */
     b += (size_t)(4 * (W + 4) + 4 * (C / (W > 0 ? W : 1))) * sizeof(IT); /* halo */
     b += (size_t)C * sizeof(unsigned short); /* fl */
+    b += (size_t)C * sizeof(IT);             /* lix */
     b += (size_t)ZONE_TMP_L * sizeof(IT);                       /* tmpL */
     return (b + 15) & ~(size_t)15;
 }
@@ -1918,7 +2115,9 @@ ZHD void zone_carve(ZoneCtxT<IT>& z, unsigned char* base, size_t bytes) {
     z.k0 = (IT*)p; p += (size_t)C * sizeof(IT);
     z.halo = (IT*)p; p += (size_t)halo_size(z) * sizeof(IT);
     z.tmpL = (IT*)p; p += (size_t)ZONE_TMP_L * sizeof(IT);
+    z.tmpCap = ZONE_TMP_L;
     z.fl = (unsigned short*)p; p += (size_t)C * sizeof(unsigned short);
+    z.lix = (IT*)p; p += (size_t)C * sizeof(IT);
     const size_t fixed = zone_fixed_bytes<IT>(C, W, z.zp.useLocal);
     p = base + fixed;
     z.nxL = (IT*)p;
