@@ -37,14 +37,13 @@ METRIC = "particle-steps/s plain vs learned-closure at fixed Poiseuille profile 
 HBM_PEAK_GBS = 8000.0
 # compulsory bytes per particle of each kernel class (a launch covers all particles):
 # stream_bc x,y,vx,vy read+write; collide vx,vy read+write + slot list (x2: in, out)
-# as staged; zone_kernel (the passes and the correction in one kernel): the
-# passes' pass-input counts and ring per cell, move record and link/state
-# hand-over per particle (about 4 B) plus the correction's slot list and vx,vy
-# read, vx,vy written, record read (40 B);
+# as staged; zone_kernel (the passes and the correction in one kernel, one
+# launch per tile pass): the tile pass's compulsory particle traffic, x,y,vx,vy
+# read and vx,vy written (48 B; the phase's figure in the step's 256 B);
 # kick x,y,vx,vy,pid and the slot list read, x,y,vx,vy,pid written in red-cell
 # order (76 B); kick_stream (the kick fused with the next step's stream, all
 # steps of a call but its last) the same plus both cell ids written (84 B)
-ALG_BYTES = {"stream_bc": 64, "collide": 48, "zone_kernel": 44, "kick": 76, "kick_stream": 84}
+ALG_BYTES = {"stream_bc": 64, "collide": 48, "zone_kernel": 48, "kick": 76, "kick_stream": 84}
 STEP_ALG_BYTES = 256  # compulsory particle traffic of one plain step
 # kernel of each timed class, as rocprofv3 names it
 KERNEL_NAME = {"stream_bc": "k_stream_bc", "collide": "k_collide", "zone_kernel": "k_zone_fused<unsigned short>",
