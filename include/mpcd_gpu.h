@@ -211,9 +211,16 @@ int mpcd_gpu_compute_cell_fields_noalloc(const mpcd_params* p, const double* x, 
 int mpcd_gpu_build_zone_layout(const mpcd_params* p, int mode, int32_t* ownerMap, int32_t* nZones, int32_t* rects4);
 
 /* replaces mpcd::run_zone_kernel_inplace (zone_pass.h:62-66) for a
- * rectangular zone [xs,xe) x [ys,ye); stats6 = ZoneKernelStats::stats */
+ * rectangular zone [xs,xe) x [ys,ye); stats6 = ZoneKernelStats::stats,
+ * moved2 = movedDense, movedSparse, diag10 = ZoneKernelStats::diag
+ * (zone_pass.cpp:924-970, reduced on the GPU: occupancy spread / gamma and
+ * out-of-band fraction of the final counts over the whole domain, |change
+ * of total momentum|, RMS of the cells' nonzero x and y momentum defects
+ * before the correction, change of kinetic energy, NaN, 0, RMS of the zone
+ * cells' final x and y velocities over the occupied cells).  Any of the
+ * three may be NULL; a NULL diag10 skips the diagnostics' work. */
 int mpcd_gpu_run_zone_kernel(const mpcd_params* p, double* x, double* v, int xs, int xe, int ys, int ye,
-                             uint64_t rng_seed, double* stats6, int32_t* moved2);
+                             uint64_t rng_seed, double* stats6, int32_t* moved2, double* diag10);
 
 /* replaces mpcd::run_zone_pass_openmp / run_zone_pass_sync_serial
  * (zone_pass_openmp.h:35-53): snapshot semantics, zones on the GPU */

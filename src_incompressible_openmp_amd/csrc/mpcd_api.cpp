@@ -302,7 +302,8 @@ ZoneKernelStats run_zone_kernel_inplace(std::vector<double>& x, std::vector<doub
     const mpcd_params p = to_c(params);
     ZoneKernelStats out{};
     int32_t moved[2] = {0, 0};
-    ok(mpcd_gpu_run_zone_kernel(&p, x.data(), v.data(), xs, xe, ys, ye, rngSeed, out.stats.data(), moved));
+    ok(mpcd_gpu_run_zone_kernel(&p, x.data(), v.data(), xs, xe, ys, ye, rngSeed, out.stats.data(), moved,
+                                out.diag.data()));
     out.movedDense = moved[0];
     out.movedSparse = moved[1];
     return out;

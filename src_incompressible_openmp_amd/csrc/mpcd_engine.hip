@@ -753,39 +753,45 @@ __global__ void k_sub_mean(double* __restrict__ vx, double* __restrict__ vy, int
 constexpr int OCC_W = 6;
 /* slots of a simulation's diagnostics buffer */
 constexpr int DIAG_OCC_BASE = 0, DIAG_OCC_SHIFTED = 5, DIAG_PSUM = 10, DIAG_USUM = 13, DIAG_SUR_OCC = 16,
-              DIAG_SUR_DIV = 21, DIAG_SLOTS = 24;
-__global__ void __launch_bounds__(TPB) k_occupancy(const int* __restrict__ cnt, int Nx, int Ny, ZoneParams zp,
-                                                   double* __restrict__ part) {
+              DIAG_SUR_DIV = 21, DIAG_ZK_KE0 = 24, DIAG_ZK_KE1 = 27, DIAG_ZK_CELLS = 30, DIAG_ZK_OCC = 36,
+              DIAG_ZK_OUT = 41, DIAG_SLOTS = 51;
+/* gamma x the clamped fluid fraction of cell c's 3x3 binomial smoothing of
+ * the count grid (no wrap) */
+__device__ __forceinline__ double occ_local_gl(const int* __restrict__ cnt, int c, int Nx, int Ny,
+                                               const ZoneParams& zp) {
+    const int ix = c / Ny, iy = c - ix * Ny;
+    long long num = 0;
+    int den = 0;
+    for (int dx = -1; dx <= 1; ++dx)
+        for (int dy = -1; dy <= 1; ++dy) {
+            const int jx = ix + dx, jy = iy + dy;
+            if (jx < 0 || jx >= Nx || jy < 0 || jy >= Ny) continue;
+            const int w = (dx == 0 ? 2 : 1) * (dy == 0 ? 2 : 1);
+            num += (long long)w * cnt[jy + Ny * jx];
+            den += w;
+        }
+    const double t = zp.fluidFracGain * (((double)num / (double)den) / dmax(zp.gamma, 1e-12));
+    return zp.gamma * ((t < 0.0) ? 0.0 : ((1.0 < t) ? 1.0 : t));
+}
+
+/* cntLow: the grid the low thresholds come from (cnt itself for the pass
+ * statistics; for a zone kernel's diagnostics the counts its sparse phase
+ * started from, zone_pass.cpp:783-795, which the final maps keep) */
+__global__ void __launch_bounds__(TPB) k_occupancy(const int* __restrict__ cnt, const int* __restrict__ cntLow,
+                                                   int Nx, int Ny, ZoneParams zp, double* __restrict__ part) {
     __shared__ double sh[OCC_W][TPB];
     const int Nc = Nx * Ny;
     double a[OCC_W] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     for (int c = blockIdx.x * TPB + (int)threadIdx.x; c < Nc; c += gridDim.x * TPB) {
-        const int m = cnt[c];
+        const int m = cnt[c], mL = cntLow[c];
         const double d = (double)m - zp.gamma;
         a[0] += d;
         a[1] += d * d;
         double lo = zp.lowThrBulk, hi = zp.highThr;
-        if (m == 0) {
-            lo = zp.lowLoc1;
-            hi = zp.highLoc1;
-            if (zp.useLocal) {
-                const int ix = c / Ny, iy = c - ix * Ny;
-                long long num = 0;
-                int den = 0;
-                for (int dx = -1; dx <= 1; ++dx)
-                    for (int dy = -1; dy <= 1; ++dy) {
-                        const int jx = ix + dx, jy = iy + dy;
-                        if (jx < 0 || jx >= Nx || jy < 0 || jy >= Ny) continue;
-                        const int w = (dx == 0 ? 2 : 1) * (dy == 0 ? 2 : 1);
-                        num += (long long)w * cnt[jy + Ny * jx];
-                        den += w;
-                    }
-                const double t = zp.fluidFracGain * (((double)num / (double)den) / dmax(zp.gamma, 1e-12));
-                const double gl = zp.gamma * ((t < 0.0) ? 0.0 : ((1.0 < t) ? 1.0 : t));
-                lo = dmax(zp.lowFrac * gl, zp.lowThrFloor);
-                hi = dmax(zp.highFrac * gl, zp.highThrFloor);
-            }
-        }
+        if (m == 0) hi = zp.useLocal ? dmax(zp.highFrac * occ_local_gl(cnt, c, Nx, Ny, zp), zp.highThrFloor)
+                                     : zp.highLoc1;
+        if (mL == 0) lo = zp.useLocal ? dmax(zp.lowFrac * occ_local_gl(cntLow, c, Nx, Ny, zp), zp.lowThrFloor)
+                                      : zp.lowLoc1;
         const bool dense = (double)m > hi, sparse = m > 0 && (double)m < lo;
         a[2] += (dense || sparse) ? 1.0 : 0.0;
         a[3] += dense ? 1.0 : 0.0;
@@ -846,6 +852,78 @@ __global__ void __launch_bounds__(TPB) k_sum_ke(const double* __restrict__ vx, c
         part[3 * (size_t)blockIdx.x + 1] = s1[0];
         part[3 * (size_t)blockIdx.x + 2] = s2[0];
     }
+}
+
+/* ---- ZoneKernelStats.diag of a single-zone call (zone_pass.cpp:924-970) ---- */
+
+/* the final counts, with the zone's cells replaced by the counts its sparse
+ * phase started from: the grid of the final low thresholds */
+__global__ void __launch_bounds__(TPB) k_zone_low_counts(const int* __restrict__ cnt, int Nc, int Ny, ZoneRect r,
+                                                         const int* __restrict__ dgCnt, int* __restrict__ out) {
+    const int c = blockIdx.x * TPB + (int)threadIdx.x;
+    if (c >= Nc) return;
+    const int ix = c / Ny, iy = c - ix * Ny;
+    const bool in = ix >= r.xs && ix < r.xe && iy >= r.ys && iy < r.ye;
+    out[c] = in ? dgCnt[(ix - r.xs) * (r.ye - r.ys) + (iy - r.ys)] : cnt[c];
+}
+
+/* Sums over the zone's cells of the kernel's per-cell record, one block:
+ * out[0..6) = sum and count of the nonzero squared momentum defects in x,
+ * the same in y, and the sums of the squared final velocities
+ * (momentum / count) of the occupied cells in x and y. */
+__global__ void __launch_bounds__(TPB) k_zone_diag_cells(const double* __restrict__ dg, const int* __restrict__ cnt,
+                                                         int Ny, ZoneRect r, double* __restrict__ out) {
+    __shared__ double sh[6][TPB];
+    const int zh = r.ye - r.ys, C = (r.xe - r.xs) * zh;
+    double a[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    for (int lc = (int)threadIdx.x; lc < C; lc += TPB) {
+        const int gc = (r.ys + lc % zh) + Ny * (r.xs + lc / zh);
+        const double ex = dg[4 * (size_t)lc] * dg[4 * (size_t)lc], ey = dg[4 * (size_t)lc + 1] * dg[4 * (size_t)lc + 1];
+        if (ex > 0.0) {
+            a[0] += ex;
+            a[1] += 1.0;
+        }
+        if (ey > 0.0) {
+            a[2] += ey;
+            a[3] += 1.0;
+        }
+        const int m = cnt[gc];
+        if (m > 0) {
+            const double ux = dg[4 * (size_t)lc + 2] / (double)m, uy = dg[4 * (size_t)lc + 3] / (double)m;
+            a[4] += ux * ux;
+            a[5] += uy * uy;
+        }
+    }
+    for (int k = 0; k < 6; ++k) sh[k][threadIdx.x] = a[k];
+    __syncthreads();
+    for (int st = TPB / 2; st > 0; st >>= 1) {
+        if (threadIdx.x < (unsigned)st)
+            for (int k = 0; k < 6; ++k) sh[k][threadIdx.x] += sh[k][threadIdx.x + st];
+        __syncthreads();
+    }
+    if (threadIdx.x < 6) out[threadIdx.x] = sh[threadIdx.x][0];
+}
+
+/* diag[0..10) from the occupancy statistics of the final counts (occ:
+ * occStd, outBand, nDense, nSparse, nEmpty), the particle sums (vx, vy,
+ * kinetic energy) before and after the kernel, and the cell sums above;
+ * every cell is active and the cells with a velocity are the occupied ones */
+__global__ void k_zone_diag_final(const double* __restrict__ occ, const double* __restrict__ ke0,
+                                  const double* __restrict__ ke1, const double* __restrict__ cs, int Nc,
+                                  double* __restrict__ diag) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    diag[0] = occ[0];
+    diag[1] = occ[1];
+    const double dX = ke1[0] - ke0[0], dY = ke1[1] - ke0[1];
+    diag[2] = sqrt(dX * dX + dY * dY);
+    diag[3] = cs[1] > 0.0 ? sqrt(cs[0] / cs[1]) : 0.0;
+    diag[4] = cs[3] > 0.0 ? sqrt(cs[2] / cs[3]) : 0.0;
+    diag[5] = ke1[2] - ke0[2];
+    diag[6] = __builtin_nan("");
+    diag[7] = 0.0;
+    const double nV = (double)Nc - occ[4];
+    diag[8] = nV > 0.0 ? sqrt(cs[4] / nV) : 0.0;
+    diag[9] = nV > 0.0 ? sqrt(cs[5] / nV) : 0.0;
 }
 
 /* ---- cell fields ---- */
@@ -1762,6 +1840,10 @@ struct mpcd_gpu_sim {
      * fields of the current state (allocated on first use) */
     bool wantDiag = false;
     double* diagDev = nullptr; /* DIAG_* slots */
+    /* a single-zone call's diagnostics: the zone kernel's per-cell record
+     * and sparse-phase start counts (non-null only during such a call) */
+    double* zdCell = nullptr;
+    int* zdCnt = nullptr;
     int* outN = nullptr;
     double* outF = nullptr; /* 9 x Nc: Ux, Uy, Px, Py, T, rho, Pkin, Pvir, P */
     WallAcc* wallPart = nullptr;
@@ -2239,6 +2321,8 @@ struct mpcd_gpu_sim {
         a.in.vsy = vy2;
         a.in.inPx = inPx;
         a.in.inPy = inPy;
+        a.in.dgCell = zdCell;
+        a.in.dgCnt = zdCnt;
         if (record) {
             /* moved particles record their cell changes for rebin_recorded()
              * (bin.rnk is -1 for every slot here: set at creation, and each
@@ -2549,11 +2633,13 @@ struct mpcd_gpu_sim {
 
     /* ---- diagnostics (device reductions; the host reads a few scalars) ---- */
 
-    /* occupancy statistics of the count grid `cnt` into diagDev[slot..slot+5) */
-    void occupancy(const int* cnt, int slot) {
+    /* occupancy statistics of the count grid `cnt` into diagDev[slot..slot+5)
+     * (cntLow: the grid of the low thresholds, if not cnt) */
+    void occupancy(const int* cnt, int slot, const int* cntLow = nullptr) {
         const int nb = (int)std::min<size_t>(nblk(Nc), 512);
         ensure_parts((size_t)nb * OCC_W);
-        hipLaunchKernelGGL(k_occupancy, dim3(nb), dim3(TPB), 0, stream, cnt, p.Nx, p.Ny, zp, part);
+        hipLaunchKernelGGL(k_occupancy, dim3(nb), dim3(TPB), 0, stream, cnt, cntLow ? cntLow : cnt, p.Nx, p.Ny,
+                           zp, part);
         hipLaunchKernelGGL(k_occupancy_final, dim3(1), dim3(TPB), 0, stream, part, nb, Nc, p.gamma, diagDev + slot);
         check_launch();
     }
@@ -2878,7 +2964,7 @@ int mpcd_gpu_occupancy_stats(const mpcd_params* p, const int32_t* counts, mpcd_o
         double* dpart = dalloc<double>((size_t)nb * OCC_W + 8);
         hipStream_t st = nullptr;
         HIP_OK(hipMemcpy(dc, counts, (size_t)Nc * sizeof(int), hipMemcpyHostToDevice));
-        hipLaunchKernelGGL(k_occupancy, dim3(nb), dim3(TPB), 0, st, dc, p->Nx, p->Ny, zp, dpart);
+        hipLaunchKernelGGL(k_occupancy, dim3(nb), dim3(TPB), 0, st, dc, dc, p->Nx, p->Ny, zp, dpart);
         hipLaunchKernelGGL(k_occupancy_final, dim3(1), dim3(TPB), 0, st, dpart, nb, Nc, p->gamma,
                            dpart + (size_t)nb * OCC_W);
         HIP_OK(hipGetLastError());
@@ -3075,7 +3161,7 @@ int mpcd_gpu_build_zone_layout(const mpcd_params* p, int mode, int32_t* ownerMap
 }
 
 int mpcd_gpu_run_zone_kernel(const mpcd_params* p, double* x, double* v, int xs, int xe, int ys, int ye,
-                             uint64_t rng_seed, double* stats6, int32_t* moved2) {
+                             uint64_t rng_seed, double* stats6, int32_t* moved2, double* diag10) {
     return guarded([&] {
         std::lock_guard<std::mutex> lk(g_cache_mu);
         if (xs < 0 || ys < 0 || xe > p->Nx || ye > p->Ny || xs >= xe || ys >= ye)
@@ -3084,10 +3170,48 @@ int mpcd_gpu_run_zone_kernel(const mpcd_params* p, double* x, double* v, int xs,
         s.set_state(x, v);
         s.bin_unshifted();
         ZoneRect r{xs, xe, ys, ye};
+        const int C = (xe - xs) * (ye - ys);
         ZoneRect* d = dalloc<ZoneRect>(1);
         HIP_OK(hipMemcpyAsync(d, &r, sizeof(r), hipMemcpyHostToDevice, s.stream));
         s.clear_err();
-        s.zone_pass(rng_seed, d, 1, (xe - xs) * (ye - ys), xe - xs, nullptr, nullptr, nullptr, 0, 1);
+        if (diag10) {
+            /* particle sums before; the kernel's per-cell record starts at zero */
+            const int nb = (int)nblk(s.n);
+            hipLaunchKernelGGL(k_sum_ke, dim3(nb), dim3(TPB), 0, s.stream, s.vx, s.vy, s.n, s.part);
+            hipLaunchKernelGGL(k_reduce_parts, dim3(1), dim3(TPB), 0, s.stream, s.part, nb, 3, s.diagDev + DIAG_ZK_KE0);
+            s.zdCell = dalloc<double>(4 * (size_t)C);
+            s.zdCnt = dalloc<int>((size_t)C);
+            hipLaunchKernelGGL(k_fill_double, dim3(nblk(4 * (size_t)C)), dim3(TPB), 0, s.stream, s.zdCell, 4 * C, 0.0);
+            s.check_launch();
+        }
+        s.zone_pass(rng_seed, d, 1, C, xe - xs, nullptr, nullptr, nullptr, 0, 1);
+        if (diag10) {
+            /* ZoneKernelStats.diag: occupancy of the final counts (low maps from
+             * the sparse phase's start counts), particle sums after, cell sums */
+            s.bin_unshifted();
+            const int nb = (int)nblk(s.n);
+            hipLaunchKernelGGL(k_sum_ke, dim3(nb), dim3(TPB), 0, s.stream, s.vx, s.vy, s.n, s.part);
+            hipLaunchKernelGGL(k_reduce_parts, dim3(1), dim3(TPB), 0, s.stream, s.part, nb, 3, s.diagDev + DIAG_ZK_KE1);
+            int* low = dalloc<int>((size_t)s.Nc);
+            hipLaunchKernelGGL(k_zone_low_counts, dim3(nblk(s.Nc)), dim3(TPB), 0, s.stream, s.bin.cnt, s.Nc, p->Ny, r,
+                               s.zdCnt, low);
+            s.check_launch();
+            s.occupancy(s.bin.cnt, DIAG_ZK_OCC, low);
+            hipLaunchKernelGGL(k_zone_diag_cells, dim3(1), dim3(TPB), 0, s.stream, s.zdCell, s.bin.cnt, p->Ny, r,
+                               s.diagDev + DIAG_ZK_CELLS);
+            hipLaunchKernelGGL(k_zone_diag_final, dim3(1), dim3(64), 0, s.stream, s.diagDev + DIAG_ZK_OCC,
+                               s.diagDev + DIAG_ZK_KE0, s.diagDev + DIAG_ZK_KE1, s.diagDev + DIAG_ZK_CELLS, s.Nc,
+                               s.diagDev + DIAG_ZK_OUT);
+            s.check_launch();
+            HIP_OK(hipMemcpyAsync(diag10, s.diagDev + DIAG_ZK_OUT, 10 * sizeof(double), hipMemcpyDeviceToHost,
+                                  s.stream));
+            HIP_OK(hipStreamSynchronize(s.stream));
+            HIP_OK(hipFree(low));
+            HIP_OK(hipFree(s.zdCell));
+            HIP_OK(hipFree(s.zdCnt));
+            s.zdCell = nullptr;
+            s.zdCnt = nullptr;
+        }
         ZoneOut o{};
         HIP_OK(hipMemcpyAsync(&o, s.zout, sizeof(o), hipMemcpyDeviceToHost, s.stream));
         s.get_state(x, v);

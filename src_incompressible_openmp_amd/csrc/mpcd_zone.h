@@ -196,6 +196,12 @@ struct ZoneIn {
      * binning update (cell of each slot, departures and arrivals per cell,
      * rank among the new cell's arrivals) */
     int *dcid = nullptr, *dout = nullptr, *dinc = nullptr, *drnk = nullptr;
+    /* non-null (a single-zone call that reports ZoneKernelStats.diag): per
+     * zone cell, the momentum defect and the final list's momentum before
+     * the correction (4 doubles), and the count the sparse phase starts
+     * from */
+    double* dgCell = nullptr;
+    int* dgCnt = nullptr;
 };
 
 /* Scalars the lanes share (LDS on the device). */
@@ -220,6 +226,8 @@ struct ZoneCtxT {
     double *x, *y, *vx, *vy, *vsx, *vsy;
     const double *inPx, *inPy; /* input run sums per global cell, or nullptr */
     int *dcid, *dout, *dinc, *drnk; /* cell-change record, or nullptr */
+    double* dgCell; /* diagnostics' per-cell record, or nullptr */
+    int* dgCnt;     /* diagnostics' sparse-phase start counts, or nullptr */
     IT* nxL;     /* links of k < nL */
     IT* nxG;     /* links of k >= nL (zone's global region, indexed by k) */
     IT* tmpG;    /* donor lists longer than ZONE_TMP_L (zone's global region) */
@@ -1622,6 +1630,8 @@ ZHD void sparse_setup(Z& z, int tid, int nt) {
         const int h = halo_slot(z, rx, ry);
         z.ext[e] = h >= 0 ? z.halo[h] : z.cnt[rx * z.zh + ry];
     }
+    if (z.dgCnt)
+        for (int lc = tid; lc < z.C; lc += nt) z.dgCnt[lc] = (int)z.cnt[lc];
 }
 
 /* Sparse pass (zone_pass.cpp:783-854), all lanes. */
@@ -1915,6 +1925,27 @@ ZHD void run_sum(const Z& z, int s0, int n, double& sx, double& sy) {
     }
 }
 
+/* The diagnostics' record of cell lc (zone_pass.cpp:885-897): the momentum
+ * defect (input run minus final list) and the final list's momentum, both
+ * before the correction. */
+template <class Z>
+ZHD void diag_cell(Z& z, int lc, double dPx, double dPy, double px, double py) {
+    double* d = z.dgCell + 4 * (size_t)lc;
+    d[0] = dPx;
+    d[1] = dPy;
+    d[2] = px;
+    d[3] = py;
+}
+
+/* the same for an occupied cell nothing changed: its list is its input run */
+template <class Z>
+ZHD void diag_cell_unchanged(Z& z, int lc) {
+    const int gc = lc_gc(z, lc);
+    double px = 0.0, py = 0.0;
+    run_sum(z, z.gstart[gc], z.gcnt[gc], px, py);
+    diag_cell(z, lc, 0.0, 0.0, px, py);
+}
+
 /* Momentum correction (zone_pass.cpp:870-909), first half, one lane per
  * cell: the velocity change of a changed cell from the momentum of its
  * input run and of its final list (both in list order).  Reads only; a cell
@@ -1923,7 +1954,10 @@ ZHD void run_sum(const Z& z, int s0, int n, double& sx, double& sy) {
 template <class Z>
 ZHD void zone_correct_sums(Z& z, int lc) {
     const int m = cnt_of(z, lc);
-    if (!fget(z, lc, ZF_CHG) || m == 0) return;
+    if (!fget(z, lc, ZF_CHG) || m == 0) {
+        if (z.dgCell && m > 0) diag_cell_unchanged(z, lc);
+        return;
+    }
     const int gc = lc_gc(z, lc);
     double pbx = 0.0, pby = 0.0;
     run_sum(z, z.gstart[gc], z.gcnt[gc], pbx, pby);
@@ -1950,6 +1984,7 @@ ZHD void zone_correct_sums(Z& z, int lc) {
             }
     }
     const double dPx = pbx - px, dPy = pby - py;
+    if (z.dgCell) diag_cell(z, lc, dPx, dPy, px, py);
     const double dux = dPx / (double)m, duy = dPy / (double)m;
     z.duG[2 * lc] = dux;
     z.duG[2 * lc + 1] = duy;
@@ -2150,6 +2185,8 @@ ZHD void zone_init(ZoneCtxT<IT>& z, const ZoneIn& in, ZoneRect r, int zid, uint6
     z.dout = in.dout;
     z.dinc = in.dinc;
     z.drnk = in.drnk;
+    z.dgCell = in.dgCell;
+    z.dgCnt = in.dgCnt;
     z.x = in.x;
     z.y = in.y;
     z.vx = in.vx;
@@ -2332,6 +2369,7 @@ template <class Z>
 ZHD double fin_cell(Z& z, int lc, double pbx, double pby) {
     const int m = cnt_of(z, lc);
     if (!fget(z, lc, ZF_CHG)) {
+        if (z.dgCell && m > 0) diag_cell_unchanged(z, lc);
         zone_correct_write(z, lc); /* the side-buffer copy of a covered cell */
         return 0.0;
     }
@@ -2371,6 +2409,7 @@ ZHD double fin_cell(Z& z, int lc, double pbx, double pby) {
                 py += ay[u];
             }
     }
+    if (z.dgCell) diag_cell(z, lc, pbx - px, pby - py, px, py);
     const double dux = (pbx - px) / (double)m, duy = (pby - py) / (double)m;
     bool corr = z.zp.corrEnabled != 0;
     if (corr && fget(z, lc, ZF_SKIP)) corr = false;

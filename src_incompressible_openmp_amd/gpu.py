@@ -176,13 +176,18 @@ def build_zone_layout(p: MpcdParams, mode: int):
     return owner, rects.reshape(-1, 4)
 
 
-def run_zone_kernel(p: MpcdParams, x: np.ndarray, v: np.ndarray, rect, seed: int):
+def run_zone_kernel(p: MpcdParams, x: np.ndarray, v: np.ndarray, rect, seed: int, diag: bool = False):
+    """run_zone_kernel_inplace on the rectangle (xs, xe, ys, ye), in place:
+    returns (stats[6], moved[2]), with diag=True also ZoneKernelStats.diag[10]
+    (reduced on the GPU)."""
     stats = np.zeros(6)
     moved = np.zeros(2, np.int32)
+    dg = np.zeros(10)
     xs, xe, ys, ye = (int(t) for t in rect)
     _check(load().mpcd_gpu_run_zone_kernel(ctypes.byref(p), _ptr(_f64(x)), _ptr(_f64(v)), xs, xe, ys, ye,
-                                           ctypes.c_uint64(seed), _ptr(stats), _ptr(moved, ctypes.c_int32)))
-    return stats, moved
+                                           ctypes.c_uint64(seed), _ptr(stats), _ptr(moved, ctypes.c_int32),
+                                           _ptr(dg) if diag else None))
+    return (stats, moved, dg) if diag else (stats, moved)
 
 
 def zone_pass(p: MpcdParams, x: np.ndarray, v: np.ndarray, layout: int, seed_base: int,

@@ -8,6 +8,7 @@ digests of the outputs of the larger cases in ``digests.json``.
 Run in the development container, where the reference sources exist:
 
     make -C oracle && python tests/golden/make_goldens.py
+    python tests/golden/make_goldens.py --zone-kernel   # zonekernel.npz only
 """
 from __future__ import annotations
 
@@ -168,9 +169,60 @@ def digest_case(name: str):
     return res
 
 
+ZONE_KERNEL_CASES = SMALL_CASES + DIGEST_CASES
+
+
+def zone_kernel_ids(n_zones: int):
+    """zones probed per layout: the first, one in the middle, the last"""
+    return sorted({0, n_zones // 2, n_zones - 1})
+
+
+def zone_kernel_case(name: str):
+    """run_zone_kernel_inplace of the original (through oracle/_ref/zone_kernel_probe)
+    on single zones of both layouts, from the case's input state: its
+    ZoneKernelStats (stats, diag, moved counts) and SHA-256 digests of the
+    state it leaves."""
+    p = case_params(name)
+    n = int(p["n"])
+    x, v, _, _ = case_state(name)
+    out = {}
+    with tempfile.TemporaryDirectory() as d:
+        write_params_kv(os.path.join(d, "params.kv"), p)
+        fileio.write_xy(os.path.join(d, "x.bin"), x, n)
+        fileio.write_xy(os.path.join(d, "v.bin"), v, n)
+        for lay in ("base", "shifted"):
+            zones, zi = None, 0
+            todo = [0]
+            while todo:
+                zi = todo.pop(0)
+                seed = 7001 + zi
+                run("zone_kernel_probe", os.path.join(d, "params.kv"), os.path.join(d, "x.bin"),
+                    os.path.join(d, "v.bin"), lay, str(zi), str(seed), os.path.join(d, "o"))
+                kv = read_kv(os.path.join(d, "o_stats.kv"))
+                if zones is None:
+                    zones = int(kv["nZones"])
+                    todo = [z for z in zone_kernel_ids(zones) if z != 0]
+                key = f"{name}_{lay}_{zi}"
+                out[key + "_seed"] = np.array(seed)
+                out[key + "_stats"] = np.array([kv[f"stats{k}"] for k in range(6)])
+                out[key + "_diag"] = np.array([kv[f"diag{k}"] for k in range(10)])
+                out[key + "_moved"] = np.array([kv["movedDense"], kv["movedSparse"]], np.int32)
+                out[key + "_x_sha256"] = np.array(digest(fileio.read_xy(os.path.join(d, "o_x.bin"), n)))
+                out[key + "_v_sha256"] = np.array(digest(fileio.read_xy(os.path.join(d, "o_v.bin"), n)))
+            out[f"{name}_{lay}_zones"] = np.array(zones)
+    return out
+
+
 def main() -> None:
     if not os.path.exists(os.path.join(REF, "main_step01_stream_bc_collision")):
         raise SystemExit("build the compiled original first: make -C oracle")
+    if "--zone-kernel" in sys.argv[1:]:
+        arrs = {}
+        for name in ZONE_KERNEL_CASES:
+            arrs.update(zone_kernel_case(name))
+        np.savez_compressed(os.path.join(HERE, "zonekernel.npz"), **arrs)
+        print("wrote zonekernel.npz")
+        return
     for name in SMALL_CASES:
         arrs = small_case(name)
         np.savez_compressed(os.path.join(HERE, f"{name}.npz"), **arrs)

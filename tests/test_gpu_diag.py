@@ -71,6 +71,45 @@ def test_occupancy_stats(gpu, oracle, case):
     occ_equal(gpu.occupancy_stats(p, clustered), oracle.occupancy_stats(p, clustered))
 
 
+def diag_close(g, o, scale_p, scale_e):
+    """ZoneKernelStats.diag of the GPU (device reductions) against the
+    oracle's (sequential sums): the occupancy terms, defect and velocity
+    RMS to 1e-12 relative; the total-momentum change (rounding of two sums
+    of the same momentum) and the energy change to 1e-12 of the sums'
+    scales; diag[6] NaN and diag[7] 0 in both."""
+    for k in (0, 1, 3, 4, 8, 9):
+        assert abs(g[k] - o[k]) <= 1e-12 * max(abs(o[k]), 1e-300), (k, g[k], o[k])
+    assert g[2] <= 1e-12 * scale_p and o[2] <= 1e-12 * scale_p, (g[2], o[2])
+    assert abs(g[5] - o[5]) <= 1e-12 * scale_e, (g[5], o[5])
+    assert np.isnan(g[6]) and np.isnan(o[6]) and g[7] == 0.0 and o[7] == 0.0
+
+
+@pytest.mark.parametrize("case", CASES)
+def test_zone_kernel_diag(gpu, oracle, case):
+    """run_zone_kernel_inplace with ZoneKernelStats.diag reduced on the GPU,
+    on the first, a middle and the last zone of both layouts: state, stats
+    and moved counts bit for bit (as without the diagnostics), diag within
+    the tolerances of diag_close."""
+    from src_incompressible_openmp_amd.params import LAYOUT_BASE, LAYOUT_SHIFTED
+
+    p = to_struct(case_params(case))
+    x, v, _, _ = case_state(case)
+    scale_p = float(np.abs(v).sum())
+    scale_e = float(0.5 * np.sum(v * v))
+    for layout in (LAYOUT_BASE, LAYOUT_SHIFTED):
+        _, rects = gpu.build_zone_layout(p, layout)
+        for zi in sorted({0, len(rects) // 2, len(rects) - 1}):
+            print(f"zone kernel diag: {case} layout {layout} zone {zi}", flush=True)
+            seed = 7001 + zi
+            xo, vo = x.copy(), v.copy()
+            st, dgo, mv = oracle.zone_kernel(CTR, p, xo, vo, layout, zi, seed)
+            xg, vg = x.copy(), v.copy()
+            sg, mg, dgg = gpu.run_zone_kernel(p, xg, vg, rects[zi], seed, diag=True)
+            assert xg.tobytes() == xo.tobytes() and vg.tobytes() == vo.tobytes()
+            assert np.array_equal(mg, mv) and sg.tobytes() == st.tobytes()
+            diag_close(dgg, dgo, scale_p, scale_e)
+
+
 @pytest.mark.parametrize("case", CASES)
 def test_step_diag(gpu, oracle, case):
     """One resident step with its device diagnostics against the oracle's

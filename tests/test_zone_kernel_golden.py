@@ -1,0 +1,74 @@
+"""The oracle's run_zone_kernel_inplace (libstdc++ mode) against the compiled
+original on single zones of both layouts: the state it leaves (SHA-256 of
+x and v), ZoneKernelStats.stats, .diag (zone_pass.cpp:924-970: occupancy
+spread and out-of-band fraction over the whole domain, global momentum and
+energy change, per-cell momentum defects, cell-velocity RMS) and the moved
+counts, all bit for bit.  Fixture: tests/golden/zonekernel.npz, written by
+tests/golden/make_goldens.py --zone-kernel through oracle/_ref/zone_kernel_probe."""
+from __future__ import annotations
+
+import hashlib
+import os
+
+import numpy as np
+import pytest
+
+from src_incompressible_openmp_amd.params import LAYOUT_BASE, LAYOUT_SHIFTED, to_struct
+from tests.golden.cases import DIGEST_CASES, SMALL_CASES, case_params, case_state
+from tests.oracle_lib import STD, get_oracle
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+FIX = os.path.join(HERE, "golden", "zonekernel.npz")
+
+
+def sha(a: np.ndarray) -> str:
+    return hashlib.sha256(np.ascontiguousarray(a, dtype="<f8").tobytes()).hexdigest()
+
+
+def same_bits(a, b) -> bool:
+    a, b = np.asarray(a, np.float64), np.asarray(b, np.float64)
+    return a.shape == b.shape and np.array_equal(a.view(np.uint64), b.view(np.uint64))
+
+
+@pytest.fixture(scope="module")
+def fix():
+    return np.load(FIX)
+
+
+@pytest.mark.parametrize("case", SMALL_CASES + DIGEST_CASES)
+@pytest.mark.parametrize("layout", ("base", "shifted"))
+def test_zone_kernel_stats_and_diag_match_original(fix, case, layout):
+    o = get_oracle()
+    p = to_struct(case_params(case))
+    x, v, _, _ = case_state(case)
+    lay = LAYOUT_BASE if layout == "base" else LAYOUT_SHIFTED
+    nz = int(fix[f"{case}_{layout}_zones"])
+    zones = sorted({0, nz // 2, nz - 1})
+    for zi in zones:
+        key = f"{case}_{layout}_{zi}"
+        xo, vo = x.copy(), v.copy()
+        st, dg, mv = o.zone_kernel(STD, p, xo, vo, lay, zi, int(fix[key + "_seed"]))
+        assert sha(xo) == str(fix[key + "_x_sha256"]), key
+        assert sha(vo) == str(fix[key + "_v_sha256"]), key
+        assert np.array_equal(mv, fix[key + "_moved"]), key
+        assert same_bits(st, fix[key + "_stats"]), key
+        # diag[6] is NaN in both (same_bits compares the NaN's bits too)
+        assert same_bits(dg, fix[key + "_diag"]), (key, dg, fix[key + "_diag"])
+
+
+def test_zone_kernel_diag_closed_forms(fix):
+    """What the diagnostics must satisfy whatever the draws: diag[6] NaN,
+    diag[7] 0, and diag[0], diag[1] equal to the occupancy spread and the
+    out-of-band fraction of the final counts computed here from the state."""
+    o = get_oracle()
+    p = to_struct(case_params("small"))
+    x, v, _, _ = case_state("small")
+    xo, vo = x.copy(), v.copy()
+    _, dg, _ = o.zone_kernel(STD, p, xo, vo, LAYOUT_BASE, 0, int(fix["small_base_0_seed"]))
+    assert np.isnan(dg[6]) and dg[7] == 0.0
+    ix = np.clip(np.floor(xo[0::2] / (p.Lx / p.Nx)).astype(int), 0, p.Nx - 1)
+    iy = np.clip(np.floor(xo[1::2] / (p.Ly / p.Ny)).astype(int), 0, p.Ny - 1)
+    cnt = np.bincount(iy + p.Ny * ix, minlength=p.Nc).astype(np.float64)
+    assert abs(dg[0] - cnt.std() / p.gamma) <= 1e-12 * max(1.0, dg[0])
+    # the momentum correction keeps the zone's total momentum to rounding
+    assert dg[2] < 1e-9 * np.abs(v).sum()
