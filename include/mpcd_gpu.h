@@ -344,6 +344,30 @@ int mpcd_gpu_sim_surrogate_counts(mpcd_gpu_sim* s, int32_t* counts3);
  * is computed from) and its applied correction / outScale [cell][2];
  * feat/target (nullable) receive the last stored sample */
 int mpcd_gpu_sim_closure_sample(mpcd_gpu_sim* s, int enable, double outScale, float* feat, float* target);
+/* Training data of the closure surrogate in the reference's closure formats
+ * (the CellFields and grids of LiquidClosureResult, liquid_closure.h:29-44,
+ * as main_step04_liquid_closure.cpp:61-71 writes them): with enable = 1
+ * every plain closure of the following steps keeps a snapshot of its input
+ * and output on the device -- the reference state's and the red state's
+ * CellFields (compute_cell_fields, common_grid.h:10-13; rhoTarget filled
+ * with the simulation's target density), the grids pdriveRaw, pdrive,
+ * duRepairX/Y, duEOSX/Y and betaRepairApplied (interzoneMask, a constant of
+ * the layouts, is copied from the simulation's own).  The call
+ * copies the last snapshot kept into every non-NULL array of ref, red and
+ * grids and into *beta, and sets *step to its step (-1 when none was kept
+ * yet; then nothing else is written).  enable = 0 stops keeping them. */
+int mpcd_gpu_sim_closure_snapshot(mpcd_gpu_sim* s, int enable, const mpcd_cell_fields* ref,
+                                  const mpcd_cell_fields* red, const mpcd_closure_grids* grids, double* beta,
+                                  int32_t* step);
+/* The surrogate's training pair of one closure snapshot (as the files of
+ * mpcd_gpu_sim_closure_snapshot / main_step04_liquid_closure hold it),
+ * computed on the GPU by the kernels of mpcd_gpu_sim_closure_sample:
+ * feat[Nc][4] from the reference state's N, Ux, Uy, T, and target[Nc][2] =
+ * (beta duRepair + duEOS) / outScale.  feat or target may be NULL. */
+int mpcd_gpu_closure_training_pair(const mpcd_params* p, const int32_t* refN, const double* refUx,
+                                   const double* refUy, const double* refT, const double* duRepairX,
+                                   const double* duRepairY, const double* duEOSX, const double* duEOSY, double beta,
+                                   double outScale, float* feat, float* target);
 /* one 16x16x32 bfloat16 matrix-core product of 64 lanes x 8 operands each
  * (a, b: 512 raw bfloat16 values, d: 256 floats), to check the lane layout */
 int mpcd_gpu_net_mfma_probe(const uint16_t* a, const uint16_t* b, float* d);

@@ -15,6 +15,12 @@
  * params.kv also holds benchmark_nSteps, benchmark_metricsEvery and
  * benchmark_dumpSteps (comma-separated steps; 0 dumps the initial state).
  *
+ * Training data for the closure surrogate: an optional params.kv key
+ * benchmark_closureDumpSteps (comma-separated steps; the reference driver
+ * does not read it) writes each listed step's closure input and output in
+ * main_step04_liquid_closure's formats and file suffixes, under the stem
+ * <out>_closure_stepNNNN.
+ *
  * Differences, all in the timing fields: the phase times are HIP event
  * times of the GPU work (reference build = force, stream, walls, collision
  * and the reference fields; each pass includes its binning update; between
@@ -38,6 +44,7 @@
 
 #include "dump_io.h"
 #include "gpu_params.h"
+#include "liquid_closure.h"
 #include "mpcd_gpu.h"
 #include "params_io.h"
 #include "state_io.h"
@@ -139,6 +146,13 @@ int main(int argc, char** argv) {
         const int metricsEvery = std::stoi(key(kv, "benchmark_metricsEvery"));
         const std::vector<int> dumps = steps_csv(key(kv, "benchmark_dumpSteps"));
         auto dumpAt = [&](int s) { return std::binary_search(dumps.begin(), dumps.end(), s); };
+        /* optional: steps whose closure input and output are written as
+         * training data (absent from the reference driver's keys, which it
+         * ignores) */
+        const auto cdIt = kv.find("benchmark_closureDumpSteps");
+        const std::vector<int> closureDumps = cdIt == kv.end() ? std::vector<int>{} : steps_csv(cdIt->second);
+        auto closureAt = [&](int s) { return std::binary_search(closureDumps.begin(), closureDumps.end(), s); };
+        auto stopAt = [&](int s) { return dumpAt(s) || closureAt(s); };
         auto rowAt = [&](int s) { return metricsEvery > 0 && (s % metricsEvery == 0 || s == nSteps); };
 
         mpcd::State st = mpcd::read_state(xPath, vPath, typePath, r0Path, hasType, hasR0, prm.n);
@@ -180,6 +194,49 @@ int main(int argc, char** argv) {
             T.dumps += seconds(t0, Clock::now());
         };
         if (dumpAt(0)) dump(0);
+        /* the closure of step s as main_step04_liquid_closure writes it
+         * (:61-71; the repaired and out fields are not kept), under the stem
+         * <out>_closure_stepNNNN, with its beta in _closure.kv */
+        auto closure_dump = [&](int s) {
+            const auto t0 = Clock::now();
+            const size_t C = (size_t)prm.Nc;
+            mpcd::CellFields F[2];
+            mpcd_cell_fields cf[2];
+            for (int k = 0; k < 2; ++k) {
+                mpcd::CellFields& G = F[k];
+                G.Nx = prm.Nx;
+                G.Ny = prm.Ny;
+                G.N.assign(C, 0);
+                for (auto* f : {&G.Ux, &G.Uy, &G.Px, &G.Py, &G.T, &G.rho, &G.Pkin, &G.Pvir, &G.P, &G.rhoTarget})
+                    f->assign(C, 0.0);
+                cf[k] = mpcd_cell_fields{G.N.data(),   G.Ux.data(),   G.Uy.data(),   G.Px.data(),
+                                         G.Py.data(),  G.T.data(),    G.rho.data(),  G.Pkin.data(),
+                                         G.Pvir.data(), G.P.data(),   G.rhoTarget.data()};
+            }
+            std::vector<double> gr[6];
+            for (auto& q : gr) q.assign(C, 0.0);
+            std::vector<std::int32_t> mask(C, 0);
+            mpcd_closure_grids cg{gr[0].data(), gr[1].data(), gr[2].data(), gr[3].data(),
+                                  gr[4].data(), gr[5].data(), mask.data()};
+            double beta = 0.0;
+            int32_t got = -1;
+            ok(mpcd_gpu_sim_closure_snapshot(sim, 1, &cf[0], &cf[1], &cg, &beta, &got));
+            if (got != s) throw std::runtime_error("closure snapshot of step " + std::to_string(s) + " missing");
+            std::ostringstream o;
+            o << out << "_closure_step" << std::setw(4) << std::setfill('0') << s;
+            const std::string pre = o.str();
+            mpcd::write_cell_fields_bin(pre + "_cellfields_ref.bin", F[0]);
+            mpcd::write_cell_fields_bin(pre + "_cellfields_red.bin", F[1]);
+            const char* names[6] = {"_pdrive_raw.bin", "_pdrive.bin", "_durepairx.bin",
+                                    "_durepairy.bin",  "_dueosx.bin", "_dueosy.bin"};
+            for (int j = 0; j < 6; ++j) mpcd::write_f64_grid_bin(pre + names[j], prm.Nx, prm.Ny, gr[j]);
+            mpcd::write_i32_grid_bin_lc(pre + "_interzone_mask.bin", prm.Nx, prm.Ny, mask);
+            std::ofstream kvo(pre + "_closure.kv");
+            if (!kvo) throw std::runtime_error("Cannot open closure snapshot file");
+            kvo << std::setprecision(17) << "step=" << s << "\nbetaRepairApplied=" << beta << "\n";
+            T.dumps += seconds(t0, Clock::now());
+        };
+        if (!closureDumps.empty()) ok(mpcd_gpu_sim_closure_snapshot(sim, 1, nullptr, nullptr, nullptr, nullptr, nullptr));
 
         ok(mpcd_gpu_sim_profile(sim, MPCD_PROFILE_PHASES)); /* phase events on every step */
         double phaseSeen[4] = {0, 0, 0, 0};
@@ -191,12 +248,13 @@ int main(int argc, char** argv) {
                 /* steps up to the next row, dump or the last step in one call
                  * (the kick of each performs the next one's stream) */
                 int e = step;
-                while (e < nSteps && !rowAt(e) && !dumpAt(e)) ++e;
-                const int k = dumpAt(e) && !rowAt(e) && e < nSteps ? e - step + 1 : e - step;
+                while (e < nSteps && !rowAt(e) && !stopAt(e)) ++e;
+                const int k = stopAt(e) && !rowAt(e) && e < nSteps ? e - step + 1 : e - step;
                 ok(mpcd_gpu_sim_step(sim, step, k, nullptr));
                 T.loop += seconds(t0, Clock::now());
                 step += k;
                 if (dumpAt(step - 1)) dump(step - 1);
+                if (closureAt(step - 1)) closure_dump(step - 1);
                 continue;
             }
             mpcd_step_metrics m{};
@@ -230,6 +288,7 @@ int main(int argc, char** argv) {
             }
             T.loop += tStep;
             if (dumpAt(step)) dump(step);
+            if (closureAt(step)) closure_dump(step);
             if (last) {
                 const auto& c = m.closure;
                 man << "finalOccStd=" << d.shifted.occStd << "\n";

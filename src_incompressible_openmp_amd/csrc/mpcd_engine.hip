@@ -1270,6 +1270,25 @@ __global__ void k_overlap_copyback(const int* __restrict__ cells, int ncells, co
     }
 }
 
+/* a plain closure's grids and applied beta into a snapshot: out = 6 grids of
+ * Nc (pdriveRaw, pdrive, duRepairX, duRepairY, duEOSX, duEOSY), then beta */
+__global__ void __launch_bounds__(TPB) k_snap_grids(int Nc, const double* __restrict__ a, const double* __restrict__ b,
+                                                    const double* __restrict__ c, const double* __restrict__ d,
+                                                    const double* __restrict__ e, const double* __restrict__ f,
+                                                    const double* __restrict__ scal, double* __restrict__ out) {
+    const int i = blockIdx.x * TPB + (int)threadIdx.x;
+    if (i < Nc) {
+        const size_t C = (size_t)Nc;
+        out[i] = a[i];
+        out[C + i] = b[i];
+        out[2 * C + i] = c[i];
+        out[3 * C + i] = d[i];
+        out[4 * C + i] = e[i];
+        out[5 * C + i] = f[i];
+    }
+    if (i == 0) out[6 * (size_t)Nc] = scal[3];
+}
+
 /* ---- closure (liquid_closure.cpp:340-436) ---- */
 
 struct ClosureArgs {
@@ -1824,6 +1843,13 @@ struct mpcd_gpu_sim {
     int surK = 0;
     double surScale = 1.0, surCap = 0.0;
     bool wantSample = false;
+    /* closure snapshots (training data in the reference's formats): the
+     * reference and red CellFields (N, then 9 fields each), the six grids
+     * and beta of the last plain closure since they were enabled */
+    bool wantSnap = false;
+    int* snapN = nullptr;     /* 2 Nc: ref, red */
+    double* snapF = nullptr;  /* 9 Nc ref, 9 Nc red, 6 Nc grids, beta */
+    int snapStep = -1, curStep = 0;
     int* redN = nullptr;
     double* redT = nullptr;
     int* refN = nullptr;    /* the reference state's counts and temperatures (the surrogate's features) */
@@ -2070,7 +2096,7 @@ struct mpcd_gpu_sim {
                         tmpg, zsize, zoff, owner[0], owner[1], cover[0], cover[1], overCells[0], overCells[1], stage,
                         refPx, refPy, refUx, refUy, pdriveRaw, redPx, redPy, redUx, redUy, duRX, duRY, duEX, duEY,
                         pdrive, cellTmp, mask, part, scal, err, wallPart, rects[0], rects[1], zout, zscratch, duG, recG, zstate,
-                        redN, redT, featBuf, tgtBuf, surFlag, diagDev, outN, outF, surDev, rowAcc, refN, refT};
+                        redN, redT, featBuf, tgtBuf, surFlag, diagDev, outN, outF, surDev, rowAcc, refN, refT, snapN, snapF};
         for (void* q : ptrs)
             if (q) (void)hipFree(q);
         for (hipEvent_t e : evPool) (void)hipEventDestroy(e);
@@ -2273,6 +2299,19 @@ struct mpcd_gpu_sim {
             f.N = refN;
             f.T = refT;
         }
+        hipLaunchKernelGGL(k_cell_fields, dim3(nblk(Nc, CELL_BLK)), dim3(CELL_BLK), cellCap * 2 * sizeof(double), stream,
+                           bin.start, bin.list, Nc, vx, vy, vc(), p.Kvirial, rho_target(), f, cellCap);
+        check_launch();
+        if (wantSnap) snap_fields(0);
+    }
+
+    /* every CellFields field of the current binning into snapshot k (0: the
+     * reference state, 1: the red state) */
+    void snap_fields(int k) {
+        const size_t C = (size_t)Nc;
+        double* b = snapF + (size_t)k * 9 * C;
+        FieldsOut f{snapN + (size_t)k * C, b, b + C, b + 2 * C, b + 3 * C, b + 4 * C, b + 5 * C, b + 6 * C, b + 7 * C,
+                    b + 8 * C};
         hipLaunchKernelGGL(k_cell_fields, dim3(nblk(Nc, CELL_BLK)), dim3(CELL_BLK), cellCap * 2 * sizeof(double), stream,
                            bin.start, bin.list, Nc, vx, vy, vc(), p.Kvirial, rho_target(), f, cellCap);
         check_launch();
@@ -2582,6 +2621,14 @@ struct mpcd_gpu_sim {
                            bin.start, bin.list, Nc, vx, vy, vc(), p.Kvirial, rho_target(), f, cellCap);
         closure_grids(nullptr);
         const int nbc = (int)nblk(Nc);
+        if (wantSnap) {
+            snap_fields(1);
+            double* gr = snapF + 18 * (size_t)Nc;
+            hipLaunchKernelGGL(k_snap_grids, dim3(nbc), dim3(TPB), 0, stream, Nc, pdriveRaw, pdrive, duRX, duRY, duEX, duEY,
+                               scal, gr);
+            check_launch();
+            snapStep = curStep;
+        }
         if (sur) sur_residual(1); /* the plain closure's residuals become the surrogate's reference */
         if (wantSample) {
             hipLaunchKernelGGL(k_features, dim3(nbc), dim3(TPB), 0, stream, g, refN, refUx, refUy, refT, p.gamma, p.kBT,
@@ -2703,6 +2750,7 @@ struct mpcd_gpu_sim {
      * fused into this step's kick when no metrics are asked) */
     void step(int s, mpcd_step_metrics* sm, double* const* stages, int next = -1) {
         check_pass_modes();
+        curStep = s;
         const uint64_t us = (uint64_t)s;
         const bool pre = streamed == s && !sm;
         phase_mark(0);
@@ -3365,6 +3413,84 @@ int mpcd_gpu_sim_closure_sample(mpcd_gpu_sim* s, int enable, double outScale, fl
         }
         s->wantSample = enable != 0;
         if (outScale > 0.0) s->surScale = outScale;
+    });
+}
+
+int mpcd_gpu_sim_closure_snapshot(mpcd_gpu_sim* s, int enable, const mpcd_cell_fields* ref,
+                                  const mpcd_cell_fields* red, const mpcd_closure_grids* grids, double* beta,
+                                  int32_t* step) {
+    return guarded([&] {
+        if (!s) throw ApiError(MPCD_E_INVALID, "null simulation");
+        const size_t C = (size_t)s->Nc;
+        if (enable && !s->snapF) {
+            s->snapN = dalloc<int>(2 * C);
+            s->snapF = dalloc<double>(24 * C + 8);
+        }
+        if (ref || red || grids || beta || step) {
+            HIP_OK(hipStreamSynchronize(s->stream));
+            if (step) *step = s->snapStep;
+            if (s->snapStep >= 0) {
+                const mpcd_cell_fields* cf[2] = {ref, red};
+                for (int k = 0; k < 2; ++k) {
+                    if (!cf[k]) continue;
+                    if (cf[k]->N)
+                        HIP_OK(hipMemcpy(cf[k]->N, s->snapN + k * C, C * sizeof(int), hipMemcpyDeviceToHost));
+                    double* hs[9] = {cf[k]->Ux,  cf[k]->Uy,   cf[k]->Px,   cf[k]->Py, cf[k]->T,
+                                     cf[k]->rho, cf[k]->Pkin, cf[k]->Pvir, cf[k]->P};
+                    for (int j = 0; j < 9; ++j)
+                        if (hs[j])
+                            HIP_OK(hipMemcpy(hs[j], s->snapF + (size_t)(9 * k + j) * C, C * sizeof(double),
+                                             hipMemcpyDeviceToHost));
+                    if (cf[k]->rhoTarget)
+                        for (size_t c = 0; c < C; ++c) cf[k]->rhoTarget[c] = s->rho_target();
+                }
+                if (grids) {
+                    double* hg[6] = {grids->pdriveRaw, grids->pdrive, grids->duRepairX,
+                                     grids->duRepairY, grids->duEOSX, grids->duEOSY};
+                    for (int j = 0; j < 6; ++j)
+                        if (hg[j])
+                            HIP_OK(hipMemcpy(hg[j], s->snapF + (size_t)(18 + j) * C, C * sizeof(double),
+                                             hipMemcpyDeviceToHost));
+                    if (grids->interzoneMask) /* a constant of the layouts */
+                        HIP_OK(hipMemcpy(grids->interzoneMask, s->mask, C * sizeof(int), hipMemcpyDeviceToHost));
+                }
+                if (beta) HIP_OK(hipMemcpy(beta, s->snapF + 24 * C, sizeof(double), hipMemcpyDeviceToHost));
+            }
+        }
+        s->wantSnap = enable != 0;
+    });
+}
+
+int mpcd_gpu_closure_training_pair(const mpcd_params* p, const int32_t* refN, const double* refUx,
+                                   const double* refUy, const double* refT, const double* duRepairX,
+                                   const double* duRepairY, const double* duEOSX, const double* duEOSY, double beta,
+                                   double outScale, float* feat, float* target) {
+    return guarded([&] {
+        if (!p || !refN || !refUx || !refUy || !refT || !duRepairX || !duRepairY || !duEOSX || !duEOSY)
+            throw ApiError(MPCD_E_INVALID, "null argument");
+        if (p->Nx <= 0 || p->Ny <= 0 || p->Nc != p->Nx * p->Ny) throw ApiError(MPCD_E_INVALID, "bad grid size");
+        if (!(outScale > 0.0)) throw ApiError(MPCD_E_INVALID, "outScale must be positive");
+        const Geom g = make_geom(*p);
+        const size_t C = (size_t)p->Nc;
+        int* dN = dalloc<int>(C);
+        double* d = dalloc<double>(8 * C + 8);
+        float* f = dalloc<float>(6 * C);
+        hipStream_t st = nullptr;
+        HIP_OK(hipMemcpy(dN, refN, C * sizeof(int), hipMemcpyHostToDevice));
+        const double* hs[7] = {refUx, refUy, refT, duRepairX, duRepairY, duEOSX, duEOSY};
+        for (int k = 0; k < 7; ++k) HIP_OK(hipMemcpy(d + k * C, hs[k], C * sizeof(double), hipMemcpyHostToDevice));
+        const double sc[4] = {0.0, 0.0, 0.0, beta};
+        HIP_OK(hipMemcpy(d + 8 * C, sc, sizeof(sc), hipMemcpyHostToDevice));
+        const int nb = (int)nblk(C);
+        hipLaunchKernelGGL(k_features, dim3(nb), dim3(TPB), 0, st, g, dN, d, d + C, d + 2 * C, p->gamma, p->kBT, f);
+        hipLaunchKernelGGL(k_closure_target, dim3(nb), dim3(TPB), 0, st, p->Nc, d + 8 * C, d + 3 * C, d + 4 * C,
+                           d + 5 * C, d + 6 * C, 1.0 / outScale, f + 4 * C);
+        HIP_OK(hipGetLastError());
+        if (feat) HIP_OK(hipMemcpy(feat, f, 4 * C * sizeof(float), hipMemcpyDeviceToHost));
+        if (target) HIP_OK(hipMemcpy(target, f + 4 * C, 2 * C * sizeof(float), hipMemcpyDeviceToHost));
+        HIP_OK(hipFree(dN));
+        HIP_OK(hipFree(d));
+        HIP_OK(hipFree(f));
     });
 }
 

@@ -62,6 +62,14 @@ class ClosureGrids(ctypes.Structure):
         ("interzoneMask", _ip)]
 
 
+def _new_fields(Nc: int):
+    """A CellFieldsC over fresh arrays, and the arrays (dict)."""
+    arrs = {"N": np.zeros(Nc, np.int32)}
+    for k in FIELD_NAMES:
+        arrs[k] = np.zeros(Nc)
+    return CellFieldsC(_ptr(arrs["N"], ctypes.c_int32), *[_ptr(arrs[k]) for k in FIELD_NAMES]), arrs
+
+
 class StepMetrics(ctypes.Structure):
     _fields_ = [("wall", WallInfo), ("base", PassMetrics), ("shifted", PassMetrics), ("closure", ClosureMetrics)]
 
@@ -223,6 +231,26 @@ def occupancy_stats(p: MpcdParams, counts: np.ndarray) -> Dict[str, float]:
     return struct_to_dict(out)
 
 
+def closure_training_pair(p: MpcdParams, ref: Dict[str, np.ndarray], grids: Dict[str, np.ndarray], beta: float,
+                          out_scale: float):
+    """The surrogate's (features [Nc,4], target [Nc,2]) of one closure
+    snapshot (reference CellFields N, Ux, Uy, T; grids duRepairX/Y,
+    duEOSX/Y; the applied beta), computed on the GPU."""
+    Nc = p.Nc
+    f = np.zeros((Nc, 4), np.float32)
+    t = np.zeros((Nc, 2), np.float32)
+    a = [np.ascontiguousarray(ref["N"], np.int32)] + [np.ascontiguousarray(ref[k], np.float64)
+                                                       for k in ("Ux", "Uy", "T")]
+    g = [np.ascontiguousarray(grids[k], np.float64) for k in ("duRepairX", "duRepairY", "duEOSX", "duEOSY")]
+    if any(q.size != Nc for q in a + g):
+        raise ValueError("every field and grid must have Nc entries")
+    _check(load().mpcd_gpu_closure_training_pair(
+        ctypes.byref(p), _ptr(a[0], ctypes.c_int32), *[_ptr(q) for q in a[1:]], *[_ptr(q) for q in g],
+        ctypes.c_double(beta), ctypes.c_double(out_scale), f.ctypes.data_as(ctypes.POINTER(ctypes.c_float)),
+        t.ctypes.data_as(ctypes.POINTER(ctypes.c_float))))
+    return f, t
+
+
 def liquid_closure(p: MpcdParams, x_ref, v_ref, x_red, v_red):
     n2 = 2 * p.n
     xo, vo = np.zeros(n2), np.zeros(n2)
@@ -340,6 +368,26 @@ class Sim:
             f.ctypes.data_as(ctypes.POINTER(ctypes.c_float)) if fetch else None,
             t.ctypes.data_as(ctypes.POINTER(ctypes.c_float)) if fetch else None))
         return (f, t) if fetch else None
+
+    def closure_snapshot(self, enable: bool = True, fetch: bool = False):
+        """Enables keeping each plain closure's input and output (the
+        reference and red CellFields, the six closure grids, beta); with
+        fetch, returns the last one kept as a dict (None if none yet)."""
+        if not fetch:
+            _check(load().mpcd_gpu_sim_closure_snapshot(self._h, int(enable), None, None, None, None, None))
+            return None
+        Nc = self.p.Nc
+        (sref, ref), (sred, red) = _new_fields(Nc), _new_fields(Nc)
+        names = ("pdriveRaw", "pdrive", "duRepairX", "duRepairY", "duEOSX", "duEOSY")
+        grids = {k: np.zeros(Nc) for k in names}
+        grids["interzoneMask"] = np.zeros(Nc, np.int32)
+        cg = ClosureGrids(*[_ptr(grids[k]) for k in names], _ptr(grids["interzoneMask"], ctypes.c_int32))
+        beta, step = ctypes.c_double(0.0), ctypes.c_int32(-1)
+        _check(load().mpcd_gpu_sim_closure_snapshot(self._h, int(enable), ctypes.byref(sref), ctypes.byref(sred),
+                                                     ctypes.byref(cg), ctypes.byref(beta), ctypes.byref(step)))
+        if step.value < 0:
+            return None
+        return {"step": step.value, "betaRepairApplied": beta.value, "ref": ref, "red": red, **grids}
 
     def stream(self) -> int:
         s = ctypes.c_void_p()
