@@ -307,13 +307,70 @@ def learned_closure(sim, p, prm, s0, args, plain_value):
     log("bench: Poiseuille profile error, plain vs surrogate, 50x50 channel")
     prof = poiseuille_profiles(dict(prm, Nx=50, Ny=50, Nc=2500, n=50 * 50 * int(prm["gamma"]), Lx=10.0, Ly=10.0),
                                theta_trained, scale, args.learned_K, args.profile_steps, args.profile_runs)
-    return {"value": value, "ms_per_step": 1e3 * el / args.steps, "speedup_vs_plain": value / plain_value,
+    return {"_theta": theta_trained, "value": value, "ms_per_step": 1e3 * el / args.steps,
+            "speedup_vs_plain": value / plain_value,
             "K": args.learned_K, "steps_learned": c1["learned"] - c0["learned"],
             "steps_plain": c1["plain"] - c0["plain"], "steps_fallback": c1["fallback"] - c0["fallback"],
             "train_forcings": list(forcings), "train_samples": int((~held).sum()), "heldout_samples": int(held.sum()),
             "train_iters": args.train_iters, "sample_s": t_sample, "train_s": t_train,
             "train_loss_first": float(np.mean(losses[:10])), "train_loss_last": float(np.mean(losses[-10:])),
             "heldout_r2": r2, "out_scale": scale, "profile": prof}
+
+
+def large_instance(args, theta, out_scale):
+    """The large single instance, 4096 x 2048 cells x 20 per cell (167.8M
+    particles): plain steps, then (with trained weights) surrogate steps
+    with the plain closure every K-th step; the 3x3 convolutions' weights do
+    not depend on the grid, so the 1024 x 512 training carries over."""
+    from src_incompressible_openmp_amd import surrogate
+
+    Nx, Ny, ppc = 4096, 2048, 20
+    prm = poiseuille_params(Nx=Nx, Ny=Ny, gamma=float(ppc))
+    p = to_struct(prm)
+    log(f"bench: large instance {Nx}x{Ny} x {ppc} (n={p.n}); building state")
+    x, v, _, _ = uniform_state(p.n, p.Lx, p.Ly, seed=1)
+    sim = gpu.Sim(p)
+    sim.set_state(x, v)
+    del x, v
+    sim.step(1, 2)
+    gpu.device_synchronize()
+    steps = args.large_steps
+    log(f"bench: large instance, timing {steps} plain steps")
+    t0 = time.perf_counter()
+    sim.step_async(3, steps)
+    sim.synchronize()
+    el = time.perf_counter() - t0
+    sim.check()
+    plain = p.n * steps / el
+    out = {"workload": f"poiseuille_{Nx}x{Ny}_ppc{ppc}", "particles": p.n, "steps": steps,
+           "plain": {"value": plain, "unit": "particle-steps/s", "ms_per_step": 1e3 * el / steps,
+                     "step_roofline_frac": plain * STEP_ALG_BYTES / (HBM_PEAK_GBS * 1e9)}}
+    if theta is not None:
+        net = surrogate.Net(Nx, Ny, True, False, max_batch=1)
+        net.set_params(theta)
+        sim.set_surrogate(net, args.learned_K, out_scale, 50.0 * out_scale)
+        s = 3 + steps
+        sim.step(s, 2)
+        s += 2
+        gpu.device_synchronize()
+        log(f"bench: large instance, timing {steps} surrogate steps")
+        c0 = sim.surrogate_counts()
+        t0 = time.perf_counter()
+        sim.step_async(s, steps)
+        sim.synchronize()
+        el = time.perf_counter() - t0
+        sim.check()
+        c1 = sim.surrogate_counts()
+        sim.set_surrogate(None, 0, 1.0, 1.0)
+        net.close()
+        val = p.n * steps / el
+        out["learned"] = {"value": val, "unit": "particle-steps/s", "ms_per_step": 1e3 * el / steps,
+                          "step_roofline_frac": val * STEP_ALG_BYTES / (HBM_PEAK_GBS * 1e9),
+                          "speedup_vs_plain": val / plain, "K": args.learned_K,
+                          "steps_learned": c1["learned"] - c0["learned"],
+                          "steps_fallback": c1["fallback"] - c0["fallback"]}
+    sim.close()
+    return out
 
 
 def main() -> int:
@@ -337,6 +394,8 @@ def main() -> int:
     ap.add_argument("--train-iters", type=int, default=600)
     ap.add_argument("--profile-steps", type=int, default=3000)
     ap.add_argument("--profile-runs", type=int, default=8)
+    ap.add_argument("--no-large", action="store_true", help="skip the 4096x2048 large-instance measurement")
+    ap.add_argument("--large-steps", type=int, default=10)
     args = ap.parse_args()
 
     if args.copy_calibration:
@@ -384,6 +443,8 @@ def main() -> int:
     value = n * args.steps / elapsed
     learned = None if args.no_learned else learned_closure(sim, p, prm, s_next, args, value)
     sim.close()
+    theta = learned.pop("_theta") if learned else None
+    large = None if args.no_large else large_instance(args, theta, learned["out_scale"] if learned else 1.0)
 
     ms_per_step = 1e3 * elapsed / args.steps
     out = {
@@ -415,6 +476,8 @@ def main() -> int:
             for k in ALG_BYTES if per[k]["launches"]}
     if learned:
         out["learned"] = learned
+    if large:
+        out["large_instance"] = large
     out["last_step"] = {"moved_dense": m["base"]["nParticlesMovedDense"] + m["shifted"]["nParticlesMovedDense"],
                         "moved_sparse": m["base"]["nParticlesMovedSparse"] + m["shifted"]["nParticlesMovedSparse"],
                         "beta_applied": m["closure"]["betaRepairApplied"]}

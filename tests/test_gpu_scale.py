@@ -55,3 +55,46 @@ def test_scale_1024x512_steps_stage_by_stage(oracle):
                 assert abs(m["closure"][k] - so["closure"][k]) <= 1e-10 * max(1.0, abs(so["closure"][k])), (s, k)
     finally:
         sim.close()
+
+
+def bits_equal(a: np.ndarray, b: np.ndarray) -> bool:
+    return a.shape == b.shape and np.array_equal(a.view(np.uint64), b.view(np.uint64))
+
+
+@pytest.mark.timeout(900)
+def test_scale_4096x2048_first_step_stage_by_stage(oracle):
+    """The large single instance: 4096 x 2048 cells x 20 particles per cell
+    (167,772,160 particles, 13,448 base zones), its first step stage by
+    stage against the oracle, at the same precision as above."""
+    from src_incompressible_openmp_amd import gpu
+
+    gpu.load()
+    p = to_struct(poiseuille_params(Nx=4096, Ny=2048, gamma=20.0))
+    og, nzg = gpu.build_zone_layout(p, LAYOUT_BASE)
+    oo, nz = oracle.layout(p, LAYOUT_BASE)
+    assert np.array_equal(og, oo) and len(nzg) == nz
+    print(f"large step 1: n={p.n}, {nz} base zones", flush=True)
+    x, v, _, _ = uniform_state(p.n, p.Lx, p.Ly, seed=1)
+    sim = gpu.Sim(p)
+    try:
+        sim.set_state(x, v)
+        sg = sim.step_stages(1)
+        xg, vg = sim.get_state()
+    finally:
+        sim.close()
+    print("  GPU step done", flush=True)
+    t0 = time.perf_counter()
+    so = oracle.step(CTR, p, x, v, 1, stages=True)
+    print(f"  oracle step {time.perf_counter() - t0:.1f} s, moved", so["base"]["nParticlesMovedDense"],
+          so["shifted"]["nParticlesMovedDense"], flush=True)
+    for a in ("ref", "base", "red"):
+        assert bits_equal(sg["x_" + a], so["x_" + a]), a
+        assert bits_equal(sg["v_" + a], so["v_" + a]), a
+    assert bits_equal(xg, x)
+    assert float(np.max(np.abs(vg - v))) <= 1e-12 * max(1.0, float(np.max(np.abs(v))))
+    m = sg["metrics"]
+    for L in ("base", "shifted"):
+        for k in ("nZones", "nParticlesMovedDense", "nParticlesMovedSparse"):
+            assert m[L][k] == so[L][k], (L, k)
+    for k in ("betaRepairApplied", "betaRepairNum", "betaRepairDen"):
+        assert abs(m["closure"][k] - so["closure"][k]) <= 1e-10 * max(1.0, abs(so["closure"][k])), k
