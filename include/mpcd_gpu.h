@@ -368,6 +368,46 @@ int mpcd_gpu_closure_training_pair(const mpcd_params* p, const int32_t* refN, co
                                    const double* refUy, const double* refT, const double* duRepairX,
                                    const double* duRepairY, const double* duEOSX, const double* duEOSY, double beta,
                                    double outScale, float* feat, float* target);
+/* ---- batched ensembles ----
+ * No counterpart in the reference, which runs one channel per process
+ * (main_benchmark_poiseuille_openmp_profiled_shifted_profiled.cpp:186-237)
+ * and its ensembles as separate runs.  An ensemble advances many
+ * independent channels -- replicas with their own parameters (grid, particle
+ * count, forcing, walls, thresholds) and seed keys -- together: every stage
+ * of the step is one launch over all replicas, and the step is captured
+ * into a HIP graph and replayed.  Replica r follows, bit for bit, a single
+ * simulation of its parameters given mpcd_gpu_sim_set_seed_key(key[r]).
+ * Replicas run the full plain step (zones and the liquid closure on;
+ * keepMeanFlow and the surface modes are refused). */
+typedef struct mpcd_gpu_ensemble mpcd_gpu_ensemble;
+
+/* the seeds of every following step of s become the benchmark driver's plus
+ * key * 0x9E3779B97F4A7C15 (key 0: the driver's own seeds, the default) */
+int mpcd_gpu_sim_set_seed_key(mpcd_gpu_sim* s, uint64_t key);
+
+/* params[nReplicas]; seedKeys[nReplicas] or NULL (every key 0) */
+int mpcd_gpu_ensemble_create(const mpcd_params* params, const uint64_t* seedKeys, int nReplicas,
+                             mpcd_gpu_ensemble** out);
+int mpcd_gpu_ensemble_destroy(mpcd_gpu_ensemble* e);
+/* total particles and cells, and the zones of the base and shifted layouts */
+int mpcd_gpu_ensemble_info(mpcd_gpu_ensemble* e, int64_t* particles, int32_t* cells, int32_t* zones2);
+/* replica r's state, interleaved host arrays in particle-id order (as
+ * mpcd_gpu_sim_set_state / _get_state) */
+int mpcd_gpu_ensemble_set_state(mpcd_gpu_ensemble* e, int replica, const double* x, const double* v);
+int mpcd_gpu_ensemble_get_state(mpcd_gpu_ensemble* e, int replica, double* x, double* v);
+/* enqueues steps firstStep .. firstStep+nSteps-1 of every replica on the
+ * ensemble's stream and returns; useGraph: replay one captured graph per
+ * step (captured at the first use) instead of launching each kernel */
+int mpcd_gpu_ensemble_step(mpcd_gpu_ensemble* e, int firstStep, int nSteps, int useGraph);
+/* waits for the enqueued steps; reports a wall-mode or capacity error */
+int mpcd_gpu_ensemble_synchronize(mpcd_gpu_ensemble* e);
+/* flow-rate statistics: with enable, every following step adds each
+ * replica's Qx = Ly * mean cell Ux of the state after the step (the
+ * driver's Qx, :105-155) to qSum and its square to qSumSq (cleared when
+ * enabled); the call first copies the sums so far (any pointer may be
+ * NULL; qSum/qSumSq have nReplicas entries) and the step count */
+int mpcd_gpu_ensemble_flow(mpcd_gpu_ensemble* e, int enable, double* qSum, double* qSumSq, int32_t* nSteps);
+
 /* one 16x16x32 bfloat16 matrix-core product of 64 lanes x 8 operands each
  * (a, b: 512 raw bfloat16 values, d: 256 floats), to check the lane layout */
 int mpcd_gpu_net_mfma_probe(const uint16_t* a, const uint16_t* b, float* d);

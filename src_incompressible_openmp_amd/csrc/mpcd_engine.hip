@@ -991,7 +991,30 @@ struct ZoneArgs {
     size_t gstride;
     ZoneOut* out;
     int* err;
+    /* an ensemble's pass (the kEns kernels): zone -> replica, the replicas'
+     * own inputs (binning views, owner and cover maps, geometry, thresholds),
+     * their zone seed bases and their first zone index in this pass */
+    const int* ensRep = nullptr;
+    const ZoneIn* ensIn = nullptr;
+    const uint64_t* ensSeed = nullptr;
+    const int* ensZoneOff = nullptr;
 };
+
+/* The inputs, zone number (1-based, within its own layout) and seed of zone
+ * zi of a pass: the pass's own, or (kEns) those of the zone's replica. */
+template <bool kEns>
+__device__ __forceinline__ const ZoneIn& zone_in_of(const ZoneArgs& a, int zi, int& zid, uint64_t& seed) {
+    if constexpr (kEns) {
+        const int r = a.ensRep[zi];
+        zid = zi - a.ensZoneOff[r] + 1;
+        seed = a.ensSeed[r] + 104729ULL * (uint64_t)zid;
+        return a.ensIn[r];
+    } else {
+        zid = zi + 1;
+        seed = a.singleSeed ? a.seedBase : a.seedBase + 104729ULL * (uint64_t)(zi + 1);
+        return a.in;
+    }
+}
 
 /* particles of each zone: its columns' runs of the binning list */
 /* One wavefront per zone (block nZ writes the terminating zero): each lane
@@ -1093,12 +1116,14 @@ constexpr int FINISH_TPB = 256;
 constexpr int FINISH_LDS = 40 * 1024; /* cell arrays, column map and as many links as fit */
 /* The correction of zone zi by TPB threads (CPT cells per thread held in
  * registers); lds/ldsBytes: the staging space (kLds). */
-template <class IT, bool kLds, int TPB, int CPT>
+template <class IT, bool kLds, int TPB, int CPT, bool kEns = false>
 __device__ __forceinline__ void zone_finish_body(const ZoneArgs& a, int zi, long long off, int tid,
                                                  unsigned char* lds, long long ldsBytes, int* queue) {
     ZoneCtxT<IT> z;
-    const uint64_t seed = a.singleSeed ? a.seedBase : a.seedBase + 104729ULL * (uint64_t)(zi + 1);
-    zone_init(z, a.in, a.rects[zi], zi + 1, seed, nullptr);
+    int zid;
+    uint64_t seed;
+    const ZoneIn& in = zone_in_of<kEns>(a, zi, zid, seed);
+    zone_init(z, in, a.rects[zi], zid, seed, nullptr);
     z.nxG = (IT*)a.nextG + off;
     z.tmpG = (IT*)a.tmpG + off;
     z.duG = a.duG + (size_t)zi * (size_t)a.duStride;
@@ -1199,7 +1224,7 @@ __global__ void __launch_bounds__(FINISH_TPB) k_zone_finish(ZoneArgs a) {
  * then all run the correction.  A zone's correction starts as
  * soon as its own passes end, beside the passes of slower zones. */
 constexpr int FUSED_TPB = 256;
-template <class IT>
+template <class IT, bool kEns = false>
 __global__ void __launch_bounds__(FUSED_TPB, 4) k_zone_fused(ZoneArgs a) { /* 4 waves per SIMD: four zones per CU */
     extern __shared__ __align__(16) unsigned char lds[];
     __shared__ ZoneShared sh;
@@ -1218,15 +1243,17 @@ __global__ void __launch_bounds__(FUSED_TPB, 4) k_zone_fused(ZoneArgs a) { /* 4 
     __builtin_amdgcn_s_setprio(3);
     const int wave = (int)threadIdx.x >> 6, lane = (int)threadIdx.x & 63;
     ZoneCtxT<IT> z;
-    const uint64_t seed = a.singleSeed ? a.seedBase : a.seedBase + 104729ULL * (uint64_t)(zi + 1);
-    zone_init(z, a.in, a.rects[zi], zi + 1, seed, &sh);
+    int zid;
+    uint64_t seed;
+    const ZoneIn& in = zone_in_of<kEns>(a, zi, zid, seed);
+    zone_init(z, in, a.rects[zi], zid, seed, &sh);
     z.nxG = (IT*)a.nextG + off;
     z.tmpG = (IT*)a.tmpG + off;
     z.duG = a.duG + (size_t)zi * (size_t)a.duStride;
     z.rec = a.rec + off;
     zone_carve(z, lds, a.ldsBytes);
     if (wave == 0) {
-        zone_setup(z, a.in, lane, ZONE_TPB);
+        zone_setup(z, in, lane, ZONE_TPB);
         ZBAR();
     }
     __syncthreads();
@@ -1252,7 +1279,7 @@ __global__ void __launch_bounds__(FUSED_TPB, 4) k_zone_fused(ZoneArgs a) { /* 4 
     }
     __builtin_amdgcn_s_setprio(0);
     __syncthreads();
-    zone_finish_body<IT, true, FUSED_TPB, 3>(a, zi, off, (int)threadIdx.x, lds, a.ldsBytes, queue);
+    zone_finish_body<IT, true, FUSED_TPB, 3, kEns>(a, zi, off, (int)threadIdx.x, lds, a.ldsBytes, queue);
 }
 
 /* velocities of the particles of cells several zones cover, from the side
@@ -1843,6 +1870,9 @@ struct mpcd_gpu_sim {
     int surK = 0;
     double surScale = 1.0, surCap = 0.0;
     bool wantSample = false;
+    /* added to every base seed of a step (0: the benchmark driver's seeds);
+     * a replica of an ensemble with seed key k uses k * SEED_KEY_MIX */
+    uint64_t seedOff = 0;
     /* closure snapshots (training data in the reference's formats): the
      * reference and red CellFields (N, then 9 fields each), the six grids
      * and beta of the last plain closure since they were enabled */
@@ -2502,11 +2532,11 @@ struct mpcd_gpu_sim {
         if (fuse) {
             const uint64_t un = (uint64_t)fuseNext;
             double sh[2];
-            mpcd_draw_shift(130363ULL + 10007ULL * un, p.a0, &sh[0], &sh[1]);
+            mpcd_draw_shift(130363ULL + 10007ULL * un + seedOff, p.a0, &sh[0], &sh[1]);
             hipLaunchKernelGGL(k_fill_int, dim3(nblk(Nc + 1)), dim3(TPB), 0, stream, bin.cntS, Nc + 1, 0);
             hipLaunchKernelGGL(k_fill_int, dim3(nblk(Nc + 1)), dim3(TPB), 0, stream, bin.cnt, Nc + 1, 0);
             hipLaunchKernelGGL(k_kick_reorder_stream, dim3(nblk(n)), dim3(TPB), 0, stream, g, bc,
-                               104729ULL + 10007ULL * un, bin.list, x, y, vx, vy, pid, x2, y2, vx2, vy2, pid2, duRX,
+                               104729ULL + 10007ULL * un + seedOff, bin.list, x, y, vx, vy, pid, x2, y2, vx2, vy2, pid2, duRX,
                                duRY, duEX, duEY, scal, cidS, cid, sh[0], sh[1], xmax, ymax, bin.cntS, bin.cnt, err);
             streamed = fuseNext;
         } else {
@@ -2759,9 +2789,9 @@ struct mpcd_gpu_sim {
         } else {
             if (streamed >= 0) throw ApiError(MPCD_E_INVALID, "internal: a fused stream was left unused");
             double sh[2];
-            mpcd_draw_shift(130363ULL + 10007ULL * us, p.a0, &sh[0], &sh[1]);
+            mpcd_draw_shift(130363ULL + 10007ULL * us + seedOff, p.a0, &sh[0], &sh[1]);
             pbeg(MPCD_PROF_STREAM_BC);
-            stream_bc(104729ULL + 10007ULL * us, 1, sh, sm ? &sm->wall : nullptr);
+            stream_bc(104729ULL + 10007ULL * us + seedOff, 1, sh, sm ? &sm->wall : nullptr);
             pend(MPCD_PROF_STREAM_BC);
         }
         /* after a fused stream both histograms are done (the previous kick) */
@@ -2769,7 +2799,7 @@ struct mpcd_gpu_sim {
         bin_cells(cidS, bin.cntS, pre, false);
         pend(MPCD_PROF_BIN);
         pbeg(MPCD_PROF_COLLIDE);
-        collide(433494437ULL + 10007ULL * us);
+        collide(433494437ULL + 10007ULL * us + seedOff);
         pend(MPCD_PROF_COLLIDE);
         pbeg(MPCD_PROF_BIN);
         bin_cells(cid, bin.cnt, pre, true); /* collision does not move particles */
@@ -2779,7 +2809,7 @@ struct mpcd_gpu_sim {
         ref_fields();
         pend(MPCD_PROF_CELL_FIELDS);
         phase_mark(1);
-        const uint64_t zs = 88172645463325252ULL + 10007ULL * us;
+        const uint64_t zs = 88172645463325252ULL + 10007ULL * us + seedOff;
         pbeg(MPCD_PROF_ZONE_PASS);
         /* the base pass's input is the state the reference fields were
          * summed over (same runs, same order): the correction takes its
@@ -3574,3 +3604,6 @@ int mpcd_gpu_liquid_closure(const mpcd_params* p, const double* xRef, const doub
 }
 
 }  // extern "C"
+
+/* batched ensembles (kernels and the C ABI) */
+#include "mpcd_ensemble.h"
