@@ -373,6 +373,67 @@ def large_instance(args, theta, out_scale):
     return out
 
 
+def ensemble_params(R: int, seed: int = 2):
+    """R replicas of the Poiseuille channel: Nx uniform in 64..512, Ny in
+    32..256, cell size 0.2, particles per cell in 5..20, body force
+    log-uniform in [1e-4, 1e-2]; seed key r + 1 for replica r."""
+    rng = np.random.default_rng(seed)
+    prms = []
+    for _ in range(R):
+        nx, ny = int(rng.integers(64, 513)), int(rng.integers(32, 257))
+        ppc = int(rng.integers(5, 21))
+        gx = float(10.0 ** rng.uniform(-4.0, -2.0))
+        prms.append(poiseuille_params(Nx=nx, Ny=ny, gamma=float(ppc), bodyForceX=gx))
+    return prms, [r + 1 for r in range(R)]
+
+
+def ensemble_leg(args):
+    """The batched ensemble: args.ensemble_replicas uneven channels, each
+    stage one launch over all replicas, the step replayed as a HIP graph;
+    eager launches timed beside it.  Flow rates: each replica's Qx over the
+    timed steps, its mean and the relative standard error of the mean (the
+    channels start from rest, so the means follow the developing flow)."""
+    R = args.ensemble_replicas
+    prms, keys = ensemble_params(R)
+    ps = [to_struct(q) for q in prms]
+    n = sum(p.n for p in ps)
+    log(f"bench: ensemble of {R} replicas, {n} particles; building states")
+    ens = gpu.Ensemble(ps, keys)
+    for r, p in enumerate(ps):
+        x, v, _, _ = uniform_state(p.n, p.Lx, p.Ly, seed=1000 + r)
+        ens.set_state(r, x, v)
+    del x, v
+    ens.step(1, 2, graph=True)
+    steps = args.ensemble_steps
+    log(f"bench: ensemble, timing {steps} graph-replayed steps")
+    ens.flow(True)
+    t0 = time.perf_counter()
+    ens.step(3, steps, graph=True, wait=False)
+    ens.synchronize()
+    el = time.perf_counter() - t0
+    q, q2, ns = ens.flow(False, fetch=True)
+    k = max(3, steps // 4)
+    log(f"bench: ensemble, timing {k} eagerly launched steps")
+    t0 = time.perf_counter()
+    ens.step(3 + steps, k, graph=False, wait=False)
+    ens.synchronize()
+    el_eager = time.perf_counter() - t0
+    info = ens.info()
+    ens.close()
+    mean = q / ns
+    var = np.maximum(q2 / ns - mean * mean, 0.0)
+    rel_se = np.sqrt(var / max(ns - 1, 1)) / np.maximum(np.abs(mean), 1e-300)
+    return {"workload": f"ensemble_{R}_poiseuille_uneven", "replicas": R, "particles": n, "cells": info["cells"],
+            "zones": info["zones"], "steps": steps, "value": n * steps / el, "unit": "particle-steps/s",
+            "ms_per_step": 1e3 * el / steps, "step_roofline_frac": n * steps / el * STEP_ALG_BYTES / (HBM_PEAK_GBS * 1e9),
+            "eager_ms_per_step": 1e3 * el_eager / k, "graph_speedup": (el_eager / k) / (el / steps),
+            "flow": {"qx_mean_min": float(mean.min()), "qx_mean_max": float(mean.max()),
+                     "ensemble_mean_rel_stderr": float(rel_se.mean()), "steps": ns},
+            "grid_ranges": {"Nx": [min(p.Nx for p in ps), max(p.Nx for p in ps)],
+                            "Ny": [min(p.Ny for p in ps), max(p.Ny for p in ps)],
+                            "ppc": [min(int(p.gamma) for p in ps), max(int(p.gamma) for p in ps)]}}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
@@ -396,6 +457,11 @@ def main() -> int:
     ap.add_argument("--profile-runs", type=int, default=8)
     ap.add_argument("--no-large", action="store_true", help="skip the 4096x2048 large-instance measurement")
     ap.add_argument("--large-steps", type=int, default=10)
+    ap.add_argument("--no-ensemble", action="store_true", help="skip the batched-ensemble measurement")
+    ap.add_argument("--ensemble-replicas", type=int, default=256)
+    ap.add_argument("--ensemble-steps", type=int, default=20)
+    ap.add_argument("--ensemble-only", action="store_true",
+                    help="only the batched-ensemble measurement (its JSON object; for profiling)")
     args = ap.parse_args()
 
     if args.copy_calibration:
@@ -407,6 +473,10 @@ def main() -> int:
             b.copy_(a)
         torch.cuda.synchronize()
         log("bench: copy calibration done (5 x 83,886,080 bytes)")
+        return 0
+
+    if args.ensemble_only:
+        print(json.dumps({"ensemble": ensemble_leg(args)}), flush=True)
         return 0
 
     if args.gpus != 1:
@@ -445,6 +515,7 @@ def main() -> int:
     sim.close()
     theta = learned.pop("_theta") if learned else None
     large = None if args.no_large else large_instance(args, theta, learned["out_scale"] if learned else 1.0)
+    ensemble = None if args.no_ensemble else ensemble_leg(args)
 
     ms_per_step = 1e3 * elapsed / args.steps
     out = {
@@ -478,6 +549,8 @@ def main() -> int:
         out["learned"] = learned
     if large:
         out["large_instance"] = large
+    if ensemble:
+        out["ensemble"] = ensemble
     out["last_step"] = {"moved_dense": m["base"]["nParticlesMovedDense"] + m["shifted"]["nParticlesMovedDense"],
                         "moved_sparse": m["base"]["nParticlesMovedSparse"] + m["shifted"]["nParticlesMovedSparse"],
                         "beta_applied": m["closure"]["betaRepairApplied"]}

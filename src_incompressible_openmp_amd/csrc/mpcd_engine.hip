@@ -1116,13 +1116,11 @@ constexpr int FINISH_TPB = 256;
 constexpr int FINISH_LDS = 40 * 1024; /* cell arrays, column map and as many links as fit */
 /* The correction of zone zi by TPB threads (CPT cells per thread held in
  * registers); lds/ldsBytes: the staging space (kLds). */
-template <class IT, bool kLds, int TPB, int CPT, bool kEns = false>
-__device__ __forceinline__ void zone_finish_body(const ZoneArgs& a, int zi, long long off, int tid,
-                                                 unsigned char* lds, long long ldsBytes, int* queue) {
+template <class IT, bool kLds, int TPB, int CPT>
+__device__ __forceinline__ void zone_finish_body(const ZoneArgs& a, const ZoneIn& in, int zid, uint64_t seed, int zi,
+                                                 long long off, int tid, unsigned char* lds, long long ldsBytes,
+                                                 int* queue) {
     ZoneCtxT<IT> z;
-    int zid;
-    uint64_t seed;
-    const ZoneIn& in = zone_in_of<kEns>(a, zi, zid, seed);
     zone_init(z, in, a.rects[zi], zid, seed, nullptr);
     z.nxG = (IT*)a.nextG + off;
     z.tmpG = (IT*)a.tmpG + off;
@@ -1213,7 +1211,10 @@ __global__ void __launch_bounds__(FINISH_TPB) k_zone_finish(ZoneArgs a) {
         const int P = a.zsize[zi];
         const long long off = a.zoff[zi];
         if (off + (long long)P > a.scratchCap) continue;
-        zone_finish_body<IT, kLds, FINISH_TPB, 4>(a, zi, off, (int)threadIdx.x, lds, FINISH_LDS, queue);
+        int zid;
+        uint64_t seed;
+        const ZoneIn& in = zone_in_of<false>(a, zi, zid, seed);
+        zone_finish_body<IT, kLds, FINISH_TPB, 4>(a, in, zid, seed, zi, off, (int)threadIdx.x, lds, FINISH_LDS, queue);
         __syncthreads(); /* the next zone reuses the staging */
     }
 }
@@ -1245,7 +1246,20 @@ __global__ void __launch_bounds__(FUSED_TPB, 4) k_zone_fused(ZoneArgs a) { /* 4 
     ZoneCtxT<IT> z;
     int zid;
     uint64_t seed;
-    const ZoneIn& in = zone_in_of<kEns>(a, zi, zid, seed);
+    const ZoneIn& inSrc = zone_in_of<kEns>(a, zi, zid, seed);
+    /* an ensemble zone's inputs (its replica's) are staged in LDS, read as
+     * cheaply as the kernel arguments */
+    const ZoneIn* inp = &inSrc;
+    if constexpr (kEns) {
+        __shared__ __align__(16) uint32_t sIn[sizeof(ZoneIn) / sizeof(uint32_t)];
+        constexpr int W = (int)(sizeof(ZoneIn) / sizeof(uint32_t));
+        static_assert(sizeof(ZoneIn) % sizeof(uint32_t) == 0, "ZoneIn is copied as 32-bit words");
+        const uint32_t* src = reinterpret_cast<const uint32_t*>(&inSrc);
+        for (int k = (int)threadIdx.x; k < W; k += FUSED_TPB) sIn[k] = src[k];
+        __syncthreads();
+        inp = reinterpret_cast<const ZoneIn*>(sIn);
+    }
+    const ZoneIn& in = *inp;
     zone_init(z, in, a.rects[zi], zid, seed, &sh);
     z.nxG = (IT*)a.nextG + off;
     z.tmpG = (IT*)a.tmpG + off;
@@ -1279,7 +1293,7 @@ __global__ void __launch_bounds__(FUSED_TPB, 4) k_zone_fused(ZoneArgs a) { /* 4 
     }
     __builtin_amdgcn_s_setprio(0);
     __syncthreads();
-    zone_finish_body<IT, true, FUSED_TPB, 3, kEns>(a, zi, off, (int)threadIdx.x, lds, a.ldsBytes, queue);
+    zone_finish_body<IT, true, FUSED_TPB, 3>(a, in, zid, seed, zi, off, (int)threadIdx.x, lds, a.ldsBytes, queue);
 }
 
 /* velocities of the particles of cells several zones cover, from the side
