@@ -119,17 +119,6 @@ __global__ void __launch_bounds__(TPB) k_ens_stream(const EnsRep* __restrict__ r
     cidU[i] = e.coff + cell_of(e.g, px, py);
 }
 
-/* global unshifted cell of every particle (the binning after a zone pass) */
-__global__ void __launch_bounds__(TPB) k_ens_cells(const EnsRep* __restrict__ reps, const int* __restrict__ poff,
-                                                   int R, int N, const double* __restrict__ x,
-                                                   const double* __restrict__ y, int* __restrict__ cid) {
-    __shared__ int sh;
-    const int b0 = blockIdx.x * TPB, i = b0 + (int)threadIdx.x;
-    const int r = ens_rep(poff, R, i < N ? i : N - 1, b0, min(TPB, N - b0), &sh);
-    if (i >= N) return;
-    cid[i] = reps[r].coff + cell_of(reps[r].g, x[i], y[i]);
-}
-
 /* SRD collision of every cell (k_collide per replica: the cell number in the
  * draws is the replica's own) */
 __global__ void __launch_bounds__(CELL_BLK) k_ens_collide(const EnsRep* __restrict__ reps,
@@ -204,7 +193,7 @@ __global__ void __launch_bounds__(CELL_BLK) k_ens_cell_fields(const EnsRep* __re
 }
 
 /* particles of each zone of every replica (k_zone_sizes on the replica's
- * cells); a zone too large for the 16-bit working set is flagged (bit 4) */
+ * cells); a zone too large for the 16-bit working set is flagged (bit 8) */
 __global__ void __launch_bounds__(64) k_ens_zone_sizes(const ZoneRect* __restrict__ rects,
                                                        const int* __restrict__ zrep,
                                                        const EnsRep* __restrict__ reps, int nZ,
@@ -225,7 +214,7 @@ __global__ void __launch_bounds__(64) k_ens_zone_sizes(const ZoneRect* __restric
     for (int o = 32; o > 0; o >>= 1) P += __shfl_xor(P, o, 64);
     if (lane == 0) {
         sizes[zi] = P;
-        if (P > 65534) atomicOr(err, 4);
+        if (P > 65534) atomicOr(err, 8);
     }
 }
 
@@ -429,9 +418,16 @@ struct mpcd_gpu_ensemble {
     double *x2 = nullptr, *y2 = nullptr, *vx2 = nullptr, *vy2 = nullptr;
     int *pid = nullptr, *pid2 = nullptr, *cidS = nullptr, *cidU = nullptr;
     int parity = 0; /* 0: the state is in x..; 1: the buffers are swapped */
-    /* binning */
+    /* binning: set A (start, key, list) is current when a step begins; the
+     * update after the base pass writes set B (startB, keyB, listB) and swaps,
+     * the one after the shifted pass swaps back */
     int *cnt = nullptr, *start = nullptr, *cursor = nullptr, *list = nullptr, *tileSum = nullptr;
     unsigned long long* key = nullptr;
+    int *startB = nullptr, *listB = nullptr;
+    unsigned long long* keyB = nullptr;
+    /* the passes' record of moved particles: departures and arrivals per cell,
+     * each arrival's rank (-1 for every slot between updates) */
+    int *dout = nullptr, *inc = nullptr, *rnk = nullptr;
     int nTiles = 0, narrow = 1, cellCap = 256;
     /* cells */
     double *refPx = nullptr, *refPy = nullptr, *refUx = nullptr, *refUy = nullptr, *pdriveRaw = nullptr;
@@ -570,9 +566,10 @@ struct mpcd_gpu_ensemble {
         HIP_OK(hipMemcpy(boff, hb.data(), hb.size() * sizeof(int), hipMemcpyHostToDevice));
         /* state and binning */
         for (double** q : {&x, &y, &vx, &vy, &x2, &y2, &vx2, &vy2}) *q = dalloc<double>((size_t)N);
-        for (int** q : {&pid, &pid2, &cidS, &cidU, &list}) *q = dalloc<int>((size_t)N);
+        for (int** q : {&pid, &pid2, &cidS, &cidU, &list, &listB, &rnk}) *q = dalloc<int>((size_t)N);
         key = dalloc<unsigned long long>((size_t)N);
-        for (int** q : {&cnt, &start, &cursor}) *q = dalloc<int>((size_t)C + 1);
+        keyB = dalloc<unsigned long long>((size_t)N);
+        for (int** q : {&cnt, &start, &cursor, &startB, &dout, &inc}) *q = dalloc<int>((size_t)C + 1);
         nTiles = (int)((std::max(C, std::max(Z[0], Z[1])) + 1 + SCAN_TILE - 1) / SCAN_TILE);
         tileSum = dalloc<int>((size_t)nTiles);
         cellCap = (int)std::min(4096.0, std::max(256.0, 1.15 * maxPpc * CELL_BLK + 2.0 * CELL_BLK));
@@ -592,6 +589,9 @@ struct mpcd_gpu_ensemble {
         err = dalloc<int>(1);
         hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, stream, err, 1, 0);
         hipLaunchKernelGGL(k_fill_double, dim3(nblk(2 * (size_t)R)), dim3(TPB), 0, stream, flowAcc, 2 * R, 0.0);
+        hipLaunchKernelGGL(k_fill_int, dim3(nblk(N)), dim3(TPB), 0, stream, rnk, N, -1);
+        hipLaunchKernelGGL(k_fill_int, dim3(nblk(C + 1)), dim3(TPB), 0, stream, dout, C + 1, 0);
+        hipLaunchKernelGGL(k_fill_int, dim3(nblk(C + 1)), dim3(TPB), 0, stream, inc, C + 1, 0);
         check_launch();
         /* interzone masks */
         {
@@ -655,9 +655,14 @@ struct mpcd_gpu_ensemble {
                     q.g = e.g;
                     q.zp = make_zone_params(prm[(size_t)r]);
                     q.gcnt = cnt + e.coff;
-                    q.gstart = start + e.coff;
-                    q.glist = list;
-                    q.gkey = key;
+                    q.gstart = (L == 0 ? start : startB) + e.coff;
+                    q.glist = L == 0 ? list : listB;
+                    q.gkey = L == 0 ? key : keyB;
+                    q.dcid = cidU; /* global cells: the recorded ones get cellOff */
+                    q.dout = dout;
+                    q.dinc = inc;
+                    q.drnk = rnk;
+                    q.cellOff = e.coff;
                     q.owner = owner[L] + e.coff;
                     q.gcover = hasCover[(size_t)r] ? cover[L] + e.coff : nullptr;
                     q.x = par ? x2 : x;
@@ -689,7 +694,7 @@ struct mpcd_gpu_ensemble {
         for (hipGraphExec_t& g : graph)
             if (g) (void)hipGraphExecDestroy(g);
         void* ptrs[] = {reps, poff, coff, boff, x, y, vx, vy, x2, y2, vx2, vy2, pid, pid2, cidS, cidU, cnt, start,
-                        cursor, list, tileSum, key, refPx, refPy, refUx, refUy, pdriveRaw, redPx, redPy, redUx, redUy,
+                        cursor, list, tileSum, key, startB, listB, keyB, dout, inc, rnk, refPx, refPy, refUx, refUy, pdriveRaw, redPx, redPy, redUx, redUy,
                         duRX, duRY, duEX, duEY, pdrive, mask, part, scal, rects[0], rects[1], zrep[0], zrep[1],
                         zoneOff[0], zoneOff[1], owner[0], owner[1], cover[0], cover[1], overCells[0], overCells[1],
                         ensIn[0][0], ensIn[0][1], ensIn[1][0], ensIn[1][1], zsize, zoff, next, tmpg, recG, zstate,
@@ -785,10 +790,24 @@ struct mpcd_gpu_ensemble {
         }
     }
 
-    void rebin() {
-        hipLaunchKernelGGL(k_ens_cells, dim3(nblk(N)), dim3(TPB), 0, stream, reps, poff, R, N, x, y, cidU);
+    /* the unshifted binning after a zone pass from its recorded moves
+     * (Sim::rebin_recorded): the runs are the stayers merged with the
+     * arrivals in pid order, what a counting sort of the new cells gives */
+    void rebin_recorded() {
+        const int nt = (C + 1 + SCAN_TILE - 1) / SCAN_TILE;
+        hipLaunchKernelGGL(k_scan_tiles, dim3(nt), dim3(TPB), 0, stream, cnt, C + 1, startB, tileSum, inc,
+                           (int*)nullptr, dout);
+        hipLaunchKernelGGL(k_scan_add_prefix, dim3(nblk(C + 1)), dim3(TPB), 0, stream, startB, C + 1, tileSum);
+        hipLaunchKernelGGL(k_place_arrivals, dim3(nblk((N + PLACE_PER - 1) / PLACE_PER)), dim3(TPB), 0, stream, cidU,
+                           pid, rnk, N, startB, cnt, dout, keyB);
+        int cpb = REB_CELLS;
+        while (cpb > 64 && (double)cpb * (double)N / (double)C * 1.25 > (double)REB_CAP) cpb >>= 1;
+        hipLaunchKernelGGL(k_rebin_merge, dim3(nblk(C, cpb)), dim3(REB_CELLS), 0, stream, start, key, cidU, C, cpb,
+                           startB, cnt, inc, dout, rnk, keyB, listB, err);
         check_launch();
-        bin(cidU);
+        std::swap(start, startB);
+        std::swap(key, keyB);
+        std::swap(list, listB);
     }
 
     /* one step of every replica, as a fixed launch sequence (the step number
@@ -806,9 +825,9 @@ struct mpcd_gpu_ensemble {
         bin(cidU); /* the collision moves no particle */
         fields(refUx, refUy, refPx, refPy, pdriveRaw);
         zone_pass(0);
-        rebin();
+        rebin_recorded();
         zone_pass(1);
-        rebin();
+        rebin_recorded();
         fields(redUx, redUy, redPx, redPy, nullptr);
         ClosureArgs ca{};
         ca.Nc = C;
@@ -875,7 +894,8 @@ struct mpcd_gpu_ensemble {
             hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, stream, err, 1, 0);
             HIP_OK(hipStreamSynchronize(stream));
             if (h & 1) throw ApiError(MPCD_E_BC_MODE, "a particle hit a wall of unsupported mode");
-            if (h & 4) throw ApiError(MPCD_E_CAPACITY, "a zone holds more particles than the 16-bit working set");
+            if (h & 8) throw ApiError(MPCD_E_CAPACITY, "a zone holds more particles than the 16-bit working set");
+            if (h & 4) throw ApiError(MPCD_E_INVALID, "binning update: a cell's stayers do not match its count");
             throw ApiError(MPCD_E_CAPACITY, "zone scratch exceeded");
         }
     }

@@ -202,6 +202,9 @@ struct ZoneIn {
      * from */
     double* dgCell = nullptr;
     int* dgCnt = nullptr;
+    /* added to the recorded new cells (an ensemble replica's first cell in
+     * the concatenated grid; 0 for a single simulation) */
+    int cellOff = 0;
 };
 
 /* Scalars the lanes share (LDS on the device). */
@@ -226,6 +229,7 @@ struct ZoneCtxT {
     double *x, *y, *vx, *vy, *vsx, *vsy;
     const double *inPx, *inPy; /* input run sums per global cell, or nullptr */
     int *dcid, *dout, *dinc, *drnk; /* cell-change record, or nullptr */
+    int cellOff;    /* added to the recorded new cells */
     double* dgCell; /* diagnostics' per-cell record, or nullptr */
     int* dgCnt;     /* diagnostics' sparse-phase start counts, or nullptr */
     IT* nxL;     /* links of k < nL */
@@ -631,7 +635,7 @@ ZHD void place_moved(Z& z, int lc, int slot, uint32_t pid, int tag) {
     if (z.dcid) {
         /* the cell the binning update will see (a particle is placed once
          * per pass); the pass's own counts stay untouched */
-        const int cn = cell_of(z.g, nx, ny), co = z.dcid[slot];
+        const int cn = cell_of(z.g, nx, ny) + z.cellOff, co = z.dcid[slot];
         if (cn != co) {
             z.dcid[slot] = cn;
 #if defined(__HIP_DEVICE_COMPILE__)
@@ -2185,6 +2189,7 @@ ZHD void zone_init(ZoneCtxT<IT>& z, const ZoneIn& in, ZoneRect r, int zid, uint6
     z.dout = in.dout;
     z.dinc = in.dinc;
     z.drnk = in.drnk;
+    z.cellOff = in.cellOff;
     z.dgCell = in.dgCell;
     z.dgCnt = in.dgCnt;
     z.x = in.x;
