@@ -35,8 +35,9 @@ def replicas():
             case_params("override"),
             bcmix,
             poiseuille_params(Nx=50, Ny=50, gamma=10.0),
-            poiseuille_params(Nx=37, Ny=21, gamma=13.0, bodyForceX=2e-4, zoneTileNx=9, zoneTileNy=7)]
-    keys = [0, 7, 123, 5, 99, 2 ** 40 + 3]
+            poiseuille_params(Nx=37, Ny=21, gamma=13.0, bodyForceX=2e-4, zoneTileNx=9, zoneTileNy=7),
+            poiseuille_params(Nx=160, Ny=90, gamma=7.0, bodyForceX=5e-3)]  # 28 base zones of the default tile
+    keys = [0, 7, 123, 5, 99, 2 ** 40 + 3, 31]
     return [to_struct(p) for p in prms], keys
 
 
