@@ -390,9 +390,10 @@ def ensemble_params(R: int, seed: int = 2):
 def ensemble_leg(args):
     """The batched ensemble: args.ensemble_replicas uneven channels, each
     stage one launch over all replicas, the step replayed as a HIP graph;
-    eager launches timed beside it.  Flow rates: each replica's Qx over the
-    timed steps, its mean and the relative standard error of the mean (the
-    channels start from rest, so the means follow the developing flow)."""
+    eager launches timed beside it (each kick but a call's last streams the
+    next step).  Flow rates: each replica's Qx over as many further steps,
+    its mean and the relative standard error of the mean (the channels
+    start from rest, so the means follow the developing flow)."""
     R = args.ensemble_replicas
     prms, keys = ensemble_params(R)
     ps = [to_struct(q) for q in prms]
@@ -406,18 +407,20 @@ def ensemble_leg(args):
     ens.step(1, 2, graph=True)
     steps = args.ensemble_steps
     log(f"bench: ensemble, timing {steps} graph-replayed steps")
-    ens.flow(True)
     t0 = time.perf_counter()
     ens.step(3, steps, graph=True, wait=False)
     ens.synchronize()
     el = time.perf_counter() - t0
-    q, q2, ns = ens.flow(False, fetch=True)
     k = max(3, steps // 4)
     log(f"bench: ensemble, timing {k} eagerly launched steps")
     t0 = time.perf_counter()
     ens.step(3 + steps, k, graph=False, wait=False)
     ens.synchronize()
     el_eager = time.perf_counter() - t0
+    log(f"bench: ensemble, flow rates over {steps} more steps")
+    ens.flow(True)
+    ens.step(3 + steps + k, steps, graph=True)
+    q, q2, ns = ens.flow(False, fetch=True)
     info = ens.info()
     ens.close()
     mean = q / ns

@@ -77,6 +77,11 @@ __global__ void k_ens_seeds(const EnsRep* __restrict__ reps, int R, const long l
     mpcd_draw_shift(130363ULL + 10007ULL * us + off, reps[r].g.a0, &sx, &sy);
     shift[2 * r] = sx;
     shift[2 * r + 1] = sy;
+    /* the next step's shift and wall seed, for a kick that streams it */
+    mpcd_draw_shift(130363ULL + 10007ULL * (us + 1ULL) + off, reps[r].g.a0, &sx, &sy);
+    shift[2 * (size_t)R + 2 * r] = sx;
+    shift[2 * (size_t)R + 2 * r + 1] = sy;
+    seeds[2 * (size_t)R + r] = 104729ULL + 10007ULL * (us + 1ULL) + off;
     const uint64_t zs = 88172645463325252ULL + 10007ULL * us + off;
     zseed[r] = zs;
     zseed[R + r] = zs + 4099ULL;
@@ -354,6 +359,53 @@ __global__ void __launch_bounds__(TPB) k_ens_kick_reorder(
     pido[s] = pid[i];
 }
 
+/* the kick and reorder fused with the next step's force, stream and wall
+ * conditions (k_kick_reorder_stream per replica): the state is written
+ * once, already streamed, with both global cell ids of the next step and
+ * their histograms (cntS, cntU, cleared before) */
+__global__ void __launch_bounds__(TPB) k_ens_kick_reorder_stream(
+    const EnsRep* __restrict__ reps, const int* __restrict__ poff, int R, int N, const uint64_t* __restrict__ bcSeedN,
+    const double* __restrict__ shiftN, const int* __restrict__ list, const double* __restrict__ x,
+    const double* __restrict__ y, const double* __restrict__ vx, const double* __restrict__ vy,
+    const int* __restrict__ pid, double* __restrict__ xo, double* __restrict__ yo, double* __restrict__ vxo,
+    double* __restrict__ vyo, int* __restrict__ pido, const double* __restrict__ duRX,
+    const double* __restrict__ duRY, const double* __restrict__ duEX, const double* __restrict__ duEY,
+    const double* __restrict__ scal, int* __restrict__ cidS, int* __restrict__ cidU, int* __restrict__ cntS,
+    int* __restrict__ cntU, int* __restrict__ err) {
+    __shared__ int sh;
+    const int b0 = blockIdx.x * TPB, s = b0 + (int)threadIdx.x;
+    const int r = ens_rep(poff, R, s < N ? s : N - 1, b0, min(TPB, N - b0), &sh);
+    if (s >= N) return;
+    const int lane = __lane_id();
+    const EnsRep& e = reps[r];
+    const double beta = scal[4 * (size_t)r + 3];
+    const int i = list[s];
+    double px = x[i], py = y[i];
+    const int c = e.coff + cell_of(e.g, px, py);
+    const double kx = beta * duRX[c], ky = beta * duRY[c];
+    double nx, ny;
+    kick_velocity(vx[i], vy[i], kx, ky, duEX[c], duEY[c], nx, ny);
+    const int pi = pid[i];
+    WallAcc acc;
+    wall_acc_zero(acc);
+    const int rc = bc_particle(e.g, e.b, bcSeedN[r], (uint32_t)pi, 1, px, py, nx, ny, acc);
+    if (rc != 0) atomicOr(err, 1);
+    xo[s] = px;
+    yo[s] = py;
+    vxo[s] = nx;
+    vyo[s] = ny;
+    pido[s] = pi;
+    const int cs = e.coff + shifted_cell_of(e.g, px, py, shiftN[2 * r], shiftN[2 * r + 1], e.xmax, e.ymax);
+    const int cu = e.coff + cell_of(e.g, px, py);
+    cidS[s] = cs;
+    cidU[s] = cu;
+    int leader, rank, size;
+    wave_groups(cs, true, lane, leader, rank, size);
+    if (lane == leader) atomicAdd(&cntS[cs], size);
+    wave_groups(cu, true, lane, leader, rank, size);
+    if (lane == leader) atomicAdd(&cntU[cu], size);
+}
+
 /* flow rate after the step: per block of a replica's cells (aligned as in
  * the repair) the sum of the cells' mean x velocity, the state being in
  * cell order (cell c holds slots start[c] .. start[c+1]) */
@@ -465,7 +517,12 @@ struct mpcd_gpu_ensemble {
     /* staging of one replica's interleaved state */
     double* stage = nullptr;
     /* graphs of one step per parity */
-    hipGraphExec_t graph[2] = {nullptr, nullptr};
+    /* graphs of one step per buffer parity, per form: [parity][pre][fuse]
+     * (pre: the previous kick streamed this step; fuse: this kick streams
+     * the next) */
+    hipGraphExec_t graph[2][2][2] = {};
+    bool streamedNext = false; /* the state is already streamed for the next step */
+    int* cntS = nullptr;       /* the shifted cells' histogram a fused kick counts */
 
     void check_launch() { HIP_OK(hipGetLastError()); }
 
@@ -583,9 +640,10 @@ struct mpcd_gpu_ensemble {
         flowAcc = dalloc<double>(2 * (size_t)R);
         stage = dalloc<double>(2 * (size_t)maxN);
         stepDev = dalloc<long long>(1);
-        shift = dalloc<double>(2 * (size_t)R);
+        shift = dalloc<double>(4 * (size_t)R);  /* this step's, then the next step's */
         zseed = dalloc<uint64_t>(2 * (size_t)R);
-        seeds = dalloc<uint64_t>(2 * (size_t)R);
+        seeds = dalloc<uint64_t>(3 * (size_t)R); /* wall and rotation per replica, then the next wall seeds */
+        cntS = dalloc<int>((size_t)C + 1);
         err = dalloc<int>(1);
         hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, stream, err, 1, 0);
         hipLaunchKernelGGL(k_fill_double, dim3(nblk(2 * (size_t)R)), dim3(TPB), 0, stream, flowAcc, 2 * R, 0.0);
@@ -691,25 +749,29 @@ struct mpcd_gpu_ensemble {
     }
 
     ~mpcd_gpu_ensemble() {
-        for (hipGraphExec_t& g : graph)
-            if (g) (void)hipGraphExecDestroy(g);
+        for (auto& a : graph)
+            for (auto& b : a)
+                for (hipGraphExec_t& g : b)
+                    if (g) (void)hipGraphExecDestroy(g);
         void* ptrs[] = {reps, poff, coff, boff, x, y, vx, vy, x2, y2, vx2, vy2, pid, pid2, cidS, cidU, cnt, start,
                         cursor, list, tileSum, key, startB, listB, keyB, dout, inc, rnk, refPx, refPy, refUx, refUy, pdriveRaw, redPx, redPy, redUx, redUy,
                         duRX, duRY, duEX, duEY, pdrive, mask, part, scal, rects[0], rects[1], zrep[0], zrep[1],
                         zoneOff[0], zoneOff[1], owner[0], owner[1], cover[0], cover[1], overCells[0], overCells[1],
                         ensIn[0][0], ensIn[0][1], ensIn[1][0], ensIn[1][1], zsize, zoff, next, tmpg, recG, zstate,
-                        duG, zout, stepDev, shift, zseed, seeds, err, flowPart, flowAcc, stage};
+                        duG, zout, stepDev, shift, zseed, seeds, err, flowPart, flowAcc, stage, cntS};
         for (void* q : ptrs)
             if (q) (void)hipFree(q);
         if (stream) (void)hipStreamDestroy(stream);
     }
 
     void drop_graphs() {
-        for (hipGraphExec_t& g : graph)
-            if (g) {
-                HIP_OK(hipGraphExecDestroy(g));
-                g = nullptr;
-            }
+        for (auto& a : graph)
+            for (auto& b : a)
+                for (hipGraphExec_t& g : b)
+                    if (g) {
+                        HIP_OK(hipGraphExecDestroy(g));
+                        g = nullptr;
+                    }
     }
 
     void swap_sets() {
@@ -722,11 +784,15 @@ struct mpcd_gpu_ensemble {
     }
 
     /* counting sort of the global cell ids `keyc` (Sim::bin_cells) */
-    void bin(const int* keyc) {
-        hipLaunchKernelGGL(k_fill_int, dim3(nblk(C + 1)), dim3(TPB), 0, stream, cnt, C + 1, 0);
-        hipLaunchKernelGGL(k_hist, dim3(bin_blocks(N)), dim3(TPB), 0, stream, keyc, N, cnt);
+    void bin(const int* keyc, int* counts = nullptr) {
+        /* counts: the histogram of keyc, already counted (by a fused kick) */
+        int* h = counts ? counts : cnt;
+        if (!counts) {
+            hipLaunchKernelGGL(k_fill_int, dim3(nblk(C + 1)), dim3(TPB), 0, stream, cnt, C + 1, 0);
+            hipLaunchKernelGGL(k_hist, dim3(bin_blocks(N)), dim3(TPB), 0, stream, keyc, N, cnt);
+        }
         const int nt = (C + 1 + SCAN_TILE - 1) / SCAN_TILE;
-        hipLaunchKernelGGL(k_scan_tiles, dim3(nt), dim3(TPB), 0, stream, cnt, C + 1, start, tileSum,
+        hipLaunchKernelGGL(k_scan_tiles, dim3(nt), dim3(TPB), 0, stream, h, C + 1, start, tileSum,
                            (const int*)nullptr, cursor);
         hipLaunchKernelGGL(k_scan_add_prefix, dim3(nblk(C + 1)), dim3(TPB), 0, stream, start, C + 1, tileSum);
         hipLaunchKernelGGL(k_scatter, dim3(bin_blocks(N)), dim3(TPB), 0, stream, keyc, pid, N, start, cursor, key);
@@ -813,16 +879,19 @@ struct mpcd_gpu_ensemble {
     /* one step of every replica, as a fixed launch sequence (the step number
      * is the device counter's); the kick leaves the state in the other
      * buffer set, which the caller swaps in */
-    void step_seq() {
+    void step_seq(bool pre, bool fuse) {
         hipLaunchKernelGGL(k_ens_seeds, dim3(nblk(R)), dim3(TPB), 0, stream, reps, R, stepDev, shift, zseed, seeds);
-        hipLaunchKernelGGL(k_ens_stream, dim3(nblk(N)), dim3(TPB), 0, stream, reps, poff, R, N, seeds, shift, x, y, vx,
-                           vy, pid, cidS, cidU, err);
-        check_launch();
-        bin(cidS);
+        if (!pre) {
+            hipLaunchKernelGGL(k_ens_stream, dim3(nblk(N)), dim3(TPB), 0, stream, reps, poff, R, N, seeds, shift, x, y,
+                               vx, vy, pid, cidS, cidU, err);
+            check_launch();
+        }
+        /* after a fused stream both histograms are counted (cntS, cnt) */
+        bin(cidS, pre ? cntS : nullptr);
         hipLaunchKernelGGL(k_ens_collide, dim3(nblk(C, CELL_BLK)), dim3(CELL_BLK), cellCap * 2 * sizeof(double), stream,
                            reps, coff, R, seeds, start, list, C, vx, vy, cellCap);
         check_launch();
-        bin(cidU); /* the collision moves no particle */
+        bin(cidU, pre ? cnt : nullptr); /* the collision moves no particle */
         fields(refUx, refUy, refPx, refPy, pdriveRaw);
         zone_pass(0);
         rebin_recorded();
@@ -846,10 +915,18 @@ struct mpcd_gpu_ensemble {
         hipLaunchKernelGGL(k_ens_reduce_beta, dim3(R), dim3(TPB), 0, stream, reps, part, scal);
         hipLaunchKernelGGL(k_ens_pdrive, dim3(nblk(C)), dim3(TPB), 0, stream, reps, coff, R, C, pdriveRaw, mask, pdrive);
         hipLaunchKernelGGL(k_ens_eos, dim3(nblk(C)), dim3(TPB), 0, stream, reps, coff, R, C, pdrive, duEX, duEY);
-        hipLaunchKernelGGL(k_ens_kick_reorder, dim3(nblk(N)), dim3(TPB), 0, stream, reps, poff, R, N, list, x, y, vx, vy,
-                           pid, x2, y2, vx2, vy2, pid2, duRX, duRY, duEX, duEY, scal);
+        if (fuse) {
+            hipLaunchKernelGGL(k_fill_int, dim3(nblk(C + 1)), dim3(TPB), 0, stream, cntS, C + 1, 0);
+            hipLaunchKernelGGL(k_fill_int, dim3(nblk(C + 1)), dim3(TPB), 0, stream, cnt, C + 1, 0);
+            hipLaunchKernelGGL(k_ens_kick_reorder_stream, dim3(nblk(N)), dim3(TPB), 0, stream, reps, poff, R, N,
+                               seeds + 2 * (size_t)R, shift + 2 * (size_t)R, list, x, y, vx, vy, pid, x2, y2, vx2, vy2,
+                               pid2, duRX, duRY, duEX, duEY, scal, cidS, cidU, cntS, cnt, err);
+        } else {
+            hipLaunchKernelGGL(k_ens_kick_reorder, dim3(nblk(N)), dim3(TPB), 0, stream, reps, poff, R, N, list, x, y, vx,
+                               vy, pid, x2, y2, vx2, vy2, pid2, duRX, duRY, duEX, duEY, scal);
+        }
         check_launch();
-        if (flowOn) {
+        if (flowOn && !fuse) {
             hipLaunchKernelGGL(k_ens_flow_parts, dim3(B), dim3(TPB), 0, stream, reps, boff, R, start, vx2, flowPart);
             hipLaunchKernelGGL(k_ens_flow_acc, dim3(R), dim3(TPB), 0, stream, reps, flowPart, flowAcc);
             check_launch();
@@ -866,21 +943,26 @@ struct mpcd_gpu_ensemble {
             check_launch();
         }
         for (int j = 0; j < k; ++j) {
+            /* every kick but the call's last streams the next step (not
+             * while the flow rates are summed: they need the kicked state) */
+            const bool pre = streamedNext, fuse = !flowOn && j + 1 < k;
             if (useGraph) {
-                if (!graph[parity]) {
+                hipGraphExec_t& ge = graph[parity][pre][fuse];
+                if (!ge) {
                     hipGraph_t gr = nullptr;
                     HIP_OK(hipStreamBeginCapture(stream, hipStreamCaptureModeRelaxed));
-                    step_seq();
+                    step_seq(pre, fuse);
                     HIP_OK(hipStreamEndCapture(stream, &gr));
-                    const hipError_t ie = hipGraphInstantiate(&graph[parity], gr, nullptr, nullptr, 0);
+                    const hipError_t ie = hipGraphInstantiate(&ge, gr, nullptr, nullptr, 0);
                     (void)hipGraphDestroy(gr);
                     HIP_OK(ie);
                 }
-                HIP_OK(hipGraphLaunch(graph[parity], stream));
+                HIP_OK(hipGraphLaunch(ge, stream));
             } else {
-                step_seq();
+                step_seq(pre, fuse);
             }
             swap_sets();
+            streamedNext = fuse;
             if (flowOn) ++flowSteps;
         }
         stepHost = (long long)first + k;

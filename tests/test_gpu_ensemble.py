@@ -94,6 +94,9 @@ def test_graph_replays_match_eager_steps(gpu):
             for s in range(1, 6):
                 ens.step(s, 1, graph=use_graph)
                 snaps.append([ens.get_state(r) for r in (0, len(ps) - 1)])
+            for s in (6, 8):  # two steps per call: the first kick streams the second step
+                ens.step(s, 2, graph=use_graph)
+                snaps.append([ens.get_state(r) for r in (0, len(ps) - 1)])
             runs.append(snaps)
         finally:
             ens.close()
