@@ -46,7 +46,7 @@ HBM_PEAK_GBS = 8000.0
 ALG_BYTES = {"stream_bc": 64, "collide": 48, "zone_kernel": 48, "kick": 76, "kick_stream": 84}
 STEP_ALG_BYTES = 256  # compulsory particle traffic of one plain step
 # kernel of each timed class, as rocprofv3 names it
-KERNEL_NAME = {"stream_bc": "k_stream_bc", "collide": "k_collide", "zone_kernel": "k_zone_fused<unsigned short>",
+KERNEL_NAME = {"stream_bc": "k_stream_bc", "collide": "k_collide", "zone_kernel": "k_zone_fused<unsigned short, false>",
                "kick": "k_kick_reorder", "kick_stream": "k_kick_reorder_stream"}
 
 

@@ -108,11 +108,14 @@ typedef struct mpcd_cell_fields {
     double *Ux, *Uy, *Px, *Py, *T, *rho, *Pkin, *Pvir, *P, *rhoTarget;
 } mpcd_cell_fields;
 
-/* The closure grids of LiquidClosureResult (liquid_closure.h:36-42); any
- * pointer may be NULL. */
+/* The closure grids of LiquidClosureResult (liquid_closure.h:36-42) and,
+ * in vRepaired (2n, interleaved, input particle order), the velocities of
+ * its stateRepaired (liquid_closure.h:31: the red state kicked by
+ * beta*duRepair, liquid_closure.cpp:379-387); any pointer may be NULL. */
 typedef struct mpcd_closure_grids {
     double *pdriveRaw, *pdrive, *duRepairX, *duRepairY, *duEOSX, *duEOSY;
     int32_t* interzoneMask;
+    double* vRepaired;
 } mpcd_closure_grids;
 
 /* Occupancy statistics of a pass (ZonePassMetrics' before/after fields,

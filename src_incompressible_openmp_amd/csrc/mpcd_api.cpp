@@ -403,8 +403,9 @@ LiquidClosureResult run_liquid_closure(const State& stateRef, const State& state
                                    &out.duEOSY})
         g->assign(Nc, 0.0);
     out.interzoneMask.assign(Nc, 0);
+    out.stateRepaired = stateRed;
     mpcd_closure_grids g{out.pdriveRaw.data(), out.pdrive.data(), out.duRepairX.data(), out.duRepairY.data(),
-                         out.duEOSX.data(), out.duEOSY.data(), out.interzoneMask.data()};
+                         out.duEOSX.data(),   out.duEOSY.data(),  out.interzoneMask.data(), out.stateRepaired.v.data()};
     mpcd_closure_metrics m{};
     ok(mpcd_gpu_liquid_closure(&p, stateRef.x.data(), stateRef.v.data(), stateRed.x.data(), stateRed.v.data(),
                                out.stateOut.x.data(), out.stateOut.v.data(), &m, &g));
@@ -423,16 +424,6 @@ LiquidClosureResult run_liquid_closure(const State& stateRef, const State& state
     out.metrics.momentumDeltaX = m.momentumDeltaX;
     out.metrics.momentumDeltaY = m.momentumDeltaY;
     out.metrics.nMaskedInterzoneCells = m.nMaskedInterzoneCells;
-    /* the repaired state: the red state kicked by beta * duRepair of its
-     * cell (liquid_closure.cpp:377-385), same arithmetic on the host; the
-     * output state keeps the library's particle order, which is the input's */
-    out.stateRepaired = stateRed;
-    const double beta = m.betaRepairApplied;
-    for (int i = 0; i < params.n; ++i) {
-        const size_t c = (size_t)cell_id_from_position(stateRed.x[2 * (size_t)i], stateRed.x[2 * (size_t)i + 1], params);
-        out.stateRepaired.v[2 * (size_t)i] += beta * out.duRepairX[c];
-        out.stateRepaired.v[2 * (size_t)i + 1] += beta * out.duRepairY[c];
-    }
     const double rhoT = rho_target(params);
     out.refFields = fields_of(stateRef.x, stateRef.v, params, rhoT);
     out.redFields = fields_of(stateRed.x, stateRed.v, params, rhoT);

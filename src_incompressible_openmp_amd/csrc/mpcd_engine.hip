@@ -1457,6 +1457,24 @@ __device__ __forceinline__ void kick_velocity(double v0x, double v0y, double kx,
     ny += ey;
 }
 
+/* The repaired velocities alone, v + beta*duRepair[c] of the particle's
+ * (red) cell (stateRepaired, liquid_closure.cpp:379-387), interleaved in
+ * particle-id order: the state of slot s is particle pid[s]. */
+__global__ void __launch_bounds__(TPB) k_repaired_velocity(Geom g, const double* __restrict__ x,
+                                                           const double* __restrict__ y, const double* __restrict__ vx,
+                                                           const double* __restrict__ vy, const int* __restrict__ pid,
+                                                           const double* __restrict__ duRX,
+                                                           const double* __restrict__ duRY,
+                                                           const double* __restrict__ scal, double* __restrict__ a) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= g.n) return;
+    const double beta = scal[3];
+    const int c = cell_of(g, x[s], y[s]);
+    const size_t q = (size_t)pid[s];
+    a[2 * q] = vx[s] + beta * duRX[c];
+    a[2 * q + 1] = vy[s] + beta * duRY[c];
+}
+
 /* v = (v + beta*duRepair[c]) + duEOS[c] for the particle's (red) cell
  * (apply_cell_kick twice, liquid_closure.cpp:379-436), fused with a reorder
  * of the state into the order of the current (red) binning: position s of
@@ -2566,6 +2584,18 @@ struct mpcd_gpu_sim {
         check_launch();
     }
 
+    /* After closure(): the repaired velocities (host, interleaved, particle
+     * order).  The kick swapped the buffers, so the second set holds the red
+     * state in its pre-kick slot order with its particle ids. */
+    void download_repaired(double* hv) {
+        if (n == 0) return;
+        hipLaunchKernelGGL(k_repaired_velocity, dim3(nblk(n)), dim3(TPB), 0, stream, g, x2, y2, vx2, vy2, pid2, duRX,
+                           duRY, scal, stage);
+        check_launch();
+        HIP_OK(hipMemcpyAsync(hv, stage, 2 * (size_t)n * sizeof(double), hipMemcpyDeviceToHost, stream));
+        HIP_OK(hipStreamSynchronize(stream));
+    }
+
     ClosureArgs closure_args() {
         ClosureArgs ca{};
         ca.Nc = Nc;
@@ -3612,6 +3642,7 @@ int mpcd_gpu_liquid_closure(const mpcd_params* p, const double* xRef, const doub
                                       s.stream));
             HIP_OK(hipStreamSynchronize(s.stream));
         }
+        if (grids && grids->vRepaired) s.download_repaired(grids->vRepaired);
         s.get_state(xOut, vOut);
         if (m) *m = cm;
     });

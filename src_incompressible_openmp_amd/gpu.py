@@ -59,7 +59,7 @@ class CellFieldsC(ctypes.Structure):
 
 class ClosureGrids(ctypes.Structure):
     _fields_ = [(k, _dp) for k in ("pdriveRaw", "pdrive", "duRepairX", "duRepairY", "duEOSX", "duEOSY")] + [
-        ("interzoneMask", _ip)]
+        ("interzoneMask", _ip), ("vRepaired", _dp)]
 
 
 def _new_fields(Nc: int):
@@ -253,16 +253,16 @@ def closure_training_pair(p: MpcdParams, ref: Dict[str, np.ndarray], grids: Dict
 
 def liquid_closure(p: MpcdParams, x_ref, v_ref, x_red, v_red):
     n2 = 2 * p.n
-    xo, vo = np.zeros(n2), np.zeros(n2)
+    xo, vo, vrep = np.zeros(n2), np.zeros(n2), np.zeros(n2)
     m = ClosureMetrics()
     grids = {k: np.zeros(p.Nc) for k in ("pdriveRaw", "pdrive", "duRepairX", "duRepairY", "duEOSX", "duEOSY")}
     mask = np.zeros(p.Nc, np.int32)
     g = ClosureGrids(*[_ptr(grids[k]) for k in ("pdriveRaw", "pdrive", "duRepairX", "duRepairY", "duEOSX",
-                                                "duEOSY")], _ptr(mask, ctypes.c_int32))
+                                                "duEOSY")], _ptr(mask, ctypes.c_int32), _ptr(vrep))
     args = [np.ascontiguousarray(a, np.float64) for a in (x_ref, v_ref, x_red, v_red)]
     _check(load().mpcd_gpu_liquid_closure(ctypes.byref(p), *[_ptr(a) for a in args], _ptr(xo), _ptr(vo),
                                           ctypes.byref(m), ctypes.byref(g)))
-    out = {"x": xo, "v": vo, "metrics": struct_to_dict(m), "mask": mask}
+    out = {"x": xo, "v": vo, "v_repaired": vrep, "metrics": struct_to_dict(m), "mask": mask}
     out.update(grids)
     return out
 

@@ -158,6 +158,7 @@ def test_closure(gpu, oracle, case):
     assert np.array_equal(g["mask"], o["mask"])
     assert bits(g["x"], o["x"])
     assert vclose(g["v"], o["v"])
+    assert vclose(g["v_repaired"], o["v_repaired"])  # stateRepaired: beta is a tree-reduced ratio
     mo, mg = o["metrics"], g["metrics"]
     assert mg["nMaskedInterzoneCells"] == mo["nMaskedInterzoneCells"]
     for k in ("betaRepairNum", "betaRepairDen", "betaRepairOpt", "betaRepairApplied", "meanPdriveRaw", "meanPdrive",
