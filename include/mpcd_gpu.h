@@ -224,6 +224,15 @@ int mpcd_gpu_build_zone_layout(const mpcd_params* p, int mode, int32_t* ownerMap
  * three may be NULL; a NULL diag10 skips the diagnostics' work. */
 int mpcd_gpu_run_zone_kernel(const mpcd_params* p, double* x, double* v, int xs, int xe, int ys, int ye,
                              uint64_t rng_seed, double* stats6, int32_t* moved2, double* diag10);
+/* replaces mpcd::run_zone_kernel_inplace (zone_pass.h:62-66) for a zone of
+ * any shape, given as ZoneDescriptor::maskVec (zone_pass.h:12-16): zone_mask
+ * holds Nc bytes, nonzero for the zone's cells (at least one).  The kernel
+ * runs over the mask's bounding rectangle; cells of it outside the mask keep
+ * their particles and are never in the dense or low lists, a target, a
+ * neighbour or a donor (zone_pass.cpp:676-678, :611, :625, :263-278, :819).
+ * Outputs as mpcd_gpu_run_zone_kernel. */
+int mpcd_gpu_run_zone_kernel_mask(const mpcd_params* p, double* x, double* v, const uint8_t* zone_mask,
+                                  uint64_t rng_seed, double* stats6, int32_t* moved2, double* diag10);
 
 /* replaces mpcd::run_zone_pass_openmp / run_zone_pass_sync_serial
  * (zone_pass_openmp.h:35-53): snapshot semantics, zones on the GPU */

@@ -152,12 +152,16 @@ struct Fields {
 };
 
 /* a zone: the cells [x0,x1) x [y0,y1) (zone_pass.cpp:191-219), listed in
- * ascending cell id */
+ * ascending cell id; or, with a mask (one entry per grid cell, as
+ * Zone.maskVec, zone_pass.cpp:20-24), the cells whose entry is nonzero
+ * within their bounding rectangle [x0,x1) x [y0,y1) */
 struct Zone {
     int id = 0;
     int x0 = 0, x1 = 0, y0 = 0, y1 = 0;
     std::vector<int> cells;
+    std::vector<char> mask;
     bool in(int c, int Ny) const {
+        if (!mask.empty()) return mask[(size_t)c] != 0;
         const int ix = c / Ny, iy = c - ix * Ny;
         return ix >= x0 && ix < x1 && iy >= y0 && iy < y1;
     }

@@ -166,6 +166,46 @@ extern "C" int oracle_zone_kernel(int mode, const mpcd_params* p, double* x, dou
     OR_CATCH
 }
 
+/* run_zone_kernel_inplace on a zone given by its mask (p->Nc bytes, nonzero
+ * in the zone), zone_pass.cpp:1065-1081 with ZoneDescriptor.maskVec */
+extern "C" int oracle_zone_kernel_mask(int mode, const mpcd_params* p, double* x, double* v, const uint8_t* mask,
+                                       uint64_t seed, double* stats6, double* diag10, int32_t* moved2) {
+    OR_TRY
+    Zone z;
+    z.mask.assign((size_t)p->Nc, 0);
+    z.x0 = p->Nx;
+    z.y0 = p->Ny;
+    for (int c = 0; c < p->Nc; ++c) {
+        if (!mask[c]) continue;
+        z.mask[(size_t)c] = 1;
+        z.cells.push_back(c);
+        const int ix = c / p->Ny, iy = c - ix * p->Ny;
+        z.x0 = std::min(z.x0, ix);
+        z.x1 = std::max(z.x1, ix + 1);
+        z.y0 = std::min(z.y0, iy);
+        z.y1 = std::max(z.y1, iy + 1);
+    }
+    if (z.cells.empty()) throw std::runtime_error("empty zone mask");
+    auto xv = vec(x, 2 * (size_t)p->n), vv = vec(v, 2 * (size_t)p->n);
+    ZoneStats st;
+    if (mode == 0) {
+        StdRngZone r(seed);
+        st = zone_kernel(*p, xv, vv, z, r);
+    } else {
+        CtrRngZone r(seed);
+        st = zone_kernel(*p, xv, vv, z, r);
+    }
+    put(xv, x);
+    put(vv, v);
+    if (stats6) std::memcpy(stats6, st.stats.data(), 6 * sizeof(double));
+    if (diag10) std::memcpy(diag10, st.diag.data(), 10 * sizeof(double));
+    if (moved2) {
+        moved2[0] = st.movedDense;
+        moved2[1] = st.movedSparse;
+    }
+    OR_CATCH
+}
+
 extern "C" int oracle_zone_pass(int mode, const mpcd_params* p, double* x, double* v, int layout, uint64_t seedBase,
                                 int gaussSeidel, mpcd_zone_pass_metrics* m, int32_t* owner) {
     OR_TRY

@@ -7,7 +7,10 @@
  * not expose ZoneKernelStats, so this is how the oracle's zone-kernel
  * statistics and diagnostics are pinned.
  *
- * Usage: zone_kernel_probe params.kv x.bin v.bin layout zone_index seed out_prefix
+ * Usage: zone_kernel_probe params.kv x.bin v.bin layout zone_index seed out_prefix [mask.bin]
+ * With mask.bin (Nc bytes, nonzero in the zone) the zone is that cell set
+ * (ZoneDescriptor.cellIds in ascending order and maskVec, zone_pass.h:12-16)
+ * instead of the layout's zone.
  * Writes <out>_x.bin, <out>_v.bin (interleaved f64) and <out>_stats.kv
  * (stats0..5, diag0..9, movedDense, movedSparse, nZones; %.17g).
  */
@@ -15,16 +18,19 @@
 #include <cstdio>
 #include <cstdlib>
 #include <exception>
+#include <fstream>
+#include <iterator>
 #include <stdexcept>
 #include <string>
+#include <vector>
 
 #include "params_io.h"
 #include "state_io.h"
 #include "zone_pass.h"
 
 int main(int argc, char** argv) {
-    if (argc != 8) {
-        std::fprintf(stderr, "usage: %s params.kv x.bin v.bin layout zone_index seed out_prefix\n", argv[0]);
+    if (argc != 8 && argc != 9) {
+        std::fprintf(stderr, "usage: %s params.kv x.bin v.bin layout zone_index seed out_prefix [mask.bin]\n", argv[0]);
         return 2;
     }
     try {
@@ -37,7 +43,20 @@ int main(int argc, char** argv) {
             std::fprintf(stderr, "zone index %d out of range (%zu zones)\n", zi, lay.zones.size());
             return 2;
         }
-        const mpcd::ZoneKernelStats r = mpcd::run_zone_kernel_inplace(st.x, st.v, p, lay.zones[(size_t)zi], seed);
+        mpcd::ZoneDescriptor zone = lay.zones[(size_t)zi];
+        if (argc == 9) {
+            std::ifstream f(argv[8], std::ios::binary);
+            const std::vector<char> m((std::istreambuf_iterator<char>(f)), std::istreambuf_iterator<char>());
+            if ((int)m.size() != p.Nc) throw std::runtime_error("mask.bin must hold Nc bytes");
+            zone.cellIds.clear();
+            zone.maskVec.assign((size_t)p.Nc, 0);
+            for (int c = 0; c < p.Nc; ++c)
+                if (m[(size_t)c]) {
+                    zone.cellIds.push_back(c);
+                    zone.maskVec[(size_t)c] = 1;
+                }
+        }
+        const mpcd::ZoneKernelStats r = mpcd::run_zone_kernel_inplace(st.x, st.v, p, zone, seed);
         const std::string out = argv[7];
         mpcd::write_xy_interleaved(out + "_x.bin", st.x, p.n);
         mpcd::write_xy_interleaved(out + "_v.bin", st.v, p.n);

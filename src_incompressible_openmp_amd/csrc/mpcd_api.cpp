@@ -137,25 +137,6 @@ double seconds_since(std::chrono::steady_clock::time_point t0) {
     return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
 }
 
-/* rectangle of a zone given by its cells (zones of the reference's layouts
- * are rectangles) */
-void zone_rect(const Params& q, const ZoneDescriptor& z, int& xs, int& xe, int& ys, int& ye) {
-    if (z.cellIds.empty()) throw std::runtime_error("run_zone_kernel_inplace: empty zone");
-    xs = q.Nx;
-    ys = q.Ny;
-    xe = ye = -1;
-    for (int c : z.cellIds) {
-        if (c < 0 || c >= q.Nc) throw std::runtime_error("run_zone_kernel_inplace: cell id out of range");
-        const int ix = c / q.Ny, iy = c % q.Ny;
-        xs = std::min(xs, ix);
-        xe = std::max(xe, ix + 1);
-        ys = std::min(ys, iy);
-        ye = std::max(ye, iy + 1);
-    }
-    if ((size_t)(xe - xs) * (size_t)(ye - ys) != z.cellIds.size())
-        throw std::runtime_error("run_zone_kernel_inplace: the zone is not a rectangle of cells");
-}
-
 OpenMPPassResult snapshot_pass(const State& in, const Params& q, const std::string& mode, std::uint64_t seed,
                                int threads) {
     if (!q.useIncompressibleRedistribution || !q.useZoneRedistribution)
@@ -297,13 +278,23 @@ ZoneKernelStats run_zone_kernel_inplace(std::vector<double>& x, std::vector<doub
                                         const ZoneDescriptor& zone, std::uint64_t rngSeed) {
     need_size(x, params, "run_zone_kernel_inplace");
     need_size(v, params, "run_zone_kernel_inplace");
-    int xs, xe, ys, ye;
-    zone_rect(params, zone, xs, xe, ys, ye);
+    /* the zone is its maskVec, as the original's kernel reads it
+     * (zone_pass.cpp:676); a descriptor without a mask of Nc entries gives
+     * its cellIds instead */
+    std::vector<std::uint8_t> mask((size_t)params.Nc, 0);
+    if ((int)zone.maskVec.size() == params.Nc) {
+        for (int c = 0; c < params.Nc; ++c) mask[(size_t)c] = zone.maskVec[(size_t)c] ? 1 : 0;
+    } else {
+        for (int c : zone.cellIds) {
+            if (c < 0 || c >= params.Nc) throw std::runtime_error("run_zone_kernel_inplace: cell id out of range");
+            mask[(size_t)c] = 1;
+        }
+    }
     const mpcd_params p = to_c(params);
     ZoneKernelStats out{};
     int32_t moved[2] = {0, 0};
-    ok(mpcd_gpu_run_zone_kernel(&p, x.data(), v.data(), xs, xe, ys, ye, rngSeed, out.stats.data(), moved,
-                                out.diag.data()));
+    ok(mpcd_gpu_run_zone_kernel_mask(&p, x.data(), v.data(), mask.data(), rngSeed, out.stats.data(), moved,
+                                     out.diag.data()));
     out.movedDense = moved[0];
     out.movedSparse = moved[1];
     return out;

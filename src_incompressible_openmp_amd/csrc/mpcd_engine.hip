@@ -2394,7 +2394,8 @@ struct mpcd_gpu_sim {
      * particles, seeded with seedBase itself (run_zone_kernel_inplace). */
     void zone_pass(uint64_t seedBase, const ZoneRect* rectsDev, int nZ, int maxC, int maxW, const int* ownerDev,
                    const unsigned char* coverDev, const int* overDev, int nOverCells, int single,
-                   const double* inPx = nullptr, const double* inPy = nullptr, bool record = false) {
+                   const double* inPx = nullptr, const double* inPy = nullptr, bool record = false,
+                   const unsigned char* zmask = nullptr) {
         hipLaunchKernelGGL(k_zone_sizes, dim3(nZ + 1), dim3(64), 0, stream, rectsDev, nZ, bin.start, p.Ny, zsize);
         const int nt = (nZ + 1 + SCAN_TILE - 1) / SCAN_TILE;
         hipLaunchKernelGGL(k_scan_tiles, dim3(nt), dim3(TPB), 0, stream, zsize, nZ + 1, zoff, tileSum,
@@ -2424,6 +2425,7 @@ struct mpcd_gpu_sim {
         a.in.inPy = inPy;
         a.in.dgCell = zdCell;
         a.in.dgCnt = zdCnt;
+        a.in.zmask = zmask;
         if (record) {
             /* moved particles record their cell changes for rebin_recorded()
              * (bin.rnk is -1 for every slot here: set at creation, and each
@@ -3282,15 +3284,21 @@ int mpcd_gpu_build_zone_layout(const mpcd_params* p, int mode, int32_t* ownerMap
     });
 }
 
-int mpcd_gpu_run_zone_kernel(const mpcd_params* p, double* x, double* v, int xs, int xe, int ys, int ye,
-                             uint64_t rng_seed, double* stats6, int32_t* moved2, double* diag10) {
-    return guarded([&] {
-        std::lock_guard<std::mutex> lk(g_cache_mu);
-        if (xs < 0 || ys < 0 || xe > p->Nx || ye > p->Ny || xs >= xe || ys >= ye)
-            throw ApiError(MPCD_E_INVALID, "zone rectangle out of range");
+/* One zone's kernel on the scratch simulation: the rectangle [xs,xe) x
+ * [ys,ye), restricted to the cells of mask (Nc bytes, host) when mask is
+ * given.  The caller holds g_cache_mu. */
+static void run_zone_kernel_on(const mpcd_params* p, double* x, double* v, int xs, int xe, int ys, int ye,
+                               const unsigned char* mask, uint64_t rng_seed, double* stats6, int32_t* moved2,
+                               double* diag10) {
+    {
         auto& s = scratch_sim(*p);
         s.set_state(x, v);
         s.bin_unshifted();
+        unsigned char* zmask = nullptr;
+        if (mask) {
+            zmask = dalloc<unsigned char>((size_t)s.Nc);
+            HIP_OK(hipMemcpyAsync(zmask, mask, (size_t)s.Nc, hipMemcpyHostToDevice, s.stream));
+        }
         ZoneRect r{xs, xe, ys, ye};
         const int C = (xe - xs) * (ye - ys);
         ZoneRect* d = dalloc<ZoneRect>(1);
@@ -3306,7 +3314,7 @@ int mpcd_gpu_run_zone_kernel(const mpcd_params* p, double* x, double* v, int xs,
             hipLaunchKernelGGL(k_fill_double, dim3(nblk(4 * (size_t)C)), dim3(TPB), 0, s.stream, s.zdCell, 4 * C, 0.0);
             s.check_launch();
         }
-        s.zone_pass(rng_seed, d, 1, C, xe - xs, nullptr, nullptr, nullptr, 0, 1);
+        s.zone_pass(rng_seed, d, 1, C, xe - xs, nullptr, nullptr, nullptr, 0, 1, nullptr, nullptr, false, zmask);
         if (diag10) {
             /* ZoneKernelStats.diag: occupancy of the final counts (low maps from
              * the sparse phase's start counts), particle sums after, cell sums */
@@ -3338,6 +3346,7 @@ int mpcd_gpu_run_zone_kernel(const mpcd_params* p, double* x, double* v, int xs,
         HIP_OK(hipMemcpyAsync(&o, s.zout, sizeof(o), hipMemcpyDeviceToHost, s.stream));
         s.get_state(x, v);
         HIP_OK(hipFree(d));
+        if (zmask) HIP_OK(hipFree(zmask));
         s.check_err();
         if (stats6) {
             stats6[0] = (double)o.nHigh;
@@ -3351,6 +3360,40 @@ int mpcd_gpu_run_zone_kernel(const mpcd_params* p, double* x, double* v, int xs,
             moved2[0] = o.movedDense;
             moved2[1] = o.movedSparse;
         }
+    }
+}
+
+int mpcd_gpu_run_zone_kernel(const mpcd_params* p, double* x, double* v, int xs, int xe, int ys, int ye,
+                             uint64_t rng_seed, double* stats6, int32_t* moved2, double* diag10) {
+    return guarded([&] {
+        std::lock_guard<std::mutex> lk(g_cache_mu);
+        if (xs < 0 || ys < 0 || xe > p->Nx || ye > p->Ny || xs >= xe || ys >= ye)
+            throw ApiError(MPCD_E_INVALID, "zone rectangle out of range");
+        run_zone_kernel_on(p, x, v, xs, xe, ys, ye, nullptr, rng_seed, stats6, moved2, diag10);
+    });
+}
+
+int mpcd_gpu_run_zone_kernel_mask(const mpcd_params* p, double* x, double* v, const uint8_t* zone_mask,
+                                  uint64_t rng_seed, double* stats6, int32_t* moved2, double* diag10) {
+    return guarded([&] {
+        std::lock_guard<std::mutex> lk(g_cache_mu);
+        validate(*p);
+        if (!zone_mask) throw ApiError(MPCD_E_INVALID, "zone mask is NULL");
+        int xs = p->Nx, xe = -1, ys = p->Ny, ye = -1;
+        long long nIn = 0;
+        for (int c = 0; c < p->Nc; ++c) {
+            if (!zone_mask[c]) continue;
+            const int ix = c / p->Ny, iy = c - ix * p->Ny;
+            xs = std::min(xs, ix);
+            xe = std::max(xe, ix + 1);
+            ys = std::min(ys, iy);
+            ye = std::max(ye, iy + 1);
+            ++nIn;
+        }
+        if (nIn == 0) throw ApiError(MPCD_E_INVALID, "the zone mask selects no cell");
+        /* a mask that fills its bounding rectangle is that rectangle */
+        const bool rect = nIn == (long long)(xe - xs) * (long long)(ye - ys);
+        run_zone_kernel_on(p, x, v, xs, xe, ys, ye, rect ? nullptr : zone_mask, rng_seed, stats6, moved2, diag10);
     });
 }
 

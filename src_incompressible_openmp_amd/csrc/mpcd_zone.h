@@ -174,7 +174,8 @@ enum : unsigned short {
     ZF_CORR = 32,   /* momentum correction applies */
     ZF_CHG = 64,    /* list changed */
     ZF_OWN = 128,   /* this zone owns the cell's input particles */
-    ZF_COVER = 256  /* several zones cover the cell */
+    ZF_COVER = 256, /* several zones cover the cell */
+    ZF_OUT = 4096   /* in the zone's rectangle but not in its mask */
 };
 
 /* Inputs of a zone pass common to all its zones. */
@@ -205,6 +206,11 @@ struct ZoneIn {
     /* added to the recorded new cells (an ensemble replica's first cell in
      * the concatenated grid; 0 for a single simulation) */
     int cellOff = 0;
+    /* per grid cell, nonzero in the zone (Zone.maskVec, zone_pass.cpp:20-24),
+     * or nullptr: the zone is its whole rectangle.  A cell of the rectangle
+     * outside the mask keeps its particles and counts but is never in the
+     * dense or low lists, a target, a neighbour or a donor. */
+    const unsigned char* zmask = nullptr;
 };
 
 /* Scalars the lanes share (LDS on the device). */
@@ -691,6 +697,7 @@ ZHD int zone_compact(Z& z, int lane, int nl, int mode) {
         if (lc < z.C) {
             const double c = (double)z.cnt[lc];
             f = (mode == ZC_DENSE) ? c > high_thr(z, lc) : (z.cnt[lc] > 0 && c < low_thr(z, lc));
+            f = f && !fget(z, lc, ZF_OUT);
         }
         const unsigned long long msk = wave_ballot(f);
         if (f) z.dl[base + wave_popc(msk & lanes_below(lane))] = (typename Z::idx_t)lc;
@@ -773,7 +780,7 @@ ZHD void dense_targets(Z& z, int src, int high, int pass, int phase, int lane, i
             int jx, jy, gc = -1;
             if (shift_idx(z, sgx, sgy, dx, dy, jx, jy)) {
                 const int lc = zone_lc(z, jx, jy);
-                if (lc >= 0 && (double)z.cnt[lc] < thr(z, lc, high)) gc = jy + z.g.Ny * jx;
+                if (lc >= 0 && !fget(z, lc, ZF_OUT) && (double)z.cnt[lc] < thr(z, lc, high)) gc = jy + z.g.Ny * jx;
             }
             z.nbGc[t] = gc;
             if (gc >= 0) z.nbTie[t] = mpcd_draw_tie(z.seed, pass, phase, (uint32_t)sgc, (uint32_t)gc);
@@ -1013,7 +1020,7 @@ ZHD int dense_rank_wave(const Z& z, int sgx, int sgy, int sgc, int high, int pas
         int jx, jy;
         if (shift_idx(z, sgx, sgy, dx, dy, jx, jy)) {
             const int l = zone_lc(z, jx, jy);
-            if (l >= 0) {
+            if (l >= 0 && !fget(z, l, ZF_OUT)) {
                 const int c = (int)z.cnt[l];
                 const double t = thr(z, l, high);
                 if ((double)c < t) {
@@ -1557,6 +1564,7 @@ ZHD int sparse_cell(Z& z, int c, int nc, int lowInt, int lane, int nl) {
         const int dy = f / 3 - 1, dx = f % 3 - 1;
         int jx, jy, q = -1;
         if (shift_idx(z, cgx, cgy, dx, dy, jx, jy)) q = zone_lc(z, jx, jy);
+        if (q >= 0 && fget(z, q, ZF_OUT)) q = -1;
         z.nbGc[o] = q;
     }
     ZBAR();
@@ -1687,7 +1695,8 @@ ZHD bool sparse_alone(const Z& z, int c) {
         for (int dy = -1; dy <= 1; ++dy)
             if ((dx || dy) && shift_idx(z, lc_gx(z, c), lc_gy(z, c), dx, dy, jx, jy)) {
                 const int q = zone_lc(z, jx, jy);
-                if (q >= 0 && iget(z.ext[(q / z.zh + 2) * H + q % z.zh + 2]) > ZONE_TMP_W) big = true;
+                if (q >= 0 && !fget(z, q, ZF_OUT) && iget(z.ext[(q / z.zh + 2) * H + q % z.zh + 2]) > ZONE_TMP_W)
+                    big = true;
             }
     return big;
 }
@@ -1861,6 +1870,7 @@ ZHD void zone_setup(Z& z, const ZoneIn& in, int lane, int nl) {
             if (!in.owner || in.owner[gc] == z.zid) f |= ZF_OWN;
             if (in.gcover && in.gcover[gc]) f |= ZF_COVER;
             if (f != ZF_OWN) mixed = true;
+            if (in.zmask && !in.zmask[gc]) f |= ZF_OUT;
             z.fl[lc] = f;
             if (lc % z.zh == 0) {
                 z.colL[lc / z.zh] = k0;
@@ -1959,7 +1969,7 @@ template <class Z>
 ZHD void zone_correct_sums(Z& z, int lc) {
     const int m = cnt_of(z, lc);
     if (!fget(z, lc, ZF_CHG) || m == 0) {
-        if (z.dgCell && m > 0) diag_cell_unchanged(z, lc);
+        if (z.dgCell && m > 0 && !fget(z, lc, ZF_OUT)) diag_cell_unchanged(z, lc);
         return;
     }
     const int gc = lc_gc(z, lc);
@@ -2374,7 +2384,7 @@ template <class Z>
 ZHD double fin_cell(Z& z, int lc, double pbx, double pby) {
     const int m = cnt_of(z, lc);
     if (!fget(z, lc, ZF_CHG)) {
-        if (z.dgCell && m > 0) diag_cell_unchanged(z, lc);
+        if (z.dgCell && m > 0 && !fget(z, lc, ZF_OUT)) diag_cell_unchanged(z, lc);
         zone_correct_write(z, lc); /* the side-buffer copy of a covered cell */
         return 0.0;
     }

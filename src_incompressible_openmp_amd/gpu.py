@@ -198,6 +198,22 @@ def run_zone_kernel(p: MpcdParams, x: np.ndarray, v: np.ndarray, rect, seed: int
     return (stats, moved, dg) if diag else (stats, moved)
 
 
+def run_zone_kernel_mask(p: MpcdParams, x: np.ndarray, v: np.ndarray, mask, seed: int, diag: bool = False):
+    """run_zone_kernel_inplace on the zone of any shape given by mask
+    (ZoneDescriptor.maskVec: p.Nc entries, nonzero in the zone), in place;
+    returns as run_zone_kernel."""
+    m = np.ascontiguousarray(mask, np.uint8)
+    if m.shape != (p.Nc,):
+        raise ValueError("the zone mask needs one entry per cell")
+    stats = np.zeros(6)
+    moved = np.zeros(2, np.int32)
+    dg = np.zeros(10)
+    _check(load().mpcd_gpu_run_zone_kernel_mask(ctypes.byref(p), _ptr(_f64(x)), _ptr(_f64(v)),
+                                                _ptr(m, ctypes.c_uint8), ctypes.c_uint64(seed), _ptr(stats),
+                                                _ptr(moved, ctypes.c_int32), _ptr(dg) if diag else None))
+    return (stats, moved, dg) if diag else (stats, moved)
+
+
 def zone_pass(p: MpcdParams, x: np.ndarray, v: np.ndarray, layout: int, seed_base: int,
               gauss_seidel: bool = False):
     """Snapshot pass (run_zone_pass_openmp), or with gauss_seidel the zones

@@ -213,6 +213,58 @@ def zone_kernel_case(name: str):
     return out
 
 
+MASK_ZONE_CASES = ("small", "override", "cfg0")
+
+
+def zone_masks(Nx: int, Ny: int):
+    """Zones that are not rectangles, as ZoneDescriptor.maskVec (one byte per
+    cell, c = iy + Ny*ix): a rectangle with a hole, an L, a random 40% of the
+    interior, and a band across the periodic x boundary."""
+    def grid():
+        return np.zeros((Nx, Ny), np.uint8)
+    notch = grid()
+    notch[1:min(Nx, 9), 1:min(Ny, 8)] = 1
+    notch[3:5, 3:5] = 0
+    ell = grid()
+    ell[: Nx // 2, :3] = 1
+    ell[:3, : (2 * Ny) // 3] = 1
+    rng = np.random.default_rng(17 + Nx)
+    scatter = grid()
+    scatter[2:Nx - 2, 1:Ny - 1] = (rng.random((Nx - 4, Ny - 2)) < 0.4).astype(np.uint8)
+    wrap = grid()
+    wrap[[Nx - 2, Nx - 1, 0, 1], 2:6] = 1
+    return {k: m.reshape(-1) for k, m in (("notch", notch), ("ell", ell), ("scatter", scatter), ("wrap", wrap))}
+
+
+def zone_kernel_mask_case(name: str):
+    """run_zone_kernel_inplace of the original on the zones of zone_masks
+    (through zone_kernel_probe's mask argument): stats, diag, moved counts
+    and SHA-256 digests of the state it leaves, with each mask."""
+    p = case_params(name)
+    n = int(p["n"])
+    x, v, _, _ = case_state(name)
+    out = {}
+    with tempfile.TemporaryDirectory() as d:
+        write_params_kv(os.path.join(d, "params.kv"), p)
+        fileio.write_xy(os.path.join(d, "x.bin"), x, n)
+        fileio.write_xy(os.path.join(d, "v.bin"), v, n)
+        for mi, (mname, mask) in enumerate(zone_masks(int(p["Nx"]), int(p["Ny"])).items()):
+            seed = 9001 + mi
+            mask.astype(np.uint8).tofile(os.path.join(d, "mask.bin"))
+            run("zone_kernel_probe", os.path.join(d, "params.kv"), os.path.join(d, "x.bin"),
+                os.path.join(d, "v.bin"), "base", "0", str(seed), os.path.join(d, "o"), os.path.join(d, "mask.bin"))
+            kv = read_kv(os.path.join(d, "o_stats.kv"))
+            key = f"{name}_mask_{mname}"
+            out[key] = mask.astype(np.uint8)
+            out[key + "_seed"] = np.array(seed)
+            out[key + "_stats"] = np.array([kv[f"stats{k}"] for k in range(6)])
+            out[key + "_diag"] = np.array([kv[f"diag{k}"] for k in range(10)])
+            out[key + "_moved"] = np.array([kv["movedDense"], kv["movedSparse"]], np.int32)
+            out[key + "_x_sha256"] = np.array(digest(fileio.read_xy(os.path.join(d, "o_x.bin"), n)))
+            out[key + "_v_sha256"] = np.array(digest(fileio.read_xy(os.path.join(d, "o_v.bin"), n)))
+    return out
+
+
 def main() -> None:
     if not os.path.exists(os.path.join(REF, "main_step01_stream_bc_collision")):
         raise SystemExit("build the compiled original first: make -C oracle")
@@ -220,6 +272,8 @@ def main() -> None:
         arrs = {}
         for name in ZONE_KERNEL_CASES:
             arrs.update(zone_kernel_case(name))
+        for name in MASK_ZONE_CASES:
+            arrs.update(zone_kernel_mask_case(name))
         np.savez_compressed(os.path.join(HERE, "zonekernel.npz"), **arrs)
         print("wrote zonekernel.npz")
         return

@@ -106,6 +106,17 @@ class Oracle:
                                                 ctypes.c_uint64(seed), _ptr(st), _ptr(dg), _ptr(mv, ctypes.c_int32)))
         return st, dg, mv
 
+    def zone_kernel_mask(self, mode: int, p: MpcdParams, x, v, mask, seed: int):
+        """run_zone_kernel_inplace on the zone given by mask (p.Nc bytes)"""
+        st = np.zeros(6)
+        dg = np.zeros(10)
+        mv = np.zeros(2, np.int32)
+        m = np.ascontiguousarray(mask, np.uint8)
+        self._check(self.lib.oracle_zone_kernel_mask(mode, ctypes.byref(p), _ptr(x), _ptr(v), _ptr(m, ctypes.c_uint8),
+                                                     ctypes.c_uint64(seed), _ptr(st), _ptr(dg),
+                                                     _ptr(mv, ctypes.c_int32)))
+        return st, dg, mv
+
     def zone_pass(self, mode: int, p: MpcdParams, x, v, layout: int, seed_base: int, gauss_seidel: bool = False):
         m = PassMetrics()
         owner = np.zeros(p.Nc, np.int32)

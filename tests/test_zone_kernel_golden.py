@@ -56,6 +56,53 @@ def test_zone_kernel_stats_and_diag_match_original(fix, case, layout):
         assert same_bits(dg, fix[key + "_diag"]), (key, dg, fix[key + "_diag"])
 
 
+MASK_CASES = ("small", "override", "cfg0")
+MASKS = ("notch", "ell", "scatter", "wrap")
+
+
+@pytest.mark.parametrize("case", MASK_CASES)
+@pytest.mark.parametrize("mask", MASKS)
+def test_masked_zone_matches_original(fix, case, mask):
+    """Zones that are not rectangles (a hole, an L, scattered cells, a band
+    across the periodic boundary): the zone is ZoneDescriptor.maskVec, which
+    filters the dense and low lists, the targets, the donors and the
+    correction (zone_pass.cpp:676-678, :611, :625, :263-278, :819, :877)."""
+    o = get_oracle()
+    p = to_struct(case_params(case))
+    x, v, _, _ = case_state(case)
+    key = f"{case}_mask_{mask}"
+    m = fix[key]
+    assert m.shape == (p.Nc,) and 0 < int(m.sum()) < p.Nc
+    xo, vo = x.copy(), v.copy()
+    st, dg, mv = o.zone_kernel_mask(STD, p, xo, vo, m, int(fix[key + "_seed"]))
+    assert sha(xo) == str(fix[key + "_x_sha256"]), key
+    assert sha(vo) == str(fix[key + "_v_sha256"]), key
+    assert np.array_equal(mv, fix[key + "_moved"]), key
+    assert same_bits(st, fix[key + "_stats"]), key
+    assert same_bits(dg, fix[key + "_diag"]), (key, dg, fix[key + "_diag"])
+
+
+def test_masked_zone_moves_stay_in_the_zone(fix):
+    """Every particle the masked kernel moves lands in a zone cell, and only
+    particles of zone cells move (oracle, counter draws)."""
+    from tests.oracle_lib import CTR
+    o = get_oracle()
+    p = to_struct(case_params("override"))
+    x, v, _, _ = case_state("override")
+    for mask in MASKS:
+        m = fix[f"override_mask_{mask}"]
+        xo, vo = x.copy(), v.copy()
+        st, _, mv = o.zone_kernel_mask(CTR, p, xo, vo, m, 4242)
+        moved = np.any(xo.reshape(-1, 2) != x.reshape(-1, 2), axis=1)
+        cx = np.floor(xo.reshape(-1, 2)[:, 0] / p.a0).astype(int) % p.Nx
+        cy = np.floor(xo.reshape(-1, 2)[:, 1] / p.a0).astype(int)
+        cx0 = np.floor(x.reshape(-1, 2)[:, 0] / p.a0).astype(int) % p.Nx
+        cy0 = np.floor(x.reshape(-1, 2)[:, 1] / p.a0).astype(int)
+        assert 0 < moved.sum() <= mv.sum(), mask  # a particle can move in several passes
+        assert np.all(m[cy[moved] + p.Ny * cx[moved]] != 0), mask
+        assert np.all(m[cy0[moved] + p.Ny * cx0[moved]] != 0), mask
+
+
 def test_zone_kernel_diag_closed_forms(fix):
     """What the diagnostics must satisfy whatever the draws: diag[6] NaN,
     diag[7] 0, and diag[0], diag[1] equal to the occupancy spread and the
