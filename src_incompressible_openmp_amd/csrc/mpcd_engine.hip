@@ -2036,6 +2036,7 @@ struct mpcd_gpu_sim {
 
     explicit mpcd_gpu_sim(const mpcd_params& prm) : p(prm) {
         validate(p);
+        if (p.n == 0) throw ApiError(MPCD_E_INVALID, "no particles: the GPU path needs n >= 1");
         g = make_geom(p);
         n = p.n;
         Nc = p.Nc;
@@ -3218,6 +3219,7 @@ int mpcd_gpu_compute_cell_fields_noalloc(const mpcd_params* p, const double* x, 
     return guarded([&] {
         if (!p || !x || !v || !out) throw ApiError(MPCD_E_INVALID, "null argument");
         validate(*p);
+        if (p->n == 0) throw ApiError(MPCD_E_INVALID, "no particles: the GPU path needs n >= 1");
         if (p->n > MPCD_NOALLOC_MAX_PARTICLES || p->Nc > MPCD_NOALLOC_MAX_CELLS)
             throw ApiError(MPCD_E_INVALID, "compute_cell_fields_noalloc: at most " +
                                                std::to_string(MPCD_NOALLOC_MAX_PARTICLES) + " particles and " +
@@ -3294,14 +3296,27 @@ static void run_zone_kernel_on(const mpcd_params* p, double* x, double* v, int x
         auto& s = scratch_sim(*p);
         s.set_state(x, v);
         s.bin_unshifted();
-        unsigned char* zmask = nullptr;
+        /* the temporaries are freed, and the simulation's diagnostics record
+         * detached, however the call ends */
+        struct Detach {
+            mpcd_gpu_sim& s;
+            ~Detach() {
+                s.zdCell = nullptr;
+                s.zdCnt = nullptr;
+            }
+        } detach{s};
+        DevBuf<unsigned char> zm;
+        DevBuf<ZoneRect> dr(1);
+        DevBuf<double> zdc;
+        DevBuf<int> zdn, low;
         if (mask) {
-            zmask = dalloc<unsigned char>((size_t)s.Nc);
-            HIP_OK(hipMemcpyAsync(zmask, mask, (size_t)s.Nc, hipMemcpyHostToDevice, s.stream));
+            zm.p = dalloc<unsigned char>((size_t)s.Nc);
+            HIP_OK(hipMemcpyAsync(zm.p, mask, (size_t)s.Nc, hipMemcpyHostToDevice, s.stream));
         }
+        const unsigned char* zmask = zm.p;
         ZoneRect r{xs, xe, ys, ye};
         const int C = (xe - xs) * (ye - ys);
-        ZoneRect* d = dalloc<ZoneRect>(1);
+        ZoneRect* d = dr.p;
         HIP_OK(hipMemcpyAsync(d, &r, sizeof(r), hipMemcpyHostToDevice, s.stream));
         s.clear_err();
         if (diag10) {
@@ -3309,8 +3324,10 @@ static void run_zone_kernel_on(const mpcd_params* p, double* x, double* v, int x
             const int nb = (int)nblk(s.n);
             hipLaunchKernelGGL(k_sum_ke, dim3(nb), dim3(TPB), 0, s.stream, s.vx, s.vy, s.n, s.part);
             hipLaunchKernelGGL(k_reduce_parts, dim3(1), dim3(TPB), 0, s.stream, s.part, nb, 3, s.diagDev + DIAG_ZK_KE0);
-            s.zdCell = dalloc<double>(4 * (size_t)C);
-            s.zdCnt = dalloc<int>((size_t)C);
+            zdc.p = dalloc<double>(4 * (size_t)C);
+            zdn.p = dalloc<int>((size_t)C);
+            s.zdCell = zdc.p;
+            s.zdCnt = zdn.p;
             hipLaunchKernelGGL(k_fill_double, dim3(nblk(4 * (size_t)C)), dim3(TPB), 0, s.stream, s.zdCell, 4 * C, 0.0);
             s.check_launch();
         }
@@ -3322,11 +3339,11 @@ static void run_zone_kernel_on(const mpcd_params* p, double* x, double* v, int x
             const int nb = (int)nblk(s.n);
             hipLaunchKernelGGL(k_sum_ke, dim3(nb), dim3(TPB), 0, s.stream, s.vx, s.vy, s.n, s.part);
             hipLaunchKernelGGL(k_reduce_parts, dim3(1), dim3(TPB), 0, s.stream, s.part, nb, 3, s.diagDev + DIAG_ZK_KE1);
-            int* low = dalloc<int>((size_t)s.Nc);
+            low.p = dalloc<int>((size_t)s.Nc);
             hipLaunchKernelGGL(k_zone_low_counts, dim3(nblk(s.Nc)), dim3(TPB), 0, s.stream, s.bin.cnt, s.Nc, p->Ny, r,
-                               s.zdCnt, low);
+                               s.zdCnt, low.p);
             s.check_launch();
-            s.occupancy(s.bin.cnt, DIAG_ZK_OCC, low);
+            s.occupancy(s.bin.cnt, DIAG_ZK_OCC, low.p);
             hipLaunchKernelGGL(k_zone_diag_cells, dim3(1), dim3(TPB), 0, s.stream, s.zdCell, s.bin.cnt, p->Ny, r,
                                s.diagDev + DIAG_ZK_CELLS);
             hipLaunchKernelGGL(k_zone_diag_final, dim3(1), dim3(64), 0, s.stream, s.diagDev + DIAG_ZK_OCC,
@@ -3336,17 +3353,10 @@ static void run_zone_kernel_on(const mpcd_params* p, double* x, double* v, int x
             HIP_OK(hipMemcpyAsync(diag10, s.diagDev + DIAG_ZK_OUT, 10 * sizeof(double), hipMemcpyDeviceToHost,
                                   s.stream));
             HIP_OK(hipStreamSynchronize(s.stream));
-            HIP_OK(hipFree(low));
-            HIP_OK(hipFree(s.zdCell));
-            HIP_OK(hipFree(s.zdCnt));
-            s.zdCell = nullptr;
-            s.zdCnt = nullptr;
         }
         ZoneOut o{};
         HIP_OK(hipMemcpyAsync(&o, s.zout, sizeof(o), hipMemcpyDeviceToHost, s.stream));
         s.get_state(x, v);
-        HIP_OK(hipFree(d));
-        if (zmask) HIP_OK(hipFree(zmask));
         s.check_err();
         if (stats6) {
             stats6[0] = (double)o.nHigh;

@@ -34,6 +34,20 @@ T* dalloc(size_t n) {
     return (T*)p;
 }
 
+/* Device memory owned by a scope: freed when the scope ends, also when an
+ * error unwinds it.  Empty until given an allocation. */
+template <class T>
+struct DevBuf {
+    T* p = nullptr;
+    DevBuf() = default;
+    explicit DevBuf(size_t n) : p(dalloc<T>(n)) {}
+    ~DevBuf() {
+        if (p) (void)hipFree(p);
+    }
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+};
+
 /* the calling thread's last error message (mpcd_gpu_last_error) */
 void set_last_error(const std::string& m);
 

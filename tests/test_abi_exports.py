@@ -4,6 +4,7 @@ import ctypes
 import os
 import re
 
+import numpy as np
 import pytest
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
@@ -56,6 +57,22 @@ def test_error_path_without_gpu_is_an_error_code(lib):
         assert n.value >= 0
     else:
         assert rc < 0 and lib.mpcd_gpu_last_error()
+
+
+def test_no_particles_is_refused_before_any_device_call():
+    """n = 0 is refused with MPCD_E_INVALID and a message by the simulation
+    and by the stateless entry points that build one, before any device
+    call (so the same holds on a box without a GPU)."""
+    from src_incompressible_openmp_amd import gpu
+    from src_incompressible_openmp_amd.params import to_struct
+    from tests.golden.cases import case_params
+
+    p = to_struct(dict(case_params("small"), n=0))
+    with pytest.raises(gpu.MpcdError, match="no particles") as e:
+        gpu.Sim(p)
+    assert e.value.code == -1  # MPCD_E_INVALID
+    with pytest.raises(gpu.MpcdError, match="no particles"):
+        gpu.compute_cell_fields(p, np.zeros(0), np.zeros(0), 10.0)
 
 
 def test_profile_classes_match_header():
