@@ -3085,8 +3085,10 @@ int mpcd_gpu_occupancy_stats(const mpcd_params* p, const int32_t* counts, mpcd_o
         const ZoneParams zp = make_zone_params(*p);
         const int Nc = p->Nc;
         const int nb = (int)std::min<size_t>(nblk(Nc), 512);
-        int* dc = dalloc<int>((size_t)Nc);
-        double* dpart = dalloc<double>((size_t)nb * OCC_W + 8);
+        DevBuf<int> dcb((size_t)Nc);
+        DevBuf<double> dpartb((size_t)nb * OCC_W + 8);
+        int* dc = dcb.p;
+        double* dpart = dpartb.p;
         hipStream_t st = nullptr;
         HIP_OK(hipMemcpy(dc, counts, (size_t)Nc * sizeof(int), hipMemcpyHostToDevice));
         hipLaunchKernelGGL(k_occupancy, dim3(nb), dim3(TPB), 0, st, dc, dc, p->Nx, p->Ny, zp, dpart);
@@ -3095,8 +3097,6 @@ int mpcd_gpu_occupancy_stats(const mpcd_params* p, const int32_t* counts, mpcd_o
         HIP_OK(hipGetLastError());
         double h[5];
         HIP_OK(hipMemcpy(h, dpart + (size_t)nb * OCC_W, sizeof(h), hipMemcpyDeviceToHost));
-        HIP_OK(hipFree(dc));
-        HIP_OK(hipFree(dpart));
         out->occStd = h[0];
         out->outBand = h[1];
         out->nDense = (int32_t)h[2];
@@ -3193,9 +3193,11 @@ int mpcd_gpu_compute_cell_fields(const mpcd_params* p, const double* x, const do
         auto& s = scratch_sim(*p);
         s.set_state(x, v);
         s.bin_unshifted();
-        std::vector<double*> d(9);
-        for (auto& q : d) q = dalloc<double>((size_t)s.Nc);
-        int* dN = dalloc<int>((size_t)s.Nc);
+        DevBuf<double> db[9];
+        double* d[9];
+        for (int k = 0; k < 9; ++k) d[k] = db[k].p = dalloc<double>((size_t)s.Nc);
+        DevBuf<int> dNb((size_t)s.Nc);
+        int* dN = dNb.p;
         FieldsOut f{dN, d[0], d[1], d[2], d[3], d[4], d[5], d[6], d[7], d[8]};
         hipLaunchKernelGGL(k_cell_fields, dim3(nblk(s.Nc, CELL_BLK)), dim3(CELL_BLK), s.cellCap * 2 * sizeof(double),
                            s.stream, s.bin.start, s.bin.list, s.Nc, s.vx, s.vy, s.vc(), p->Kvirial, rhoTarget, f,
@@ -3205,12 +3207,10 @@ int mpcd_gpu_compute_cell_fields(const mpcd_params* p, const double* x, const do
         if (out->N) HIP_OK(hipMemcpyAsync(out->N, dN, (size_t)s.Nc * sizeof(int), hipMemcpyDeviceToHost, s.stream));
         double* hs[9] = {out->Ux, out->Uy, out->Px, out->Py, out->T, out->rho, out->Pkin, out->Pvir, out->P};
         for (int k = 0; k < 9; ++k)
-            if (hs[k]) HIP_OK(hipMemcpyAsync(hs[k], d[(size_t)k], nb, hipMemcpyDeviceToHost, s.stream));
+            if (hs[k]) HIP_OK(hipMemcpyAsync(hs[k], d[k], nb, hipMemcpyDeviceToHost, s.stream));
         HIP_OK(hipStreamSynchronize(s.stream));
         if (out->rhoTarget)
             for (int c = 0; c < s.Nc; ++c) out->rhoTarget[c] = rhoTarget;
-        for (auto q : d) HIP_OK(hipFree(q));
-        HIP_OK(hipFree(dN));
     });
 }
 
@@ -3435,10 +3435,17 @@ int mpcd_gpu_zone_pass(const mpcd_params* p, double* x, double* v, int layoutMod
                 ov[(size_t)c] = 1;
                 oc.push_back(c);
             }
-        ZoneRect* d = dalloc<ZoneRect>(r.size());
-        int* dOwn = dalloc<int>(own.size());
-        unsigned char* dCov = oc.empty() ? nullptr : dalloc<unsigned char>(ov.size());
-        int* dOver = oc.empty() ? nullptr : dalloc<int>(oc.size());
+        DevBuf<ZoneRect> db(r.size());
+        DevBuf<int> dOwnb(own.size()), dOverb;
+        DevBuf<unsigned char> dCovb;
+        if (!oc.empty()) {
+            dCovb.p = dalloc<unsigned char>(ov.size());
+            dOverb.p = dalloc<int>(oc.size());
+        }
+        ZoneRect* d = db.p;
+        int* dOwn = dOwnb.p;
+        unsigned char* dCov = dCovb.p;
+        int* dOver = dOverb.p;
         HIP_OK(hipMemcpyAsync(d, r.data(), r.size() * sizeof(ZoneRect), hipMemcpyHostToDevice, s.stream));
         HIP_OK(hipMemcpyAsync(dOwn, own.data(), own.size() * sizeof(int), hipMemcpyHostToDevice, s.stream));
         if (dCov) {
@@ -3452,10 +3459,6 @@ int mpcd_gpu_zone_pass(const mpcd_params* p, double* x, double* v, int layoutMod
         mpcd_zone_pass_metrics mm{};
         s.pass_metrics((int)r.size(), &mm);
         s.get_state(x, v);
-        HIP_OK(hipFree(d));
-        HIP_OK(hipFree(dOwn));
-        if (dCov) HIP_OK(hipFree(dCov));
-        if (dOver) HIP_OK(hipFree(dOver));
         if (m) *m = mm;
         if (ownerMap) std::memcpy(ownerMap, own.data(), own.size() * sizeof(int32_t));
     });
@@ -3599,9 +3602,12 @@ int mpcd_gpu_closure_training_pair(const mpcd_params* p, const int32_t* refN, co
         if (!(outScale > 0.0)) throw ApiError(MPCD_E_INVALID, "outScale must be positive");
         const Geom g = make_geom(*p);
         const size_t C = (size_t)p->Nc;
-        int* dN = dalloc<int>(C);
-        double* d = dalloc<double>(8 * C + 8);
-        float* f = dalloc<float>(6 * C);
+        DevBuf<int> dNb(C);
+        DevBuf<double> db(8 * C + 8);
+        DevBuf<float> fb(6 * C);
+        int* dN = dNb.p;
+        double* d = db.p;
+        float* f = fb.p;
         hipStream_t st = nullptr;
         HIP_OK(hipMemcpy(dN, refN, C * sizeof(int), hipMemcpyHostToDevice));
         const double* hs[7] = {refUx, refUy, refT, duRepairX, duRepairY, duEOSX, duEOSY};
@@ -3615,9 +3621,6 @@ int mpcd_gpu_closure_training_pair(const mpcd_params* p, const int32_t* refN, co
         HIP_OK(hipGetLastError());
         if (feat) HIP_OK(hipMemcpy(feat, f, 4 * C * sizeof(float), hipMemcpyDeviceToHost));
         if (target) HIP_OK(hipMemcpy(target, f + 4 * C, 2 * C * sizeof(float), hipMemcpyDeviceToHost));
-        HIP_OK(hipFree(dN));
-        HIP_OK(hipFree(d));
-        HIP_OK(hipFree(f));
     });
 }
 
@@ -3632,7 +3635,8 @@ int mpcd_gpu_zone_pass_gauss_seidel(const mpcd_params* p, double* x, double* v, 
         s.set_state(x, v);
         std::vector<int32_t> own;
         const auto r = layout_rects(*p, layoutMode, &own);
-        ZoneRect* d = dalloc<ZoneRect>(r.size());
+        DevBuf<ZoneRect> db(r.size());
+        ZoneRect* d = db.p;
         HIP_OK(hipMemcpyAsync(d, r.data(), r.size() * sizeof(ZoneRect), hipMemcpyHostToDevice, s.stream));
         s.ensure_zout(1);
         s.clear_err();
@@ -3663,7 +3667,6 @@ int mpcd_gpu_zone_pass_gauss_seidel(const mpcd_params* p, double* x, double* v, 
         mm.corrRmsDU = cnt > 0 ? std::sqrt(acc / (double)cnt) : 0.0;
         mm.corrMaxDU = mx;
         s.get_state(x, v);
-        HIP_OK(hipFree(d));
         if (m) *m = mm;
         if (ownerMap) std::memcpy(ownerMap, own.data(), own.size() * sizeof(int32_t));
     });
