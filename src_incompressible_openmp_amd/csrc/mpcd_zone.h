@@ -1361,6 +1361,127 @@ ZHD bool dense_conflict(const Z& z, int a, int b) {
     return dx <= 4 && dy <= 4;
 }
 
+/* The levels of the dense list z.dl[0..m) into z.ext[0..m) (0 for a cell
+ * that is skipped) and their count into z.sh->mwLevels, on all threads of
+ * the workgroup.  A cell's level is one more than the highest level of the
+ * earlier cells it conflicts with: those within Chebyshev range 4, and every
+ * earlier and later cell when either takes the one-lane form.  That is the
+ * longest path to the cell in the order's conflict graph, which the serial
+ * rule (one more than the earlier conflicting cells', at least one more than
+ * the last one-lane cell, a one-lane cell above every earlier cell) gives
+ * as well.  Rounds of max-updates raise every level towards that fixed
+ * point and stop when a round changes none; a level depends only on earlier
+ * cells, so this takes at most m + 1 rounds.  Each thread holds one cell.
+ * Working word of a cell before the rounds: 1 | ONE | LIVE. */
+constexpr unsigned DL_LIVE = 0x8000u, DL_ONE = 0x4000u, DL_LEVEL = 0x3fffu;
+
+/* The rounds, out of line: the zone kernel holds its per-zone context in
+ * registers, and inlined these loops would spill to scratch.
+ * Conflicts are within Chebyshev range 4, so each live cell finds its
+ * earlier conflicting cells in its 9 x 9 window through the list position of
+ * every zone cell (pos, after the m working words in ext), once, as a mask of
+ * list positions; each round then reads only those cells' levels.  Returns
+ * 0, having changed nothing the serial rule reads, when the list holds a
+ * one-lane cell, is longer than the workgroup or leaves ext no room for pos. */
+template <class IT>
+__device__ __attribute__((noinline)) int dense_level_rounds(const IT* __restrict__ dl, IT* __restrict__ ext, int nExt,
+                                                           int m, int C, int zw, int zh, int wrapX, int wrapY,
+                                                           int* levels, int tid) {
+    constexpr int NT = 64 * ZONE_WAVES;
+    constexpr unsigned NONE = (unsigned)(IT)~(IT)0;
+    bool one = false;
+    unsigned wk = 0u;
+    if (tid < m) {
+        wk = (unsigned)ext[tid];
+        one = (wk & DL_LIVE) && (wk & DL_ONE);
+    }
+    if (__syncthreads_or(one) || m > NT || m + C > nExt) return 0;
+    IT* pos = ext + m;
+    for (int lc = tid; lc < C; lc += NT) pos[lc] = (IT)NONE;
+    __syncthreads();
+    const bool live = (wk & DL_LIVE) != 0u;
+    int cx = 0, cy = 0;
+    if (live) {
+        const int c = (int)dl[tid];
+        cx = c / zh;
+        cy = c - cx * zh;
+        pos[c] = (IT)tid;
+    }
+    __syncthreads();
+    unsigned b0 = 0u, b1 = 0u, b2 = 0u, b3 = 0u, b4 = 0u, b5 = 0u, b6 = 0u, b7 = 0u;
+    if (live)
+        for (int dx = -4; dx <= 4; ++dx) {
+            int x = cx + dx;
+            if (wrapX) x = x < 0 ? x + zw : (x >= zw ? x - zw : x);
+            else if (x < 0 || x >= zw) continue;
+            for (int dy = -4; dy <= 4; ++dy) {
+                int y = cy + dy;
+                if (wrapY) y = y < 0 ? y + zh : (y >= zh ? y - zh : y);
+                else if (y < 0 || y >= zh) continue;
+                const unsigned q = (unsigned)pos[x * zh + y];
+                if (q < (unsigned)tid) {
+                    const unsigned bit = 1u << (q & 31u), w = q >> 5;
+                    b0 |= w == 0u ? bit : 0u;
+                    b1 |= w == 1u ? bit : 0u;
+                    b2 |= w == 2u ? bit : 0u;
+                    b3 |= w == 3u ? bit : 0u;
+                    b4 |= w == 4u ? bit : 0u;
+                    b5 |= w == 5u ? bit : 0u;
+                    b6 |= w == 6u ? bit : 0u;
+                    b7 |= w == 7u ? bit : 0u;
+                }
+            }
+        }
+    int lv = live ? 1 : 0;
+    if (tid < m) ext[tid] = (IT)lv;
+    __syncthreads();
+    for (;;) {
+        bool changed = false;
+        if (live) {
+            int best = 0;
+            const unsigned bw[8] = {b0, b1, b2, b3, b4, b5, b6, b7};
+#pragma unroll
+            for (int w = 0; w < 8; ++w)
+                for (unsigned mm = bw[w]; mm; mm &= mm - 1u) best = imax(best, (int)ext[32 * w + __builtin_ctz(mm)]);
+            if (best + 1 != lv) {
+                lv = best + 1;
+                ext[tid] = (IT)lv;
+                changed = true;
+            }
+        }
+        if (!__syncthreads_or(changed)) break;
+    }
+    int top = lv;
+    for (int o = 32; o > 0; o >>= 1) top = imax(top, __shfl_xor(top, o, 64));
+    if ((tid & 63) == 0) atomicMax(levels, top);
+    return 1;
+}
+
+/* Levels of the dense list into z.ext[0..m) and their count into
+ * z.sh->mwLevels on all threads, or false (the same on every thread) when
+ * the serial rule must assign them. */
+template <class Z>
+__device__ __forceinline__ bool dense_levels(Z& z, int tid) {
+    typedef typename Z::idx_t IT;
+    const int m = z.sh->mwCount;
+    for (int k = tid; k < m; k += 64 * ZONE_WAVES) {
+        const int c = (int)z.dl[k];
+        const int nc = cnt_of(z, c);
+        const double hl = high_thr(z, c);
+        unsigned w = 0u;
+        if (nc > 0 && (double)nc > hl) {
+            const int needOut = imax(0, (int)MPCD_CEIL((double)nc - hl));
+            const bool one = !(needOut > 0 && imin(needOut + 1, nc) <= ZONE_MAX_MOVE);
+            w = DL_LIVE | (one ? DL_ONE : 0u) | 1u;
+        }
+        z.ext[k] = (IT)w;
+    }
+    __syncthreads();
+    return dense_level_rounds<IT>(z.dl, z.ext, (z.zw + 4) * (z.zh + 4), m, z.C, z.zw, z.zh,
+                                  (int)(z.g.perx && z.zw == z.g.Nx), (int)(z.g.pery && z.zh == z.g.Ny),
+                                  &z.sh->mwLevels, tid) != 0;
+}
+
 template <class Z>
 __device__ __forceinline__ void zone_dense_passes_mw(Z& z, int wave, int lane) {
     typedef typename Z::idx_t IT;
@@ -1374,7 +1495,15 @@ __device__ __forceinline__ void zone_dense_passes_mw(Z& z, int wave, int lane) {
             const int m = zone_compact(z, lane, 64, ZC_DENSE);
             ZBAR();
             zone_shuffle(z.dl, m, lane, 64, ShufDraw{z.seed, ZS_DENSE, (uint32_t)pass, 0u});
-            /* levels in z.ext[0..m) (free until the sparse phase); 0 = skipped */
+            if (lane == 0) {
+                z.sh->mwCount = m;
+                z.sh->mwLevels = 0;
+            }
+        }
+        __syncthreads();
+        /* levels in z.ext[0..m) (free until the sparse phase); 0 = skipped */
+        if (!dense_levels(z, tid) && wave == 0) {
+            const int m = z.sh->mwCount;
             int floorLev = 0, maxLev = 0;
             for (int k = 0; k < m; ++k) {
                 const int c = (int)z.dl[k];
