@@ -1277,6 +1277,7 @@ __global__ void __launch_bounds__(FUSED_TPB, 4) k_zone_fused(ZoneArgs a) { /* 4 
     __syncthreads();
     /* the dense passes and the sparse phase on all four wavefronts */
     zone_dense_passes_mw(z, wave, lane);
+    if (threadIdx.x == 0 && sh.schedWait) atomicOr(a.err, 16);
     zone_sparse_pass_mw(z, wave, lane);
     if (wave == 0) {
         zone_dump(z, zone_state_at<IT>(a.zstate + (size_t)zi * a.zsStride, z.C, z.zw), lane, ZONE_TPB);
@@ -2300,6 +2301,7 @@ struct mpcd_gpu_sim {
         if (h & 1) throw ApiError(MPCD_E_BC_MODE, "Unsupported boundary mode at a wall a particle crossed");
         if (h & 2) throw ApiError(MPCD_E_CAPACITY, "zone scratch capacity exceeded");
         if (h & 4) throw ApiError(MPCD_E_INVALID, "binning update: a cell's stayers do not match its count");
+        if (h & 16) throw ApiError(MPCD_E_INVALID, "dense phase: a wait for earlier cells hit its bound");
     }
 
     /* ---- stages ---- */

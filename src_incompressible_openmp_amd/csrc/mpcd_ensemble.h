@@ -978,6 +978,7 @@ struct mpcd_gpu_ensemble {
             if (h & 1) throw ApiError(MPCD_E_BC_MODE, "a particle hit a wall of unsupported mode");
             if (h & 8) throw ApiError(MPCD_E_CAPACITY, "a zone holds more particles than the 16-bit working set");
             if (h & 4) throw ApiError(MPCD_E_INVALID, "binning update: a cell's stayers do not match its count");
+            if (h & 16) throw ApiError(MPCD_E_INVALID, "dense phase: a wait for earlier cells hit its bound");
             throw ApiError(MPCD_E_CAPACITY, "zone scratch exceeded");
         }
     }

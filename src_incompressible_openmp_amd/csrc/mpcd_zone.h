@@ -217,6 +217,8 @@ struct ZoneIn {
 struct ZoneShared {
     int nOrd, startBin, nr, stage, P, allOwned;
     int mwCount, mwLevels; /* the multi-wavefront dense phase's list length and level count */
+    int mwNext;            /* the dependency-driven dense phase's next list position */
+    int schedWait;         /* nonzero: a wait of the dependency-driven dense phase hit its bound */
     int nHigh, nLow, movedDense, movedSparse;
     double stats4, stats5;
 };
@@ -1373,7 +1375,7 @@ ZHD bool dense_conflict(const Z& z, int a, int b) {
  * point and stop when a round changes none; a level depends only on earlier
  * cells, so this takes at most m + 1 rounds.  Each thread holds one cell.
  * Working word of a cell before the rounds: 1 | ONE | LIVE. */
-constexpr unsigned DL_LIVE = 0x8000u, DL_ONE = 0x4000u, DL_LEVEL = 0x3fffu;
+constexpr unsigned DL_LIVE = 0x8000u, DL_ONE = 0x4000u, DL_DONE = 0x2000u, DL_LEVEL = 0x3fffu;
 
 /* The rounds, out of line: the zone kernel holds its per-zone context in
  * registers, and inlined these loops would spill to scratch.
@@ -1482,12 +1484,112 @@ __device__ __forceinline__ bool dense_levels(Z& z, int tid) {
                                   &z.sh->mwLevels, tid) != 0;
 }
 
+/* The dense list without levels: the wavefronts take its cells in list
+ * order from a shared counter, and a cell starts once every earlier live
+ * cell within Chebyshev range 4 (found through pos, the list position of
+ * each zone cell, after the m working words in ext) is done.  Conflicting
+ * cells thus keep the list order, as with levels, and a wavefront waits only
+ * for the cells its own depends on.  Cells are taken in increasing order and
+ * wait only on smaller positions, which are taken already, so the smallest
+ * unfinished cell can always run.  Each wait is bounded all the same: past
+ * DS_WAIT_MAX polls it records the fact (schedWait, reported as an error)
+ * and goes on.  Returns false, having run nothing, for a list with a
+ * one-lane cell or one that leaves ext no room for pos; the caller then
+ * uses the levels. */
+constexpr int DS_WAIT_MAX = 1 << 22;
+
+template <class Z>
+__device__ __forceinline__ bool dense_dynamic(Z& z, int tid, int wave, int lane, int pass, int& nHighW,
+                                              int& movedW) {
+    typedef typename Z::idx_t IT;
+    constexpr int NT = 64 * ZONE_WAVES;
+    constexpr unsigned NONE = (unsigned)(IT)~(IT)0;
+    const int m = z.sh->mwCount;
+    const int nExt = (z.zw + 4) * (z.zh + 4);
+    bool one = false;
+    for (int k = tid; k < m; k += NT) {
+        const int c = (int)z.dl[k];
+        const int nc = cnt_of(z, c);
+        const double hl = high_thr(z, c);
+        unsigned w = 0u;
+        if (nc > 0 && (double)nc > hl) {
+            const int needOut = imax(0, (int)MPCD_CEIL((double)nc - hl));
+            one = one || !(needOut > 0 && imin(needOut + 1, nc) <= ZONE_MAX_MOVE);
+            w = DL_LIVE;
+        }
+        z.ext[k] = (IT)w;
+    }
+    if (__syncthreads_or(one) || m + z.C > nExt) return false;
+    IT* pos = z.ext + m;
+    for (int lc = tid; lc < z.C; lc += NT) pos[lc] = (IT)NONE;
+    __syncthreads();
+    for (int k = tid; k < m; k += NT)
+        if ((unsigned)z.ext[k] & DL_LIVE) pos[(int)z.dl[k]] = (IT)k;
+    if (tid == 0) z.sh->mwNext = 0;
+    __syncthreads();
+    const bool wrapX = z.g.perx && z.zw == z.g.Nx, wrapY = z.g.pery && z.zh == z.g.Ny;
+    volatile IT* vext = z.ext;
+    for (;;) {
+        int k = 0;
+        if (lane == 0) k = atomicAdd(&z.sh->mwNext, 1);
+        k = __shfl(k, 0, 64);
+        if (k >= m) break;
+        if (!((unsigned)vext[k] & DL_LIVE)) continue;
+        const int c = (int)z.dl[k];
+        /* the window's earlier live cells: lane t checks offsets t and t + 64 of the 81 */
+        int q0 = -1, q1 = -1;
+        {
+            const int cx = c / z.zh, cy = c - cx * z.zh;
+            for (int h = 0; h < 2; ++h) {
+                const int t = lane + 64 * h;
+                int qq = -1;
+                if (t < 81) {
+                    int x = cx + t / 9 - 4, y = cy + t % 9 - 4;
+                    bool in = true;
+                    if (wrapX) x = x < 0 ? x + z.zw : (x >= z.zw ? x - z.zw : x);
+                    else in = in && x >= 0 && x < z.zw;
+                    if (wrapY) y = y < 0 ? y + z.zh : (y >= z.zh ? y - z.zh : y);
+                    else in = in && y >= 0 && y < z.zh;
+                    if (in) {
+                        const unsigned pq = (unsigned)pos[x * z.zh + y];
+                        if (pq < (unsigned)k) qq = (int)pq;
+                    }
+                }
+                if (h == 0) q0 = qq;
+                else q1 = qq;
+            }
+        }
+        for (int polls = 0;; ++polls) {
+            const bool wait0 = q0 >= 0 && !((unsigned)vext[q0] & DL_DONE);
+            const bool wait1 = q1 >= 0 && !((unsigned)vext[q1] & DL_DONE);
+            if (wave_ballot(wait0 || wait1) == 0ull) break;
+            if (polls >= DS_WAIT_MAX) {
+                if (lane == 0) z.sh->schedWait = 1;
+                break;
+            }
+            __builtin_amdgcn_s_sleep(1);
+        }
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+        const int nc = cnt_of(z, c);
+        const double hl = high_thr(z, c);
+        const int needOut = imax(0, (int)MPCD_CEIL((double)nc - hl));
+        ++nHighW;
+        movedW += dense_cell_wave(z, c, nc, hl, needOut, pass, lane);
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        if (lane == 0) vext[k] = (IT)(DL_LIVE | DL_DONE);
+        ZWSYNC();
+    }
+    __syncthreads();
+    return true;
+}
+
 template <class Z>
 __device__ __forceinline__ void zone_dense_passes_mw(Z& z, int wave, int lane) {
     typedef typename Z::idx_t IT;
     const int tid = wave * 64 + lane;
     int nHighW = 0, movedW = 0;
     if (wave > 0) z.mvk = reinterpret_cast<int*>(z.cand) + (wave - 1) * ZONE_MAX_MOVE;
+    if (tid == 0) z.sh->schedWait = 0;
     for (int pass = 1; pass <= z.zp.maxPasses; ++pass) {
         for (int lc = tid; lc < z.C; lc += 64 * ZONE_WAVES) zone_maps(z, lc, 0);
         __syncthreads();
@@ -1501,6 +1603,7 @@ __device__ __forceinline__ void zone_dense_passes_mw(Z& z, int wave, int lane) {
             }
         }
         __syncthreads();
+        if (dense_dynamic(z, tid, wave, lane, pass, nHighW, movedW)) continue;
         /* levels in z.ext[0..m) (free until the sparse phase); 0 = skipped */
         if (!dense_levels(z, tid) && wave == 0) {
             const int m = z.sh->mwCount;
