@@ -1279,6 +1279,7 @@ __global__ void __launch_bounds__(FUSED_TPB, 4) k_zone_fused(ZoneArgs a) { /* 4 
     zone_dense_passes_mw(z, wave, lane);
     if (threadIdx.x == 0 && sh.schedWait) atomicOr(a.err, 16);
     zone_sparse_pass_mw(z, wave, lane);
+    if (threadIdx.x == 0 && sh.schedWait) atomicOr(a.err, 16);
     if (wave == 0) {
         zone_dump(z, zone_state_at<IT>(a.zstate + (size_t)zi * a.zsStride, z.C, z.zw), lane, ZONE_TPB);
         if (lane == 0) {

@@ -1976,94 +1976,142 @@ __device__ __forceinline__ void zone_sparse_pass_mw(Z& z, int wave, int lane) {
     }
     __syncthreads();
     const int m = z.sh->mwCount;
+    bool anyAlone = false;
     for (int k = tid; k < m; k += 64 * ZONE_WAVES) {
         const int c = (int)z.dl[k];
         z.lix[c] = (IT)k;
-        if (sparse_alone(z, c)) z.fl[c] = (unsigned short)(z.fl[c] | ZF_ALONE);
+        if (sparse_alone(z, c)) {
+            z.fl[c] = (unsigned short)(z.fl[c] | ZF_ALONE);
+            anyAlone = true;
+        }
     }
-    __syncthreads();
+    anyAlone = __syncthreads_or(anyAlone);
     const int lowInt = (int)MPCD_CEIL(z.zp.lowThrBulk);
     const bool wx = z.g.perx && z.zw == z.g.Nx, wy = z.g.pery && z.zh == z.g.Ny;
     int nLowW = 0, movedW = 0;
-    /* the first unfinished cell and the first unfinished one that runs alone */
-    auto scan = [&](int& done, int& first, int& firstAlone) {
-        done = 0;
-        first = m;
-        firstAlone = m;
-        for (int b0 = 0; b0 < m; b0 += 64) {
-            const int k = b0 + lane;
-            const unsigned short f = k < m ? z.fl[(int)z.dl[k]] : (unsigned short)ZF_DONE;
-            const unsigned long long dm = wave_ballot(f & ZF_DONE);
-            done += wave_popc(dm & (b0 + 64 <= m ? ~0ull : lanes_below(m - b0)));
-            const unsigned long long um = ~dm;
-            if (first == m && um) first = b0 + (int)__builtin_ctzll(um);
-            const unsigned long long am = wave_ballot(!(f & ZF_DONE) && (f & ZF_ALONE));
-            if (firstAlone == m && am) firstAlone = b0 + (int)__builtin_ctzll(am);
-        }
-    };
-    int done, first, firstAlone;
-    scan(done, first, firstAlone);
-    for (int round = 0; round < m && done < m; ++round) {
-        /* 1. the round's ready cells, marked ZF_READY: the first unfinished
-         * cell, and every cell before the first unfinished lone one that
-         * runs with others and has no unfinished earlier cell within range 2
-         * (lix of finished cells is cleared) */
-        for (int k = tid; k < m; k += 64 * ZONE_WAVES) {
+    if (!anyAlone) {
+        /* No cell runs alone: the wavefronts take the cells in list order and
+         * each waits only until no earlier unfinished cell lies within range
+         * 2 of its own (lix of a finished cell is cleared), the waits bounded
+         * as in the dense phase's schedule. */
+        if (tid == 0) z.sh->mwNext = 0;
+        __syncthreads();
+        volatile IT* vlix = z.lix;
+        for (;;) {
+            int k = 0;
+            if (lane == 0) k = atomicAdd(&z.sh->mwNext, 1);
+            k = __shfl(k, 0, 64);
+            if (k >= m) break;
             const int c = (int)z.dl[k];
-            const unsigned short f = z.fl[c];
-            if (f & ZF_DONE) continue;
-            bool ready = k == first;
-            if (!ready && k < firstAlone && !(f & ZF_ALONE)) {
-                ready = true;
+            int q = -1;
+            if (lane < 25) {
                 const int rx = c / z.zh, ry = c - rx * z.zh;
-                for (int dx = -2; dx <= 2 && ready; ++dx)
-                    for (int dy = -2; dy <= 2; ++dy) {
-                        int qx = rx + dx, qy = ry + dy;
-                        if (wx) qx = wrapi(qx, z.zw);
-                        if (wy) qy = wrapi(qy, z.zh);
-                        if (qx < 0 || qx >= z.zw || qy < 0 || qy >= z.zh) continue;
-                        if ((int)z.lix[qx * z.zh + qy] < k) {
-                            ready = false;
-                            break;
-                        }
-                    }
+                int qx = rx + lane / 5 - 2, qy = ry + lane % 5 - 2;
+                if (wx) qx = wrapi(qx, z.zw);
+                if (wy) qy = wrapi(qy, z.zh);
+                if (qx >= 0 && qx < z.zw && qy >= 0 && qy < z.zh) q = qx * z.zh + qy;
             }
-            if (ready) z.fl[c] = (unsigned short)(f | ZF_READY);
+            for (int polls = 0;; ++polls) {
+                const bool wait = q >= 0 && (int)vlix[q] < k;
+                if (wave_ballot(wait) == 0ull) break;
+                if (polls >= DS_WAIT_MAX) {
+                    if (lane == 0) z.sh->schedWait = 1;
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(1);
+            }
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+            const int nc = cnt_of(z, c);
+            if (nc > 0) {
+                ++nLowW;
+                movedW += sparse_cell(z, c, nc, lowInt, lane, 64);
+            }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+            if (lane == 0) vlix[c] = none;
+            ZWSYNC();
         }
         __syncthreads();
-        /* 2. the ready cells, the t-th on wavefront t mod 4; ZF_READY stays
-         * as it is until every wavefront has dealt the round */
-        int t = 0;
-        for (int b0 = 0; b0 < m; b0 += 64) {
-            const int k = b0 + lane;
-            unsigned long long msk = wave_ballot(k < m && (z.fl[(int)z.dl[k]] & ZF_READY));
-            while (msk) {
-                const int bit = (int)__builtin_ctzll(msk);
-                msk &= msk - 1ull;
-                if ((t++ % ZONE_WAVES) != wave) continue;
-                const int c = (int)z.dl[b0 + bit];
-                const int nc = cnt_of(z, c);
-                if (nc > 0) {
-                    ++nLowW;
-                    movedW += sparse_cell(z, c, nc, lowInt, lane, 64);
-                }
-                ZWSYNC();
-                if (lane == 0) {
-                    z.fl[c] = (unsigned short)(z.fl[c] | ZF_DONE);
-                    z.lix[c] = none;
-                }
-                ZWSYNC();
+    } else {
+        /* the first unfinished cell and the first unfinished one that runs alone */
+        auto scan = [&](int& done, int& first, int& firstAlone) {
+            done = 0;
+            first = m;
+            firstAlone = m;
+            for (int b0 = 0; b0 < m; b0 += 64) {
+                const int k = b0 + lane;
+                const unsigned short f = k < m ? z.fl[(int)z.dl[k]] : (unsigned short)ZF_DONE;
+                const unsigned long long dm = wave_ballot(f & ZF_DONE);
+                done += wave_popc(dm & (b0 + 64 <= m ? ~0ull : lanes_below(m - b0)));
+                const unsigned long long um = ~dm;
+                if (first == m && um) first = b0 + (int)__builtin_ctzll(um);
+                const unsigned long long am = wave_ballot(!(f & ZF_DONE) && (f & ZF_ALONE));
+                if (firstAlone == m && am) firstAlone = b0 + (int)__builtin_ctzll(am);
             }
-        }
-        __syncthreads();
-        /* 3. clear the round's marks; every wavefront scans the flags the
-         * same way */
-        for (int k = tid; k < m; k += 64 * ZONE_WAVES) {
-            const int c = (int)z.dl[k];
-            z.fl[c] = (unsigned short)(z.fl[c] & ~ZF_READY);
-        }
+        };
+        int done, first, firstAlone;
         scan(done, first, firstAlone);
-        __syncthreads();
+        for (int round = 0; round < m && done < m; ++round) {
+            /* 1. the round's ready cells, marked ZF_READY: the first unfinished
+             * cell, and every cell before the first unfinished lone one that
+             * runs with others and has no unfinished earlier cell within range 2
+             * (lix of finished cells is cleared) */
+            for (int k = tid; k < m; k += 64 * ZONE_WAVES) {
+                const int c = (int)z.dl[k];
+                const unsigned short f = z.fl[c];
+                if (f & ZF_DONE) continue;
+                bool ready = k == first;
+                if (!ready && k < firstAlone && !(f & ZF_ALONE)) {
+                    ready = true;
+                    const int rx = c / z.zh, ry = c - rx * z.zh;
+                    for (int dx = -2; dx <= 2 && ready; ++dx)
+                        for (int dy = -2; dy <= 2; ++dy) {
+                            int qx = rx + dx, qy = ry + dy;
+                            if (wx) qx = wrapi(qx, z.zw);
+                            if (wy) qy = wrapi(qy, z.zh);
+                            if (qx < 0 || qx >= z.zw || qy < 0 || qy >= z.zh) continue;
+                            if ((int)z.lix[qx * z.zh + qy] < k) {
+                                ready = false;
+                                break;
+                            }
+                        }
+                }
+                if (ready) z.fl[c] = (unsigned short)(f | ZF_READY);
+            }
+            __syncthreads();
+            /* 2. the ready cells, the t-th on wavefront t mod 4; ZF_READY stays
+             * as it is until every wavefront has dealt the round */
+            int t = 0;
+            for (int b0 = 0; b0 < m; b0 += 64) {
+                const int k = b0 + lane;
+                unsigned long long msk = wave_ballot(k < m && (z.fl[(int)z.dl[k]] & ZF_READY));
+                while (msk) {
+                    const int bit = (int)__builtin_ctzll(msk);
+                    msk &= msk - 1ull;
+                    if ((t++ % ZONE_WAVES) != wave) continue;
+                    const int c = (int)z.dl[b0 + bit];
+                    const int nc = cnt_of(z, c);
+                    if (nc > 0) {
+                        ++nLowW;
+                        movedW += sparse_cell(z, c, nc, lowInt, lane, 64);
+                    }
+                    ZWSYNC();
+                    if (lane == 0) {
+                        z.fl[c] = (unsigned short)(z.fl[c] | ZF_DONE);
+                        z.lix[c] = none;
+                    }
+                    ZWSYNC();
+                }
+            }
+            __syncthreads();
+            /* 3. clear the round's marks; every wavefront scans the flags the
+             * same way */
+            for (int k = tid; k < m; k += 64 * ZONE_WAVES) {
+                const int c = (int)z.dl[k];
+                z.fl[c] = (unsigned short)(z.fl[c] & ~ZF_READY);
+            }
+            scan(done, first, firstAlone);
+            __syncthreads();
+        }
     }
     if (lane == 0) {
         atomicAdd(&z.sh->nLow, nLowW);
