@@ -216,8 +216,8 @@ struct ZoneIn {
 /* Scalars the lanes share (LDS on the device). */
 struct ZoneShared {
     int nOrd, startBin, nr, stage, P, allOwned;
-    int mwCount, mwLevels; /* the multi-wavefront dense phase's list length and level count */
-    int mwNext;            /* the dependency-driven dense phase's next list position */
+    int mwCount;           /* the multi-wavefront phases' list length */
+    int mwNext;            /* and the next list position a wavefront takes */
     int schedWait;         /* nonzero: a wait of the dependency-driven dense phase hit its bound */
     int nHigh, nLow, movedDense, movedSparse;
     double stats4, stats5;
@@ -1339,259 +1339,44 @@ ZHD void zone_dense_passes(Z& z, int lane, int nl) {
  * of it and writes the lists, counts and flags of itself and of targets
  * within range 2; its own count changes only by itself (a dense cell is above
  * its high threshold, a target below it).  Cells of the shuffled list more
- * than 4 cells apart therefore touch disjoint cells and commute.  Each cell
- * gets a level, one more than the highest level of the earlier cells within
- * range 4; the cells of one level run on the four wavefronts at once, level
- * after level, so every conflicting pair keeps the list order and the
- * result is the serial loop's bit for bit.  A cell whose movers exceed a
- * wavefront takes the one-lane form, which uses the zone's shared scratch:
- * it gets a level of its own, above every earlier cell and below every later
- * one, and wavefront 0 runs it.  Waves 1-3 stage their movers in the one-lane
- * form's candidate scratch, free while they run. */
+ * than 4 cells apart therefore touch disjoint cells and commute.  A cell
+ * whose movers exceed a wavefront takes the one-lane form, which uses the
+ * zone's shared scratch.  Waves 1-3 stage their movers in the one-lane form's
+ * candidate scratch, free while they run. */
 constexpr int ZONE_WAVES = 4;
 static_assert(24 * sizeof(Cand) >= (ZONE_WAVES - 1) * ZONE_MAX_MOVE * sizeof(int),
               "the candidate scratch holds the other wavefronts' movers");
 
-template <class Z>
-ZHD bool dense_conflict(const Z& z, int a, int b) {
-    int dx = a / z.zh - b / z.zh, dy = a % z.zh - b % z.zh;
-    dx = dx < 0 ? -dx : dx;
-    dy = dy < 0 ? -dy : dy;
-    /* a zone spanning a periodic axis reaches around it */
-    if (z.g.perx && z.zw == z.g.Nx) dx = imin(dx, z.zw - dx);
-    if (z.g.pery && z.zh == z.g.Ny) dy = imin(dy, z.zh - dy);
-    return dx <= 4 && dy <= 4;
-}
-
-/* The levels of the dense list z.dl[0..m) into z.ext[0..m) (0 for a cell
- * that is skipped) and their count into z.sh->mwLevels, on all threads of
- * the workgroup.  A cell's level is one more than the highest level of the
- * earlier cells it conflicts with: those within Chebyshev range 4, and every
- * earlier and later cell when either takes the one-lane form.  That is the
- * longest path to the cell in the order's conflict graph, which the serial
- * rule (one more than the earlier conflicting cells', at least one more than
- * the last one-lane cell, a one-lane cell above every earlier cell) gives
- * as well.  Rounds of max-updates raise every level towards that fixed
- * point and stop when a round changes none; a level depends only on earlier
- * cells, so this takes at most m + 1 rounds.  Each thread holds one cell.
- * Working word of a cell before the rounds: 1 | ONE | LIVE. */
-constexpr unsigned DL_LIVE = 0x8000u, DL_ONE = 0x4000u, DL_DONE = 0x2000u, DL_LEVEL = 0x3fffu;
-
-/* The rounds, out of line: the zone kernel holds its per-zone context in
- * registers, and inlined these loops would spill to scratch.
- * Conflicts are within Chebyshev range 4, so each live cell finds its
- * earlier conflicting cells in its 9 x 9 window through the list position of
- * every zone cell (pos, after the m working words in ext), once, as a mask of
- * list positions; each round then reads only those cells' levels.  Returns
- * 0, having changed nothing the serial rule reads, when the list holds a
- * one-lane cell, is longer than the workgroup or leaves ext no room for pos. */
-template <class IT>
-__device__ __attribute__((noinline)) int dense_level_rounds(const IT* __restrict__ dl, IT* __restrict__ ext, int nExt,
-                                                           int m, int C, int zw, int zh, int wrapX, int wrapY,
-                                                           int* levels, int tid) {
-    constexpr int NT = 64 * ZONE_WAVES;
-    constexpr unsigned NONE = (unsigned)(IT)~(IT)0;
-    bool one = false;
-    unsigned wk = 0u;
-    if (tid < m) {
-        wk = (unsigned)ext[tid];
-        one = (wk & DL_LIVE) && (wk & DL_ONE);
-    }
-    if (__syncthreads_or(one) || m > NT || m + C > nExt) return 0;
-    IT* pos = ext + m;
-    for (int lc = tid; lc < C; lc += NT) pos[lc] = (IT)NONE;
-    __syncthreads();
-    const bool live = (wk & DL_LIVE) != 0u;
-    int cx = 0, cy = 0;
-    if (live) {
-        const int c = (int)dl[tid];
-        cx = c / zh;
-        cy = c - cx * zh;
-        pos[c] = (IT)tid;
-    }
-    __syncthreads();
-    unsigned b0 = 0u, b1 = 0u, b2 = 0u, b3 = 0u, b4 = 0u, b5 = 0u, b6 = 0u, b7 = 0u;
-    if (live)
-        for (int dx = -4; dx <= 4; ++dx) {
-            int x = cx + dx;
-            if (wrapX) x = x < 0 ? x + zw : (x >= zw ? x - zw : x);
-            else if (x < 0 || x >= zw) continue;
-            for (int dy = -4; dy <= 4; ++dy) {
-                int y = cy + dy;
-                if (wrapY) y = y < 0 ? y + zh : (y >= zh ? y - zh : y);
-                else if (y < 0 || y >= zh) continue;
-                const unsigned q = (unsigned)pos[x * zh + y];
-                if (q < (unsigned)tid) {
-                    const unsigned bit = 1u << (q & 31u), w = q >> 5;
-                    b0 |= w == 0u ? bit : 0u;
-                    b1 |= w == 1u ? bit : 0u;
-                    b2 |= w == 2u ? bit : 0u;
-                    b3 |= w == 3u ? bit : 0u;
-                    b4 |= w == 4u ? bit : 0u;
-                    b5 |= w == 5u ? bit : 0u;
-                    b6 |= w == 6u ? bit : 0u;
-                    b7 |= w == 7u ? bit : 0u;
-                }
-            }
-        }
-    int lv = live ? 1 : 0;
-    if (tid < m) ext[tid] = (IT)lv;
-    __syncthreads();
-    for (;;) {
-        bool changed = false;
-        if (live) {
-            int best = 0;
-            const unsigned bw[8] = {b0, b1, b2, b3, b4, b5, b6, b7};
-#pragma unroll
-            for (int w = 0; w < 8; ++w)
-                for (unsigned mm = bw[w]; mm; mm &= mm - 1u) best = imax(best, (int)ext[32 * w + __builtin_ctz(mm)]);
-            if (best + 1 != lv) {
-                lv = best + 1;
-                ext[tid] = (IT)lv;
-                changed = true;
-            }
-        }
-        if (!__syncthreads_or(changed)) break;
-    }
-    int top = lv;
-    for (int o = 32; o > 0; o >>= 1) top = imax(top, __shfl_xor(top, o, 64));
-    if ((tid & 63) == 0) atomicMax(levels, top);
-    return 1;
-}
-
-/* Levels of the dense list into z.ext[0..m) and their count into
- * z.sh->mwLevels on all threads, or false (the same on every thread) when
- * the serial rule must assign them. */
-template <class Z>
-__device__ __forceinline__ bool dense_levels(Z& z, int tid) {
-    typedef typename Z::idx_t IT;
-    const int m = z.sh->mwCount;
-    for (int k = tid; k < m; k += 64 * ZONE_WAVES) {
-        const int c = (int)z.dl[k];
-        const int nc = cnt_of(z, c);
-        const double hl = high_thr(z, c);
-        unsigned w = 0u;
-        if (nc > 0 && (double)nc > hl) {
-            const int needOut = imax(0, (int)MPCD_CEIL((double)nc - hl));
-            const bool one = !(needOut > 0 && imin(needOut + 1, nc) <= ZONE_MAX_MOVE);
-            w = DL_LIVE | (one ? DL_ONE : 0u) | 1u;
-        }
-        z.ext[k] = (IT)w;
-    }
-    __syncthreads();
-    return dense_level_rounds<IT>(z.dl, z.ext, (z.zw + 4) * (z.zh + 4), m, z.C, z.zw, z.zh,
-                                  (int)(z.g.perx && z.zw == z.g.Nx), (int)(z.g.pery && z.zh == z.g.Ny),
-                                  &z.sh->mwLevels, tid) != 0;
-}
-
-/* The dense list without levels: the wavefronts take its cells in list
- * order from a shared counter, and a cell starts once every earlier live
- * cell within Chebyshev range 4 (found through pos, the list position of
- * each zone cell, after the m working words in ext) is done.  Conflicting
- * cells thus keep the list order, as with levels, and a wavefront waits only
- * for the cells its own depends on.  Cells are taken in increasing order and
- * wait only on smaller positions, which are taken already, so the smallest
- * unfinished cell can always run.  Each wait is bounded all the same: past
- * DS_WAIT_MAX polls it records the fact (schedWait, reported as an error)
- * and goes on.  Returns false, having run nothing, for a list with a
- * one-lane cell or one that leaves ext no room for pos; the caller then
- * uses the levels. */
+/* The dense list on the four wavefronts in list order.  Each wavefront takes
+ * the next cell from a counter in LDS; a cell starts once every earlier live
+ * cell within Chebyshev range 4 is done.  Those are found through pos (the
+ * list position of each zone cell, after the m working words in ext), and a
+ * finished cell sets DL_DONE in its word.  Conflicting cells thus keep the
+ * list order and the result is the serial loop's bit for bit, while a
+ * wavefront waits only for the cells its own depends on.  Cells are taken in
+ * increasing order and wait only on smaller positions, which are taken
+ * already, so the smallest unfinished cell can always run; each wait is
+ * bounded all the same (past DS_WAIT_MAX polls it records schedWait, which
+ * the kernel reports as an error, and goes on).  A list with a cell that
+ * needs the one-lane form (its movers exceed a wavefront), or one that leaves
+ * ext no room for pos, runs on wavefront 0 alone, in list order. */
+constexpr unsigned DL_LIVE = 0x8000u, DL_ONE = 0x4000u, DL_DONE = 0x2000u;
 constexpr int DS_WAIT_MAX = 1 << 22;
-
-template <class Z>
-__device__ __forceinline__ bool dense_dynamic(Z& z, int tid, int wave, int lane, int pass, int& nHighW,
-                                              int& movedW) {
-    typedef typename Z::idx_t IT;
-    constexpr int NT = 64 * ZONE_WAVES;
-    constexpr unsigned NONE = (unsigned)(IT)~(IT)0;
-    const int m = z.sh->mwCount;
-    const int nExt = (z.zw + 4) * (z.zh + 4);
-    bool one = false;
-    for (int k = tid; k < m; k += NT) {
-        const int c = (int)z.dl[k];
-        const int nc = cnt_of(z, c);
-        const double hl = high_thr(z, c);
-        unsigned w = 0u;
-        if (nc > 0 && (double)nc > hl) {
-            const int needOut = imax(0, (int)MPCD_CEIL((double)nc - hl));
-            one = one || !(needOut > 0 && imin(needOut + 1, nc) <= ZONE_MAX_MOVE);
-            w = DL_LIVE;
-        }
-        z.ext[k] = (IT)w;
-    }
-    if (__syncthreads_or(one) || m + z.C > nExt) return false;
-    IT* pos = z.ext + m;
-    for (int lc = tid; lc < z.C; lc += NT) pos[lc] = (IT)NONE;
-    __syncthreads();
-    for (int k = tid; k < m; k += NT)
-        if ((unsigned)z.ext[k] & DL_LIVE) pos[(int)z.dl[k]] = (IT)k;
-    if (tid == 0) z.sh->mwNext = 0;
-    __syncthreads();
-    const bool wrapX = z.g.perx && z.zw == z.g.Nx, wrapY = z.g.pery && z.zh == z.g.Ny;
-    volatile IT* vext = z.ext;
-    for (;;) {
-        int k = 0;
-        if (lane == 0) k = atomicAdd(&z.sh->mwNext, 1);
-        k = __shfl(k, 0, 64);
-        if (k >= m) break;
-        if (!((unsigned)vext[k] & DL_LIVE)) continue;
-        const int c = (int)z.dl[k];
-        /* the window's earlier live cells: lane t checks offsets t and t + 64 of the 81 */
-        int q0 = -1, q1 = -1;
-        {
-            const int cx = c / z.zh, cy = c - cx * z.zh;
-            for (int h = 0; h < 2; ++h) {
-                const int t = lane + 64 * h;
-                int qq = -1;
-                if (t < 81) {
-                    int x = cx + t / 9 - 4, y = cy + t % 9 - 4;
-                    bool in = true;
-                    if (wrapX) x = x < 0 ? x + z.zw : (x >= z.zw ? x - z.zw : x);
-                    else in = in && x >= 0 && x < z.zw;
-                    if (wrapY) y = y < 0 ? y + z.zh : (y >= z.zh ? y - z.zh : y);
-                    else in = in && y >= 0 && y < z.zh;
-                    if (in) {
-                        const unsigned pq = (unsigned)pos[x * z.zh + y];
-                        if (pq < (unsigned)k) qq = (int)pq;
-                    }
-                }
-                if (h == 0) q0 = qq;
-                else q1 = qq;
-            }
-        }
-        for (int polls = 0;; ++polls) {
-            const bool wait0 = q0 >= 0 && !((unsigned)vext[q0] & DL_DONE);
-            const bool wait1 = q1 >= 0 && !((unsigned)vext[q1] & DL_DONE);
-            if (wave_ballot(wait0 || wait1) == 0ull) break;
-            if (polls >= DS_WAIT_MAX) {
-                if (lane == 0) z.sh->schedWait = 1;
-                break;
-            }
-            __builtin_amdgcn_s_sleep(1);
-        }
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-        const int nc = cnt_of(z, c);
-        const double hl = high_thr(z, c);
-        const int needOut = imax(0, (int)MPCD_CEIL((double)nc - hl));
-        ++nHighW;
-        movedW += dense_cell_wave(z, c, nc, hl, needOut, pass, lane);
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-        if (lane == 0) vext[k] = (IT)(DL_LIVE | DL_DONE);
-        ZWSYNC();
-    }
-    __syncthreads();
-    return true;
-}
 
 template <class Z>
 __device__ __forceinline__ void zone_dense_passes_mw(Z& z, int wave, int lane) {
     typedef typename Z::idx_t IT;
+    constexpr int NT = 64 * ZONE_WAVES;
+    constexpr unsigned NONE = (unsigned)(IT)~(IT)0;
     const int tid = wave * 64 + lane;
     int nHighW = 0, movedW = 0;
     if (wave > 0) z.mvk = reinterpret_cast<int*>(z.cand) + (wave - 1) * ZONE_MAX_MOVE;
     if (tid == 0) z.sh->schedWait = 0;
+    const int nExt = (z.zw + 4) * (z.zh + 4);
+    const bool wrapX = z.g.perx && z.zw == z.g.Nx, wrapY = z.g.pery && z.zh == z.g.Ny;
+    volatile IT* vext = z.ext;
     for (int pass = 1; pass <= z.zp.maxPasses; ++pass) {
-        for (int lc = tid; lc < z.C; lc += 64 * ZONE_WAVES) zone_maps(z, lc, 0);
+        for (int lc = tid; lc < z.C; lc += NT) zone_maps(z, lc, 0);
         __syncthreads();
         if (wave == 0) {
             const int m = zone_compact(z, lane, 64, ZC_DENSE);
@@ -1599,67 +1384,86 @@ __device__ __forceinline__ void zone_dense_passes_mw(Z& z, int wave, int lane) {
             zone_shuffle(z.dl, m, lane, 64, ShufDraw{z.seed, ZS_DENSE, (uint32_t)pass, 0u});
             if (lane == 0) {
                 z.sh->mwCount = m;
-                z.sh->mwLevels = 0;
+                z.sh->mwNext = 0;
             }
         }
         __syncthreads();
-        if (dense_dynamic(z, tid, wave, lane, pass, nHighW, movedW)) continue;
-        /* levels in z.ext[0..m) (free until the sparse phase); 0 = skipped */
-        if (!dense_levels(z, tid) && wave == 0) {
-            const int m = z.sh->mwCount;
-            int floorLev = 0, maxLev = 0;
-            for (int k = 0; k < m; ++k) {
+        const int m = z.sh->mwCount;
+        bool one = false;
+        for (int k = tid; k < m; k += NT) {
+            const int c = (int)z.dl[k];
+            const int nc = cnt_of(z, c);
+            const double hl = high_thr(z, c);
+            unsigned w = 0u;
+            if (nc > 0 && (double)nc > hl) {
+                const int needOut = imax(0, (int)MPCD_CEIL((double)nc - hl));
+                const bool o = !(needOut > 0 && imin(needOut + 1, nc) <= ZONE_MAX_MOVE);
+                one = one || o;
+                w = DL_LIVE | (o ? DL_ONE : 0u);
+            }
+            z.ext[k] = (IT)w;
+        }
+        const bool serial = __syncthreads_or(one) || m + z.C > nExt;
+        IT* pos = z.ext + m;
+        if (!serial) {
+            for (int lc = tid; lc < z.C; lc += NT) pos[lc] = (IT)NONE;
+            __syncthreads();
+            for (int k = tid; k < m; k += NT)
+                if ((unsigned)z.ext[k] & DL_LIVE) pos[(int)z.dl[k]] = (IT)k;
+            __syncthreads();
+        }
+        if (!serial || wave == 0)
+            for (;;) {
+                int k = 0;
+                if (lane == 0) k = atomicAdd(&z.sh->mwNext, 1);
+                k = __shfl(k, 0, 64);
+                if (k >= m) break;
+                const unsigned wk = (unsigned)vext[k];
+                if (!(wk & DL_LIVE)) continue;
                 const int c = (int)z.dl[k];
+                if (!serial) {
+                    /* the window's earlier live cells: lane t checks offsets t and t + 64 of the 81 */
+                    const int cx = c / z.zh, cy = c - cx * z.zh;
+                    int q0 = -1, q1 = -1;
+                    for (int h = 0; h < 2; ++h) {
+                        const int t = lane + 64 * h;
+                        int qq = -1;
+                        if (t < 81) {
+                            int x = cx + t / 9 - 4, y = cy + t % 9 - 4;
+                            bool in = true;
+                            if (wrapX) x = x < 0 ? x + z.zw : (x >= z.zw ? x - z.zw : x);
+                            else in = in && x >= 0 && x < z.zw;
+                            if (wrapY) y = y < 0 ? y + z.zh : (y >= z.zh ? y - z.zh : y);
+                            else in = in && y >= 0 && y < z.zh;
+                            if (in) {
+                                const unsigned pq = (unsigned)pos[x * z.zh + y];
+                                if (pq < (unsigned)k) qq = (int)pq;
+                            }
+                        }
+                        if (h == 0) q0 = qq;
+                        else q1 = qq;
+                    }
+                    for (int polls = 0;; ++polls) {
+                        const bool wait0 = q0 >= 0 && !((unsigned)vext[q0] & DL_DONE);
+                        const bool wait1 = q1 >= 0 && !((unsigned)vext[q1] & DL_DONE);
+                        if (wave_ballot(wait0 || wait1) == 0ull) break;
+                        if (polls >= DS_WAIT_MAX) {
+                            if (lane == 0) z.sh->schedWait = 1;
+                            break;
+                        }
+                        __builtin_amdgcn_s_sleep(1);
+                    }
+                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+                }
                 const int nc = cnt_of(z, c);
                 const double hl = high_thr(z, c);
-                int lv = 0;
-                if (nc > 0 && (double)nc > hl) {
-                    const int needOut = imax(0, (int)MPCD_CEIL((double)nc - hl));
-                    const bool one = !(needOut > 0 && imin(needOut + 1, nc) <= ZONE_MAX_MOVE);
-                    int best = 0;
-                    if (one) {
-                        best = maxLev;
-                    } else {
-                        for (int j = lane; j < k; j += 64) {
-                            const int lj = iget(z.ext[j]);
-                            if (lj > best && dense_conflict(z, c, (int)z.dl[j])) best = lj;
-                        }
-                        for (int o = 32; o > 0; o >>= 1) best = imax(best, __shfl_xor(best, o, 64));
-                    }
-                    lv = imax(best, floorLev) + 1;
-                    if (one) floorLev = lv;
-                    maxLev = imax(maxLev, lv);
-                }
-                ZWSYNC();
-                if (lane == 0) z.ext[k] = (IT)lv;
-                ZWSYNC();
-            }
-            if (lane == 0) {
-                z.sh->mwCount = m;
-                z.sh->mwLevels = maxLev;
-            }
-        }
-        __syncthreads();
-        const int m = z.sh->mwCount, nLev = z.sh->mwLevels;
-        for (int l = 1; l <= nLev; ++l) {
-            int t = 0; /* cells of this level met so far, dealt round robin to the wavefronts */
-            for (int b0 = 0; b0 < m; b0 += 64) {
-                const int k = b0 + lane;
-                unsigned long long msk = wave_ballot(k < m && iget(z.ext[k]) == l);
-                while (msk) {
-                    const int bit = (int)__builtin_ctzll(msk);
-                    msk &= msk - 1ull;
-                    if ((t++ % ZONE_WAVES) != wave) continue;
-                    const int c = (int)z.dl[b0 + bit];
-                    const int nc = cnt_of(z, c);
-                    const double hl = high_thr(z, c);
-                    const int needOut = imax(0, (int)MPCD_CEIL((double)nc - hl));
-                    if (imin(needOut + 1, nc) <= ZONE_MAX_MOVE) {
-                        ++nHighW;
-                        movedW += dense_cell_wave(z, c, nc, hl, needOut, pass, lane);
-                        continue;
-                    }
-                    /* the one-lane form (alone in its level, on wavefront 0) */
+                const int needOut = imax(0, (int)MPCD_CEIL((double)nc - hl));
+                if (!(wk & DL_ONE)) {
+                    ++nHighW;
+                    movedW += dense_cell_wave(z, c, nc, hl, needOut, pass, lane);
+                } else {
+                    /* the one-lane form (serial lists only: wavefront 0, the
+                     * zone's shared scratch) */
                     int moved = 0;
                     if (lane == 0) {
                         ++z.sh->nHigh;
@@ -1689,9 +1493,13 @@ __device__ __forceinline__ void zone_dense_passes_mw(Z& z, int wave, int lane) {
                     }
                     ZBAR();
                 }
+                if (!serial) {
+                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+                    if (lane == 0) vext[k] = (IT)(wk | DL_DONE);
+                    ZWSYNC();
+                }
             }
-            __syncthreads();
-        }
+        __syncthreads();
     }
     if (lane == 0) {
         atomicAdd(&z.sh->nHigh, nHighW);
@@ -1902,18 +1710,16 @@ ZHD void zone_sparse_pass(Z& z, int lane, int nl) {
  * transfers read and change only itself and its zone neighbours at range 1
  * (its own count changes only by itself: a low cell is never a donor), and
  * the smoothed counts they rank by are the phase's starting counts, so low
- * cells more than 2 apart commute.  Rounds: every unfinished cell with no
- * unfinished earlier cell within range 2 (found through lix, each cell's list
- * index, cleared when the cell is finished) runs in this round, the k-th such
- * cell on wavefront k mod 4.  A cell one of whose neighbours starts with more
- * particles than a wavefront's share of the transfer scratch (ZF_ALONE)
- * conflicts with every cell and runs alone when it is the first unfinished,
- * on wavefront 0 (which has the full scratch and the global region).
+ * cells more than 2 apart commute.  The wavefronts take the cells in list
+ * order from a counter in LDS, and a cell starts once no earlier unfinished
+ * cell lies within range 2 of it (lix, each cell's list index, is cleared
+ * when the cell is finished); the waits are bounded as in the dense phase.
  * Conflicting cells keep the list order, so the result is the serial loop's
- * bit for bit.  Each round marks its cells (ZF_READY) before any runs, and
- * finished cells carry ZF_DONE, so every wavefront deals the same cells. */
+ * bit for bit.  A list with a cell one of whose neighbours starts with more
+ * particles than a wavefront's share of the transfer scratch runs on
+ * wavefront 0 alone, in list order (it has the full scratch and the global
+ * region). */
 constexpr int ZONE_TMP_W = 112; /* transfer scratch of each of wavefronts 1..3 */
-enum : unsigned short { ZF_DONE = 512, ZF_READY = 1024, ZF_ALONE = 2048 };
 
 /* a low cell with a zone neighbour at range 1 that starts the phase with
  * more particles than a wavefront's transfer scratch holds (donor lists only
@@ -1963,10 +1769,7 @@ __device__ __forceinline__ void zone_sparse_pass_mw(Z& z, int wave, int lane) {
     }
     const IT none = (IT)~(IT)0;
     sparse_setup(z, tid, 64 * ZONE_WAVES);
-    for (int lc = tid; lc < z.C; lc += 64 * ZONE_WAVES) {
-        z.fl[lc] = (unsigned short)(z.fl[lc] & ~(ZF_DONE | ZF_READY | ZF_ALONE));
-        z.lix[lc] = none;
-    }
+    for (int lc = tid; lc < z.C; lc += 64 * ZONE_WAVES) z.lix[lc] = none;
     __syncthreads();
     if (wave == 0) {
         const int m = zone_compact(z, lane, 64, ZC_LOW);
@@ -1980,145 +1783,59 @@ __device__ __forceinline__ void zone_sparse_pass_mw(Z& z, int wave, int lane) {
     for (int k = tid; k < m; k += 64 * ZONE_WAVES) {
         const int c = (int)z.dl[k];
         z.lix[c] = (IT)k;
-        if (sparse_alone(z, c)) {
-            z.fl[c] = (unsigned short)(z.fl[c] | ZF_ALONE);
-            anyAlone = true;
-        }
+        anyAlone = anyAlone || sparse_alone(z, c);
     }
     anyAlone = __syncthreads_or(anyAlone);
     const int lowInt = (int)MPCD_CEIL(z.zp.lowThrBulk);
     const bool wx = z.g.perx && z.zw == z.g.Nx, wy = z.g.pery && z.zh == z.g.Ny;
     int nLowW = 0, movedW = 0;
-    if (!anyAlone) {
-        /* No cell runs alone: the wavefronts take the cells in list order and
-         * each waits only until no earlier unfinished cell lies within range
-         * 2 of its own (lix of a finished cell is cleared), the waits bounded
-         * as in the dense phase's schedule. */
-        if (tid == 0) z.sh->mwNext = 0;
-        __syncthreads();
-        volatile IT* vlix = z.lix;
+    const bool serial = anyAlone;
+    if (tid == 0) z.sh->mwNext = 0;
+    __syncthreads();
+    volatile IT* vlix = z.lix;
+    if (!serial || wave == 0)
         for (;;) {
             int k = 0;
             if (lane == 0) k = atomicAdd(&z.sh->mwNext, 1);
             k = __shfl(k, 0, 64);
             if (k >= m) break;
             const int c = (int)z.dl[k];
-            int q = -1;
-            if (lane < 25) {
-                const int rx = c / z.zh, ry = c - rx * z.zh;
-                int qx = rx + lane / 5 - 2, qy = ry + lane % 5 - 2;
-                if (wx) qx = wrapi(qx, z.zw);
-                if (wy) qy = wrapi(qy, z.zh);
-                if (qx >= 0 && qx < z.zw && qy >= 0 && qy < z.zh) q = qx * z.zh + qy;
-            }
-            for (int polls = 0;; ++polls) {
-                const bool wait = q >= 0 && (int)vlix[q] < k;
-                if (wave_ballot(wait) == 0ull) break;
-                if (polls >= DS_WAIT_MAX) {
-                    if (lane == 0) z.sh->schedWait = 1;
-                    break;
+            if (!serial) {
+                int q = -1;
+                if (lane < 25) {
+                    const int rx = c / z.zh, ry = c - rx * z.zh;
+                    int qx = rx + lane / 5 - 2, qy = ry + lane % 5 - 2;
+                    if (wx) qx = wrapi(qx, z.zw);
+                    if (wy) qy = wrapi(qy, z.zh);
+                    if (qx >= 0 && qx < z.zw && qy >= 0 && qy < z.zh) q = qx * z.zh + qy;
                 }
-                __builtin_amdgcn_s_sleep(1);
+                for (int polls = 0;; ++polls) {
+                    const bool wait = q >= 0 && (int)vlix[q] < k;
+                    if (wave_ballot(wait) == 0ull) break;
+                    if (polls >= DS_WAIT_MAX) {
+                        if (lane == 0) z.sh->schedWait = 1;
+                        break;
+                    }
+                    __builtin_amdgcn_s_sleep(1);
+                }
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
             }
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
             const int nc = cnt_of(z, c);
             if (nc > 0) {
                 ++nLowW;
                 movedW += sparse_cell(z, c, nc, lowInt, lane, 64);
             }
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-            if (lane == 0) vlix[c] = none;
-            ZWSYNC();
+            if (!serial) {
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+                if (lane == 0) vlix[c] = none;
+                ZWSYNC();
+            }
         }
-        __syncthreads();
-    } else {
-        /* the first unfinished cell and the first unfinished one that runs alone */
-        auto scan = [&](int& done, int& first, int& firstAlone) {
-            done = 0;
-            first = m;
-            firstAlone = m;
-            for (int b0 = 0; b0 < m; b0 += 64) {
-                const int k = b0 + lane;
-                const unsigned short f = k < m ? z.fl[(int)z.dl[k]] : (unsigned short)ZF_DONE;
-                const unsigned long long dm = wave_ballot(f & ZF_DONE);
-                done += wave_popc(dm & (b0 + 64 <= m ? ~0ull : lanes_below(m - b0)));
-                const unsigned long long um = ~dm;
-                if (first == m && um) first = b0 + (int)__builtin_ctzll(um);
-                const unsigned long long am = wave_ballot(!(f & ZF_DONE) && (f & ZF_ALONE));
-                if (firstAlone == m && am) firstAlone = b0 + (int)__builtin_ctzll(am);
-            }
-        };
-        int done, first, firstAlone;
-        scan(done, first, firstAlone);
-        for (int round = 0; round < m && done < m; ++round) {
-            /* 1. the round's ready cells, marked ZF_READY: the first unfinished
-             * cell, and every cell before the first unfinished lone one that
-             * runs with others and has no unfinished earlier cell within range 2
-             * (lix of finished cells is cleared) */
-            for (int k = tid; k < m; k += 64 * ZONE_WAVES) {
-                const int c = (int)z.dl[k];
-                const unsigned short f = z.fl[c];
-                if (f & ZF_DONE) continue;
-                bool ready = k == first;
-                if (!ready && k < firstAlone && !(f & ZF_ALONE)) {
-                    ready = true;
-                    const int rx = c / z.zh, ry = c - rx * z.zh;
-                    for (int dx = -2; dx <= 2 && ready; ++dx)
-                        for (int dy = -2; dy <= 2; ++dy) {
-                            int qx = rx + dx, qy = ry + dy;
-                            if (wx) qx = wrapi(qx, z.zw);
-                            if (wy) qy = wrapi(qy, z.zh);
-                            if (qx < 0 || qx >= z.zw || qy < 0 || qy >= z.zh) continue;
-                            if ((int)z.lix[qx * z.zh + qy] < k) {
-                                ready = false;
-                                break;
-                            }
-                        }
-                }
-                if (ready) z.fl[c] = (unsigned short)(f | ZF_READY);
-            }
-            __syncthreads();
-            /* 2. the ready cells, the t-th on wavefront t mod 4; ZF_READY stays
-             * as it is until every wavefront has dealt the round */
-            int t = 0;
-            for (int b0 = 0; b0 < m; b0 += 64) {
-                const int k = b0 + lane;
-                unsigned long long msk = wave_ballot(k < m && (z.fl[(int)z.dl[k]] & ZF_READY));
-                while (msk) {
-                    const int bit = (int)__builtin_ctzll(msk);
-                    msk &= msk - 1ull;
-                    if ((t++ % ZONE_WAVES) != wave) continue;
-                    const int c = (int)z.dl[b0 + bit];
-                    const int nc = cnt_of(z, c);
-                    if (nc > 0) {
-                        ++nLowW;
-                        movedW += sparse_cell(z, c, nc, lowInt, lane, 64);
-                    }
-                    ZWSYNC();
-                    if (lane == 0) {
-                        z.fl[c] = (unsigned short)(z.fl[c] | ZF_DONE);
-                        z.lix[c] = none;
-                    }
-                    ZWSYNC();
-                }
-            }
-            __syncthreads();
-            /* 3. clear the round's marks; every wavefront scans the flags the
-             * same way */
-            for (int k = tid; k < m; k += 64 * ZONE_WAVES) {
-                const int c = (int)z.dl[k];
-                z.fl[c] = (unsigned short)(z.fl[c] & ~ZF_READY);
-            }
-            scan(done, first, firstAlone);
-            __syncthreads();
-        }
-    }
+    __syncthreads();
     if (lane == 0) {
         atomicAdd(&z.sh->nLow, nLowW);
         atomicAdd(&z.sh->movedSparse, movedW);
     }
-    for (int lc = tid; lc < z.C; lc += 64 * ZONE_WAVES)
-        z.fl[lc] = (unsigned short)(z.fl[lc] & ~(ZF_DONE | ZF_READY | ZF_ALONE));
     __syncthreads();
 }
 #else
