@@ -2376,7 +2376,9 @@ ZHD void fin_write(Z& z, int sl, unsigned short ow, bool corr, double vx, double
     }
 }
 
-/* phase 2 for cell lc; returns |du|^2 (0 for a cell nothing changed) */
+/* phase 2 for cell lc; returns |du|^2 (0 for a cell nothing changed).  Every
+ * entry's link is set to lc, its final cell, once the walk has read it: phase
+ * 3 reads the links of the moved particles, and nothing reads the others. */
 template <class Z>
 ZHD double fin_cell(Z& z, int lc, double pbx, double pby) {
     const int m = cnt_of(z, lc);
@@ -2431,7 +2433,7 @@ ZHD double fin_cell(Z& z, int lc, double pbx, double pby) {
         for (int u = 0; u < FIN_L; ++u)
             if (sl[u] >= 0) {
                 fin_write(z, sl[u], ow[u], corr, ax[u], ay[u], dux, duy);
-                if (z.rec[kk[u]] != ZM_NONE) nx_set(z, kk[u], lc);
+                nx_set(z, kk[u], lc);
             }
     } else {
         k = head_of(z, lc);
@@ -2456,7 +2458,7 @@ ZHD double fin_cell(Z& z, int lc, double pbx, double pby) {
             for (int u = 0; u < FIN_L; ++u)
                 if (sl[u] >= 0) {
                     fin_write(z, sl[u], ow[u], corr, ax[u], ay[u], dux, duy);
-                    if (z.rec[kk[u]] != ZM_NONE) nx_set(z, kk[u], lc);
+                    nx_set(z, kk[u], lc);
                 }
         }
     }
