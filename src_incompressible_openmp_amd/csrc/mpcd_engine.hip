@@ -1109,13 +1109,13 @@ __global__ void __launch_bounds__(ZONE_TPB) k_zone(ZoneArgs a) {
 }
 
 /* The momentum correction of the zones k_zone left (zone_pass.cpp:870-922),
- * FINISH_TPB threads per zone: one thread per cell for the sums and the
- * writes, then one wavefront for the ordered stats.  The same per-cell code
- * as the single-wavefront form, so the same results. */
+ * FINISH_TPB threads per zone: one thread per cell for the input-run sums,
+ * a group of FIN_G lanes per cell for the final lists' sums and writes (the
+ * sums in list order), then one wavefront for the ordered stats. */
 constexpr int FINISH_TPB = 256;
 constexpr int FINISH_LDS = 40 * 1024; /* cell arrays, column map and as many links as fit */
-/* The correction of zone zi by TPB threads (CPT cells per thread held in
- * registers); lds/ldsBytes: the staging space (kLds). */
+/* The correction of zone zi by TPB threads (zones of more than CPT x TPB
+ * cells take the two-sweep form); lds/ldsBytes: the staging space (kLds). */
 template <class IT, bool kLds, int TPB, int CPT>
 __device__ __forceinline__ void zone_finish_body(const ZoneArgs& a, const ZoneIn& in, int zid, uint64_t seed, int zi,
                                                  long long off, int tid, unsigned char* lds, long long ldsBytes,
@@ -1165,21 +1165,45 @@ __device__ __forceinline__ void zone_finish_body(const ZoneArgs& a, const ZoneIn
         z.nL = nL;
         __syncthreads();
     }
-    /* phase 1: input-run sums, kept by the thread of each cell */
-    double pbx[CPT], pby[CPT];
+    /* phase 1: input-run sums of the changed cells, into the zone's per-cell
+     * region (the base pass has them from the reference fields) */
     const bool regs = z.C <= CPT * TPB;
-    if (regs)
-        for (int j = 0; j < CPT; ++j) {
-            const int lc = tid + j * TPB;
-            pbx[j] = pby[j] = 0.0;
-            if (lc < z.C) fin_input_sum(z, lc, pbx[j], pby[j]);
+    if (regs && !z.inPx)
+        for (int lc = tid; lc < z.C; lc += TPB) {
+            double bx, by;
+            fin_input_sum(z, lc, bx, by);
+            z.duG[2 * lc] = bx;
+            z.duG[2 * lc + 1] = by;
         }
     __syncthreads();
-    /* phase 2: final lists */
+    /* phase 2: final lists, a group of FIN_G lanes per changed cell */
     if (regs) {
-        for (int j = 0; j < CPT; ++j) {
-            const int lc = tid + j * TPB;
-            if (lc < z.C) st.dsq[lc] = fin_cell(z, lc, pbx[j], pby[j]);
+        static_assert(TPB % FIN_G == 0, "whole groups");
+        const int r = tid % FIN_G, gi = tid / FIN_G;
+        for (int lc0 = 0; lc0 < z.C; lc0 += TPB / FIN_G) {
+            const int lc = lc0 + gi;
+            if (lc >= z.C) continue;
+            if (!fget(z, lc, ZF_CHG) || cnt_of(z, lc) == 0) {
+                if (r == 0) {
+                    /* nothing to correct: the diagnostics' record, the side-buffer copy of a covered cell */
+                    if (z.dgCell && cnt_of(z, lc) > 0 && !fget(z, lc, ZF_CHG) && !fget(z, lc, ZF_OUT))
+                        diag_cell_unchanged(z, lc);
+                    if (!fget(z, lc, ZF_CHG)) zone_correct_write(z, lc);
+                    st.dsq[lc] = 0.0;
+                }
+                continue;
+            }
+            double bx, by;
+            if (z.inPx) {
+                const int gc = lc_gc(z, lc);
+                bx = z.inPx[gc];
+                by = z.inPy[gc];
+            } else {
+                bx = z.duG[2 * lc];
+                by = z.duG[2 * lc + 1];
+            }
+            const double d = fin_cell_group(z, lc, r, tid & 63, bx, by);
+            if (r == 0) st.dsq[lc] = d;
         }
     } else {
         /* more cells than threads x CPT: the two-sweep form, positions included */

@@ -2376,34 +2376,57 @@ ZHD void fin_write(Z& z, int sl, unsigned short ow, bool corr, double vx, double
     }
 }
 
-/* phase 2 for cell lc; returns |du|^2 (0 for a cell nothing changed).  Every
- * entry's link is set to lc, its final cell, once the walk has read it: phase
- * 3 reads the links of the moved particles, and nothing reads the others. */
+constexpr int FIN_G = 4;
+
+#if defined(__HIP_DEVICE_COMPILE__)
+/* Phase 2 for changed, occupied cell lc on a group of FIN_G consecutive
+ * lanes (r: the lane's rank; every lane of the group passes the same cell).
+ * Every entry's link is set to lc, its final cell, once the walk has read
+ * it: phase 3 reads the links of the moved particles, and nothing reads the
+ * others.
+ * The group takes the final list FIN_G * FIN_L entries at a time, lane r
+ * entries [FIN_L r, FIN_L (r + 1)) of each chunk, so a list of up to
+ * FIN_G * FIN_L entries has each slot and velocity loaded once instead of
+ * once per pass.  The momentum sums keep the list order: lane r adds its
+ * entries to the running sum lane r - 1 hands on.  A linked list is walked by
+ * the whole group in step.  pbx, pby: the cell's input-run momentum.  Returns
+ * |du|^2 (the same on every lane of the group). */
 template <class Z>
-ZHD double fin_cell(Z& z, int lc, double pbx, double pby) {
-    const int m = cnt_of(z, lc);
-    if (!fget(z, lc, ZF_CHG)) {
-        if (z.dgCell && m > 0 && !fget(z, lc, ZF_OUT)) diag_cell_unchanged(z, lc);
-        zone_correct_write(z, lc); /* the side-buffer copy of a covered cell */
-        return 0.0;
+__device__ __forceinline__ void fin_group_entries(const Z& z, bool mat, int h, int t, int m, int c0, int r, int& kw,
+                                                  int* kk) {
+    for (int u = 0; u < FIN_L; ++u) kk[u] = -1;
+    if (!mat) {
+        for (int u = 0; u < FIN_L; ++u) {
+            const int e = c0 + FIN_L * r + u;
+            kk[u] = (e < m && h + e <= t) ? h + e : -1;
+        }
+        return;
     }
-    if (m == 0) return 0.0;
+    int k = kw;
+    for (int a = 0; a < FIN_G; ++a)
+        for (int u = 0; u < FIN_L; ++u) {
+            if (a == r) kk[u] = k;
+            if (k >= 0) k = nx_get(z, k);
+        }
+    kw = k;
+}
+
+template <class Z>
+__device__ __forceinline__ double fin_cell_group(Z& z, int lc, int r, int lane, double pbx, double pby) {
+    constexpr int CH = FIN_G * FIN_L;
+    const int m = cnt_of(z, lc);
     const int kb = (int)z.k0[lc], ke = kb + in_cnt(z, lc);
     const bool mat = fget(z, lc, ZF_MAT);
-    const int t = tail_of(z, lc);
+    const int t = tail_of(z, lc), h = head_of(z, lc);
     const unsigned short ownLc = (unsigned short)(z.fl[lc] & (ZF_OWN | ZF_COVER));
-    double px = 0.0, py = 0.0;
+    const int g0 = lane - r;
     int kk[FIN_L], sl[FIN_L];
     unsigned short ow[FIN_L];
     double ax[FIN_L], ay[FIN_L];
-    int k = head_of(z, lc);
-    bool single = m <= FIN_L;
-    /* sums in list order; a short list keeps its entries */
-    while (k >= 0) {
-        for (int u = 0; u < FIN_L; ++u) {
-            kk[u] = k;
-            if (k >= 0) k = mat ? nx_get(z, k) : (k < t ? k + 1 : -1);
-        }
+    double px = 0.0, py = 0.0;
+    int kw = h;
+    for (int c0 = 0; c0 < m; c0 += CH) {
+        fin_group_entries(z, mat, h, t, m, c0, r, kw, kk);
         for (int u = 0; u < FIN_L; ++u) {
             sl[u] = -1;
             ow[u] = 0;
@@ -2417,31 +2440,28 @@ ZHD double fin_cell(Z& z, int lc, double pbx, double pby) {
             ax[u] = sl[u] >= 0 ? z.vx[sl[u]] : 0.0;
             ay[u] = sl[u] >= 0 ? z.vy[sl[u]] : 0.0;
         }
-        for (int u = 0; u < FIN_L; ++u)
-            if (sl[u] >= 0) {
-                px += ax[u];
-                py += ay[u];
-            }
+        for (int a = 0; a < FIN_G; ++a) {
+            if (r == a)
+                for (int u = 0; u < FIN_L; ++u)
+                    if (sl[u] >= 0) {
+                        px += ax[u];
+                        py += ay[u];
+                    }
+            px = __shfl(px, g0 + a, 64);
+            py = __shfl(py, g0 + a, 64);
+        }
     }
-    if (z.dgCell) diag_cell(z, lc, pbx - px, pby - py, px, py);
+    if (z.dgCell && r == 0) diag_cell(z, lc, pbx - px, pby - py, px, py);
     const double dux = (pbx - px) / (double)m, duy = (pby - py) / (double)m;
     bool corr = z.zp.corrEnabled != 0;
     if (corr && fget(z, lc, ZF_SKIP)) corr = false;
     if (corr && fget(z, lc, ZF_RECV) && !fget(z, lc, ZF_BB)) corr = false;
     if (corr && !(dux != 0.0 || duy != 0.0)) corr = false;
-    if (single) {
-        for (int u = 0; u < FIN_L; ++u)
-            if (sl[u] >= 0) {
-                fin_write(z, sl[u], ow[u], corr, ax[u], ay[u], dux, duy);
-                nx_set(z, kk[u], lc);
-            }
-    } else {
-        k = head_of(z, lc);
-        while (k >= 0) {
-            for (int u = 0; u < FIN_L; ++u) {
-                kk[u] = k;
-                if (k >= 0) k = mat ? nx_get(z, k) : (k < t ? k + 1 : -1);
-            }
+    if (m > CH) {
+        /* a longer list: walk it again for the writes */
+        kw = h;
+        for (int c0 = 0; c0 < m; c0 += CH) {
+            fin_group_entries(z, mat, h, t, m, c0, r, kw, kk);
             for (int u = 0; u < FIN_L; ++u) {
                 sl[u] = -1;
                 ow[u] = 0;
@@ -2461,9 +2481,20 @@ ZHD double fin_cell(Z& z, int lc, double pbx, double pby) {
                     nx_set(z, kk[u], lc);
                 }
         }
+    } else {
+        for (int u = 0; u < FIN_L; ++u)
+            if (sl[u] >= 0) {
+                fin_write(z, sl[u], ow[u], corr, ax[u], ay[u], dux, duy);
+                nx_set(z, kk[u], lc);
+            }
     }
     return dux * dux + duy * duy;
 }
+#else
+/* (the host passes of the compilers see a declaration only) */
+template <class Z>
+ZHD double fin_cell_group(Z&, int, int, int, double, double) { return 0.0; }
+#endif
 
 /* phase 3, all threads of the zone: the new positions of the moved
  * particles (links hold their final cells).  Each wavefront scans its share
