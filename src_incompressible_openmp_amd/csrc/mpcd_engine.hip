@@ -1109,9 +1109,9 @@ __global__ void __launch_bounds__(ZONE_TPB) k_zone(ZoneArgs a) {
 }
 
 /* The momentum correction of the zones k_zone left (zone_pass.cpp:870-922),
- * FINISH_TPB threads per zone: one thread per cell for the input-run sums,
- * a group of FIN_G lanes per cell for the final lists' sums and writes (the
- * sums in list order), then one wavefront for the ordered stats. */
+ * FINISH_TPB threads per zone: a group of FIN_G lanes per cell for the
+ * input-run sums and for the final lists' sums and writes (the sums in list
+ * order), then one wavefront for the ordered stats. */
 constexpr int FINISH_TPB = 256;
 constexpr int FINISH_LDS = 40 * 1024; /* cell arrays, column map and as many links as fit */
 /* The correction of zone zi by TPB threads (zones of more than CPT x TPB
@@ -1165,21 +1165,25 @@ __device__ __forceinline__ void zone_finish_body(const ZoneArgs& a, const ZoneIn
         z.nL = nL;
         __syncthreads();
     }
-    /* phase 1: input-run sums of the changed cells, into the zone's per-cell
-     * region (the base pass has them from the reference fields) */
+    /* phase 1: input-run sums of the changed, occupied cells, into the zone's
+     * per-cell region (the base pass has them from the reference fields) */
+    static_assert(TPB % FIN_G == 0, "whole groups");
+    const int r = tid % FIN_G, gi = tid / FIN_G;
     const bool regs = z.C <= CPT * TPB;
     if (regs && !z.inPx)
-        for (int lc = tid; lc < z.C; lc += TPB) {
+        for (int lc0 = 0; lc0 < z.C; lc0 += TPB / FIN_G) {
+            const int lc = lc0 + gi;
+            if (lc >= z.C || !fget(z, lc, ZF_CHG) || cnt_of(z, lc) == 0) continue;
             double bx, by;
-            fin_input_sum(z, lc, bx, by);
-            z.duG[2 * lc] = bx;
-            z.duG[2 * lc + 1] = by;
+            fin_input_sum_group(z, lc, r, tid & 63, bx, by);
+            if (r == 0) {
+                z.duG[2 * lc] = bx;
+                z.duG[2 * lc + 1] = by;
+            }
         }
     __syncthreads();
     /* phase 2: final lists, a group of FIN_G lanes per changed cell */
     if (regs) {
-        static_assert(TPB % FIN_G == 0, "whole groups");
-        const int r = tid % FIN_G, gi = tid / FIN_G;
         for (int lc0 = 0; lc0 < z.C; lc0 += TPB / FIN_G) {
             const int lc = lc0 + gi;
             if (lc >= z.C) continue;

@@ -2315,49 +2315,22 @@ ZHD int in_cnt(const Z& z, int lc) {
     return end - (int)z.k0[lc];
 }
 
-/* ---- the correction kernel's per-cell form (one thread per cell) ----
+/* ---- the correction kernel's per-cell form (a group of lanes per cell) ----
  * Phase 1 sums every changed cell's input run; after a barrier, phase 2
- * walks each cell's final list once: a list of up to FIN_L entries stays in
- * registers from the sums to the writes (a particle is in exactly one final
- * list, and every input-run read happened in phase 1), a longer one is
- * walked twice.  The moved particles' links are set to their final cell for
- * phase 3, which places them.  Same sums in the same order as
+ * walks each cell's final list once: a list of up to FIN_G x FIN_L entries
+ * stays in registers from the sums to the writes (a particle is in exactly
+ * one final list, and every input-run read happened in phase 1), a longer
+ * one is walked twice.  The moved particles' links are set to their final
+ * cell for phase 3, which places them.  Same sums in the same order as
  * zone_correct_sums / zone_correct_write. */
 constexpr int FIN_L = 8;
+constexpr int FIN_G = 4;
 
 /* column of particle k of cell lc's final list: its own input run lies in
  * lc's column */
 template <class Z>
 ZHD int fin_col(const Z& z, int lc, int kb, int ke, int k) {
     return (k >= kb && k < ke) ? lc / z.zh : col_of(z, k);
-}
-
-template <class Z>
-ZHD void fin_input_sum(const Z& z, int lc, double& pbx, double& pby) {
-    pbx = 0.0;
-    pby = 0.0;
-    if (!fget(z, lc, ZF_CHG) || cnt_of(z, lc) == 0) return;
-    const int gc = lc_gc(z, lc);
-    if (z.inPx) {
-        pbx = z.inPx[gc];
-        pby = z.inPy[gc];
-        return;
-    }
-    const int s0 = z.gstart[gc], n = z.gcnt[gc];
-    for (int j0 = 0; j0 < n; j0 += FIN_L) {
-        int sl[FIN_L];
-        double ax[FIN_L], ay[FIN_L];
-        for (int u = 0; u < FIN_L; ++u) sl[u] = (j0 + u < n) ? z.glist[s0 + j0 + u] : -1;
-        for (int u = 0; u < FIN_L; ++u) {
-            ax[u] = sl[u] >= 0 ? z.vx[sl[u]] : 0.0;
-            ay[u] = sl[u] >= 0 ? z.vy[sl[u]] : 0.0;
-        }
-        for (int u = 0; u < FIN_L; ++u)
-            if (sl[u] >= 0) {
-                pbx += ax[u];
-                pby += ay[u];
-            }
-    }
 }
 
 template <class Z>
@@ -2376,8 +2349,6 @@ ZHD void fin_write(Z& z, int sl, unsigned short ow, bool corr, double vx, double
     }
 }
 
-constexpr int FIN_G = 4;
-
 #if defined(__HIP_DEVICE_COMPILE__)
 /* Phase 2 for changed, occupied cell lc on a group of FIN_G consecutive
  * lanes (r: the lane's rank; every lane of the group passes the same cell).
@@ -2391,6 +2362,43 @@ constexpr int FIN_G = 4;
  * entries to the running sum lane r - 1 hands on.  A linked list is walked by
  * the whole group in step.  pbx, pby: the cell's input-run momentum.  Returns
  * |du|^2 (the same on every lane of the group). */
+/* Phase 1 for changed, occupied cell lc on a group of FIN_G lanes: the
+ * momentum of its input run (the particles binned into it, before the
+ * passes), FIN_G * FIN_L entries at a time, summed in run order as
+ * fin_cell_group sums the final list.  Every lane of the group gets the sums. */
+template <class Z>
+__device__ __forceinline__ void fin_input_sum_group(const Z& z, int lc, int r, int lane, double& pbx, double& pby) {
+    constexpr int CH = FIN_G * FIN_L;
+    const int gc = lc_gc(z, lc);
+    const int s0 = z.gstart[gc], n = z.gcnt[gc];
+    const int g0 = lane - r;
+    double px = 0.0, py = 0.0;
+    for (int c0 = 0; c0 < n; c0 += CH) {
+        int sl[FIN_L];
+        double ax[FIN_L], ay[FIN_L];
+        for (int u = 0; u < FIN_L; ++u) {
+            const int e = c0 + FIN_L * r + u;
+            sl[u] = e < n ? z.glist[s0 + e] : -1;
+        }
+        for (int u = 0; u < FIN_L; ++u) {
+            ax[u] = sl[u] >= 0 ? z.vx[sl[u]] : 0.0;
+            ay[u] = sl[u] >= 0 ? z.vy[sl[u]] : 0.0;
+        }
+        for (int a = 0; a < FIN_G; ++a) {
+            if (r == a)
+                for (int u = 0; u < FIN_L; ++u)
+                    if (sl[u] >= 0) {
+                        px += ax[u];
+                        py += ay[u];
+                    }
+            px = __shfl(px, g0 + a, 64);
+            py = __shfl(py, g0 + a, 64);
+        }
+    }
+    pbx = px;
+    pby = py;
+}
+
 template <class Z>
 __device__ __forceinline__ void fin_group_entries(const Z& z, bool mat, int h, int t, int m, int c0, int r, int& kw,
                                                   int* kk) {
@@ -2492,6 +2500,8 @@ __device__ __forceinline__ double fin_cell_group(Z& z, int lc, int r, int lane, 
 }
 #else
 /* (the host passes of the compilers see a declaration only) */
+template <class Z>
+ZHD void fin_input_sum_group(const Z&, int, int, int, double&, double&) {}
 template <class Z>
 ZHD double fin_cell_group(Z&, int, int, int, double, double) { return 0.0; }
 #endif
