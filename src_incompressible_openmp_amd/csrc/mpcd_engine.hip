@@ -1217,12 +1217,16 @@ __device__ __forceinline__ void zone_finish_body(const ZoneArgs& a, const ZoneIn
     }
     __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
-    /* phase 3: moved particles' positions (the two-sweep form placed them) */
-    if (regs) fin_place(z, tid, TPB, queue + (tid & ~63));
-    if (tid < 64) {
+    /* phase 3: moved particles' positions (the two-sweep form placed them) on
+     * all wavefronts but the last, which meanwhile takes the ordered stats of
+     * the per-cell |du|^2 (phase 3 reads none of them) */
+    static_assert(TPB >= 128 && TPB % 64 == 0, "one wavefront for the stats, at least one for phase 3");
+    constexpr int TS = TPB - 64;
+    if (regs && tid < TS) fin_place(z, tid, TS, queue + (tid & ~63));
+    if (tid >= TS) {
         double rms, mx;
-        zone_correct_stats(z, st.dsq, tid, 64, rms, mx);
-        if (tid == 0) {
+        zone_correct_stats(z, st.dsq, tid - TS, 64, rms, mx);
+        if (tid == TS) {
             a.out[zi].stats4 = rms;
             a.out[zi].stats5 = mx;
         }

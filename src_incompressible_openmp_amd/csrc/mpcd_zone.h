@@ -2544,21 +2544,29 @@ ZHD void fin_place(Z& z, int tid, int nthr, int* queue) {
 }
 
 /* The stats of zone_correct from the per-cell |du|^2 in cell order (one
- * wavefront: each chunk of nl cells added lane by lane). */
+ * wavefront: each chunk of nl cells added lane by lane; the loads of four
+ * chunks are issued before their adds). */
 template <class Z>
 ZHD void zone_correct_stats(const Z& z, const double* dsq, int lane, int nl, double& rms, double& mxo) {
+    constexpr int NB = 4;
     double acc = 0.0, mx = 0.0;
     int nDU = 0;
-    for (int b0 = 0; b0 < z.C; b0 += nl) {
-        const int lc = b0 + lane;
-        const double d = lc < z.C ? dsq[lc] : 0.0;
-        mx = dmax(mx, MPCD_SQRT(d));
-        unsigned long long pos = wave_ballot(d > 0.0);
-        nDU += wave_popc(pos);
-        while (pos) {
-            const int l = __builtin_ctzll(pos);
-            pos &= pos - 1ull;
-            acc += wave_readlane_d(d, l);
+    for (int b0 = 0; b0 < z.C; b0 += NB * nl) {
+        double dv[NB];
+        for (int j = 0; j < NB; ++j) {
+            const int lc = b0 + j * nl + lane;
+            dv[j] = lc < z.C ? dsq[lc] : 0.0;
+        }
+        for (int j = 0; j < NB; ++j) {
+            const double d = dv[j];
+            mx = dmax(mx, MPCD_SQRT(d));
+            unsigned long long pos = wave_ballot(d > 0.0);
+            nDU += wave_popc(pos);
+            while (pos) {
+                const int l = __builtin_ctzll(pos);
+                pos &= pos - 1ull;
+                acc += wave_readlane_d(d, l);
+            }
         }
     }
     mxo = wave_max(mx);
