@@ -1114,12 +1114,22 @@ __global__ void __launch_bounds__(ZONE_TPB) k_zone(ZoneArgs a) {
  * order), then one wavefront for the ordered stats. */
 constexpr int FINISH_TPB = 256;
 constexpr int FINISH_LDS = 40 * 1024; /* cell arrays, column map and as many links as fit */
+/* The passes' zone, still in the workgroup's LDS (zone_carve's layout), for
+ * the correction in the same kernel: its particle count, ownership flag and
+ * the number of links the passes held in LDS. */
+struct ZoneHandoff {
+    int P, allOwned, nL;
+};
+
 /* The correction of zone zi by TPB threads (zones of more than CPT x TPB
- * cells take the two-sweep form); lds/ldsBytes: the staging space (kLds). */
+ * cells take the two-sweep form); lds/ldsBytes: the staging space (kLds).
+ * hand (kLds only): the passes left the zone in lds, so the cell arrays and
+ * column map are already in place and the links move down from behind the
+ * passes' scratch; otherwise the zone state comes from global memory. */
 template <class IT, bool kLds, int TPB, int CPT>
 __device__ __forceinline__ void zone_finish_body(const ZoneArgs& a, const ZoneIn& in, int zid, uint64_t seed, int zi,
                                                  long long off, int tid, unsigned char* lds, long long ldsBytes,
-                                                 int* queue) {
+                                                 int* queue, const ZoneHandoff* hand = nullptr) {
     ZoneCtxT<IT> z;
     zone_init(z, in, a.rects[zi], zid, seed, nullptr);
     z.nxG = (IT*)a.nextG + off;
@@ -1127,8 +1137,49 @@ __device__ __forceinline__ void zone_finish_body(const ZoneArgs& a, const ZoneIn
     z.duG = a.duG + (size_t)zi * (size_t)a.duStride;
     z.rec = a.rec + off;
     const ZoneState<IT> st = zone_state_at<IT>(a.zstate + (size_t)zi * a.zsStride, z.C, z.zw);
-    zone_attach_state(z, st);
-    if (kLds) {
+    if (kLds && hand) {
+        z.P = hand->P;
+        z.allOwned = hand->allOwned;
+    } else {
+        zone_attach_state(z, st);
+    }
+    if (kLds && hand) {
+        unsigned char* p = lds;
+        IT* cnt = (IT*)p; p += (size_t)z.C * sizeof(IT);
+        IT* head = (IT*)p; p += (size_t)z.C * sizeof(IT);
+        IT* tail = (IT*)p; p += (size_t)z.C * sizeof(IT);
+        IT* k0 = (IT*)p; p += (size_t)z.C * sizeof(IT);
+        unsigned short* fl = (unsigned short*)p; p += (size_t)z.C * sizeof(unsigned short);
+        p = lds + (((size_t)(p - lds) + 3) & ~(size_t)3);
+        int* colL = (int*)p; p += (size_t)z.zw * sizeof(int);
+        int* colG = (int*)p; p += (size_t)z.zw * sizeof(int);
+        IT* links = (IT*)p;
+        const long long room = (ldsBytes - (long long)(p - lds)) / (long long)sizeof(IT);
+        const int nL = (int)(room < (long long)z.P ? (room > 0 ? room : 0) : z.P);
+        /* the passes' links k < hand->nL, behind their scratch, move down in
+         * chunks no longer than the distance they move, so a chunk overwrites
+         * only links earlier chunks have copied; the rest come from the
+         * zone's global link region */
+        const IT* src = (const IT*)(lds + zone_fixed_bytes<IT>(z.C, z.zw, z.zp.useLocal));
+        const int gap = (int)(src - links);
+        const int nP = hand->nL < nL ? hand->nL : nL;
+        for (int c0 = 0; c0 < nP; c0 += gap) {
+            const int c1 = c0 + gap < nP ? c0 + gap : nP;
+            for (int k = c0 + tid; k < c1; k += TPB) links[k] = src[k];
+            __syncthreads();
+        }
+        for (int k = nP + tid; k < nL; k += TPB) links[k] = z.nxG[k];
+        z.cnt = cnt;
+        z.head = head;
+        z.tail = tail;
+        z.k0 = k0;
+        z.fl = fl;
+        z.colL = colL;
+        z.colG = colG;
+        z.nxL = links;
+        z.nL = nL;
+        __syncthreads();
+    } else if (kLds) {
         /* the state and the first links to LDS */
         unsigned char* p = lds;
         IT* cnt = (IT*)p; p += (size_t)z.C * sizeof(IT);
@@ -1312,22 +1363,22 @@ __global__ void __launch_bounds__(FUSED_TPB, 4) k_zone_fused(ZoneArgs a) { /* 4 
     if (threadIdx.x == 0 && sh.schedWait) atomicOr(a.err, 16);
     zone_sparse_pass_mw(z, wave, lane);
     if (threadIdx.x == 0 && sh.schedWait) atomicOr(a.err, 16);
-    if (wave == 0) {
-        zone_dump(z, zone_state_at<IT>(a.zstate + (size_t)zi * a.zsStride, z.C, z.zw), lane, ZONE_TPB);
-        if (lane == 0) {
-            ZoneOut o;
-            o.nHigh = sh.nHigh;
-            o.nLow = sh.nLow;
-            o.movedDense = sh.movedDense;
-            o.movedSparse = sh.movedSparse;
-            o.stats4 = 0.0;
-            o.stats5 = 0.0;
-            a.out[zi] = o;
-        }
+    if (threadIdx.x == 0) {
+        ZoneOut o;
+        o.nHigh = sh.nHigh;
+        o.nLow = sh.nLow;
+        o.movedDense = sh.movedDense;
+        o.movedSparse = sh.movedSparse;
+        o.stats4 = 0.0;
+        o.stats5 = 0.0;
+        a.out[zi] = o;
     }
     __builtin_amdgcn_s_setprio(0);
+    /* the correction takes the zone over where the passes left it in LDS */
+    const ZoneHandoff hand{z.P, z.allOwned, z.nL};
     __syncthreads();
-    zone_finish_body<IT, true, FUSED_TPB, 3>(a, in, zid, seed, zi, off, (int)threadIdx.x, lds, a.ldsBytes, queue);
+    zone_finish_body<IT, true, FUSED_TPB, 3>(a, in, zid, seed, zi, off, (int)threadIdx.x, lds, a.ldsBytes, queue,
+                                            &hand);
 }
 
 /* velocities of the particles of cells several zones cover, from the side

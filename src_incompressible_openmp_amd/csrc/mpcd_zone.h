@@ -2123,6 +2123,7 @@ ZHD size_t zone_fixed_bytes(int C, int W, int useLocal) {
     b += (size_t)C * sizeof(unsigned short); /* fl */
     b += (size_t)C * sizeof(IT);             /* lix */
     b += (size_t)ZONE_TMP_L * sizeof(IT);                       /* tmpL */
+    b += 4 + 16;                                                /* alignment after fl and after colG */
     return (b + 15) & ~(size_t)15;
 }
 
@@ -2132,6 +2133,19 @@ template <class IT>
 ZHD void zone_carve(ZoneCtxT<IT>& z, unsigned char* base, size_t bytes) {
     const int C = z.C, W = z.zw;
     unsigned char* p = base;
+    /* first what the correction keeps (cell arrays, column map), at the
+     * offsets of the correction's own LDS layout, so the correction that
+     * follows the passes in one kernel finds them in place */
+    z.cnt = (IT*)p; p += (size_t)C * sizeof(IT);
+    z.head = (IT*)p; p += (size_t)C * sizeof(IT);
+    z.tail = (IT*)p; p += (size_t)C * sizeof(IT);
+    z.k0 = (IT*)p; p += (size_t)C * sizeof(IT);
+    z.fl = (unsigned short*)p; p += (size_t)C * sizeof(unsigned short);
+    p = base + (((size_t)(p - base) + 3) & ~(size_t)3);
+    z.colL = (int*)p; p += (size_t)W * sizeof(int);
+    z.colG = (int*)p; p += (size_t)W * sizeof(int);
+    p = base + (((size_t)(p - base) + 15) & ~(size_t)15);
+    /* then the passes' scratch */
     if (z.zp.useLocal) {
         z.lowA = (double*)p; p += (size_t)C * sizeof(double);
         z.highA = (double*)p; p += (size_t)C * sizeof(double);
@@ -2145,24 +2159,17 @@ ZHD void zone_carve(ZoneCtxT<IT>& z, unsigned char* base, size_t bytes) {
     z.dsc = (double*)p; p += 8 * sizeof(double);
     z.nsm4 = (double*)p; p += 4 * sizeof(double);
     z.mvk = (int*)p; p += ZONE_MAX_MOVE * sizeof(int);
-    z.colL = (int*)p; p += (size_t)W * sizeof(int);
-    z.colG = (int*)p; p += (size_t)W * sizeof(int);
     z.ord = (int*)p; p += 24 * sizeof(int);
     z.nbGc = (int*)p; p += 24 * sizeof(int);
     z.rnk = (int*)p; p += 24 * sizeof(int);
     z.recvT = (int*)p; p += 48 * sizeof(int);
     z.dord = (int*)p; p += 8 * sizeof(int);
     z.dord2 = (int*)p; p += 16 * sizeof(int);
-    z.cnt = (IT*)p; p += (size_t)C * sizeof(IT);
-    z.head = (IT*)p; p += (size_t)C * sizeof(IT);
-    z.tail = (IT*)p; p += (size_t)C * sizeof(IT);
     z.dl = (IT*)p; p += (size_t)C * sizeof(IT);
     z.ext = (IT*)p; p += (size_t)(z.zw + 4) * (size_t)(z.zh + 4) * sizeof(IT);
-    z.k0 = (IT*)p; p += (size_t)C * sizeof(IT);
     z.halo = (IT*)p; p += (size_t)halo_size(z) * sizeof(IT);
     z.tmpL = (IT*)p; p += (size_t)ZONE_TMP_L * sizeof(IT);
     z.tmpCap = ZONE_TMP_L;
-    z.fl = (unsigned short*)p; p += (size_t)C * sizeof(unsigned short);
     z.lix = (IT*)p; p += (size_t)C * sizeof(IT);
     const size_t fixed = zone_fixed_bytes<IT>(C, W, z.zp.useLocal);
     p = base + fixed;
