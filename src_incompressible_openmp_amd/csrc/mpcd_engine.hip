@@ -1349,13 +1349,11 @@ __global__ void __launch_bounds__(FUSED_TPB, 4) k_zone_fused(ZoneArgs a) { /* 4 
     z.duG = a.duG + (size_t)zi * (size_t)a.duStride;
     z.rec = a.rec + off;
     zone_carve(z, lds, a.ldsBytes);
-    if (wave == 0) {
-        zone_setup(z, in, lane, ZONE_TPB);
-        ZBAR();
-    }
+    static_assert(FUSED_TPB == 256, "zone_setup_mw and the phases take four wavefronts");
+    zone_setup_mw(z, in, wave, lane);
     __syncthreads();
     z.P = sh.P;
-    z.allOwned = sh.allOwned;
+    z.allOwned = zone_setup_all_owned(z);
     for (int k = (int)threadIdx.x; k < z.P; k += FUSED_TPB) z.rec[k] = (unsigned char)ZM_NONE;
     __syncthreads();
     /* the dense passes and the sparse phase on all four wavefronts */

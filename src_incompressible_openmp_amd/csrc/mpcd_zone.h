@@ -220,6 +220,7 @@ struct ZoneShared {
     int mwNext;            /* and the next list position a wavefront takes */
     int schedWait;         /* nonzero: a wait of the dependency-driven dense phase hit its bound */
     int nHigh, nLow, movedDense, movedSparse;
+    int mixedW[3];         /* zone_setup_mw: wavefront 1..3 saw a cell not plainly owned */
     double stats4, stats5;
 };
 
@@ -1845,6 +1846,44 @@ ZHD void zone_sparse_pass_mw(Z&, int, int) {}
 
 /* ---- set-up, correction, statistics ---- */
 
+/* zone_setup's flags of the cell with global index gc */
+template <class Z>
+ZHD unsigned short setup_flags(const Z& z, const ZoneIn& in, int gc) {
+    unsigned short f = 0;
+    if (!in.owner || in.owner[gc] == z.zid) f |= ZF_OWN;
+    if (in.gcover && in.gcover[gc]) f |= ZF_COVER;
+    if (in.zmask && !in.zmask[gc]) f |= ZF_OUT;
+    return f;
+}
+
+/* zone_setup's entry h of the ring of pass-input counts around the zone */
+template <class Z>
+ZHD void setup_halo(Z& z, int h) {
+    const int W4 = z.zw + 4;
+    int hx, hy;
+    if (h < 2 * W4) {
+        hx = h / 2;
+        hy = h % 2;
+    } else if (h < 4 * W4) {
+        hx = (h - 2 * W4) / 2;
+        hy = z.zh + 2 + (h - 2 * W4) % 2;
+    } else if (h < 4 * W4 + 2 * z.zh) {
+        hx = (h - 4 * W4) / z.zh;
+        hy = 2 + (h - 4 * W4) % z.zh;
+    } else {
+        hx = z.zw + 2 + (h - 4 * W4 - 2 * z.zh) / z.zh;
+        hy = 2 + (h - 4 * W4 - 2 * z.zh) % z.zh;
+    }
+    int gx = z.xs + hx - 2, gy = z.ys + hy - 2;
+    if (z.g.perx) gx = wrapi(gx, z.g.Nx);
+    if (z.g.pery) gy = wrapi(gy, z.g.Ny);
+    int v = 0;
+    if (gx >= 0 && gx < z.g.Nx && gy >= 0 && gy < z.g.Ny) v = z.gcnt[gy + z.g.Ny * gx];
+    /* the unsigned 16-bit type keeps its top value for "read the global count" */
+    if (sizeof(typename Z::idx_t) == 2 && v >= 0xFFFF) v = -1;
+    z.halo[h] = (typename Z::idx_t)v;
+}
+
 /* Zone setup (zone_pass.cpp:680-685), all lanes: counts, implicit list of
  * each cell (its run of local indices, from a wave prefix sum over the
  * zone's cells), the column map and the ownership flags. */
@@ -1863,11 +1902,8 @@ ZHD void zone_setup(Z& z, const ZoneIn& in, int lane, int nl) {
             z.k0[lc] = (typename Z::idx_t)k0;
             head_set(z, lc, m > 0 ? k0 : -1);
             tail_set(z, lc, m > 0 ? k0 + m - 1 : -1);
-            unsigned short f = 0;
-            if (!in.owner || in.owner[gc] == z.zid) f |= ZF_OWN;
-            if (in.gcover && in.gcover[gc]) f |= ZF_COVER;
-            if (f != ZF_OWN) mixed = true;
-            if (in.zmask && !in.zmask[gc]) f |= ZF_OUT;
+            const unsigned short f = setup_flags(z, in, gc);
+            if ((f & (ZF_OWN | ZF_COVER)) != ZF_OWN) mixed = true;
             z.fl[lc] = f;
             if (lc % z.zh == 0) {
                 z.colL[lc / z.zh] = k0;
@@ -1880,31 +1916,7 @@ ZHD void zone_setup(Z& z, const ZoneIn& in, int lane, int nl) {
     const bool anyMixed = wave_ballot(mixed) != 0ull;
     /* the ring of pass-input counts around the zone */
     const int nh = halo_size(z);
-    for (int h = lane; h < nh; h += nl) {
-        const int W4 = z.zw + 4;
-        int hx, hy;
-        if (h < 2 * W4) {
-            hx = h / 2;
-            hy = h % 2;
-        } else if (h < 4 * W4) {
-            hx = (h - 2 * W4) / 2;
-            hy = z.zh + 2 + (h - 2 * W4) % 2;
-        } else if (h < 4 * W4 + 2 * z.zh) {
-            hx = (h - 4 * W4) / z.zh;
-            hy = 2 + (h - 4 * W4) % z.zh;
-        } else {
-            hx = z.zw + 2 + (h - 4 * W4 - 2 * z.zh) / z.zh;
-            hy = 2 + (h - 4 * W4 - 2 * z.zh) % z.zh;
-        }
-        int gx = z.xs + hx - 2, gy = z.ys + hy - 2;
-        if (z.g.perx) gx = wrapi(gx, z.g.Nx);
-        if (z.g.pery) gy = wrapi(gy, z.g.Ny);
-        int v = 0;
-        if (gx >= 0 && gx < z.g.Nx && gy >= 0 && gy < z.g.Ny) v = z.gcnt[gy + z.g.Ny * gx];
-        /* the unsigned 16-bit type keeps its top value for "read the global count" */
-        if (sizeof(typename Z::idx_t) == 2 && v >= 0xFFFF) v = -1;
-        z.halo[h] = (typename Z::idx_t)v;
-    }
+    for (int h = lane; h < nh; h += nl) setup_halo(z, h);
     if (lane == 0) {
         z.sh->P = base;
         z.sh->allOwned = anyMixed ? 0 : 1;
@@ -1915,6 +1927,60 @@ ZHD void zone_setup(Z& z, const ZoneIn& in, int lane, int nl) {
         z.sh->stats4 = 0.0;
         z.sh->stats5 = 0.0;
     }
+}
+
+/* zone_setup on the four wavefronts of a 256-thread workgroup: wavefront 0
+ * takes the counts, runs and column starts (its prefix sum over the cells),
+ * wavefronts 1..3 the flags, the column map's global starts and the halo
+ * ring; each of them records in sh->mixedW whether it saw a cell that is not
+ * plainly owned.  After a workgroup barrier, zone_setup_all_owned gives the
+ * zone's allOwned. */
+template <class Z>
+ZHD void zone_setup_mw(Z& z, const ZoneIn& in, int wave, int lane) {
+    if (wave == 0) {
+        int base = 0;
+        for (int b0 = 0; b0 < z.C; b0 += 64) {
+            const int lc = b0 + lane;
+            const int m = lc < z.C ? z.gcnt[lc_gc(z, lc)] : 0;
+            const int incl = wave_shfl_up_add(m, lane);
+            if (lc < z.C) {
+                const int k0 = base + incl - m;
+                cnt_set(z, lc, m);
+                z.k0[lc] = (typename Z::idx_t)k0;
+                head_set(z, lc, m > 0 ? k0 : -1);
+                tail_set(z, lc, m > 0 ? k0 + m - 1 : -1);
+                if (lc % z.zh == 0) z.colL[lc / z.zh] = k0;
+            }
+            base += wave_bcast(incl, 63);
+        }
+        if (lane == 0) {
+            z.sh->P = base;
+            z.sh->nHigh = 0;
+            z.sh->nLow = 0;
+            z.sh->movedDense = 0;
+            z.sh->movedSparse = 0;
+            z.sh->stats4 = 0.0;
+            z.sh->stats5 = 0.0;
+        }
+        return;
+    }
+    const int t = (wave - 1) * 64 + lane;
+    bool mixed = false;
+    for (int lc = t; lc < z.C; lc += 192) {
+        const int gc = lc_gc(z, lc);
+        const unsigned short f = setup_flags(z, in, gc);
+        if ((f & (ZF_OWN | ZF_COVER)) != ZF_OWN) mixed = true;
+        z.fl[lc] = f;
+        if (lc % z.zh == 0) z.colG[lc / z.zh] = z.gstart[gc];
+    }
+    const int nh = halo_size(z);
+    for (int h = t; h < nh; h += 192) setup_halo(z, h);
+    const bool anyMixed = wave_ballot(mixed) != 0ull;
+    if (lane == 0) z.sh->mixedW[wave - 1] = anyMixed ? 1 : 0;
+}
+template <class Z>
+ZHD int zone_setup_all_owned(const Z& z) {
+    return (z.sh->mixedW[0] | z.sh->mixedW[1] | z.sh->mixedW[2]) ? 0 : 1;
 }
 
 /* sum over the slots glist[s0 .. s0+n) in order, loads issued 8 at a time */
