@@ -1304,9 +1304,10 @@ __global__ void __launch_bounds__(FINISH_TPB) k_zone_finish(ZoneArgs a) {
 
 /* The passes and the correction of a zone in one workgroup of FUSED_TPB
  * threads (four wavefronts of at most 128 registers, so four zones share a
- * CU): wavefront 0 runs the passes while the others wait at the barrier,
- * then all run the correction.  A zone's correction starts as
- * soon as its own passes end, beside the passes of slower zones. */
+ * CU): all four wavefronts run the set-up, the scheduled dense and sparse
+ * phases, then the correction, which takes the zone over in LDS where the
+ * passes left it.  A zone's correction starts as soon as its own passes
+ * end, beside the passes of slower zones. */
 constexpr int FUSED_TPB = 256;
 template <class IT, bool kEns = false>
 __global__ void __launch_bounds__(FUSED_TPB, 4) k_zone_fused(ZoneArgs a) { /* 4 waves per SIMD: four zones per CU */
